@@ -1,0 +1,184 @@
+"""Benchmark: N = 4,000,000 2D Gaussian-RBF solve to rtol 1e-10 on one MI355X.
+
+One step = the whole solve path with the inputs already in HBM: cell list and
+sort, buffer / truncation lists, RASM setup (blocks, factorisation, core rows)
+and GMRES(30) with left RASM to rtol 1e-10, from particles and RHS to gamma.
+Workload: 2000 x 2000 jittered lattice on [-1,1]^2, sigma = h/0.8, 4x4-spacing
+cells, two-cell buffer, truncation 5 sigma beyond the cell, Lamb-Oseen RHS.
+
+Prints one JSON line (see the keys below).  --gpus > 1 is refused: this build
+measures one GPU.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
+PEAK_HBM_GBS = 8000.0     # MI355X HBM3E
+FLOP_PER_PAIR_2D = 15     # 2d/matmult.cxx:44-45 (exp counted as one flop)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(p, its, budget_s=30.0):
+    """Host-core oracle (the reference's CPU algorithm: LU blocks, GMRES(30))
+    timed on a bounded sample of the same N = 4M workload and scaled to the
+    full solve: lists at full size, one full matvec, the block LU of every
+    k-th cluster and its apply, then t = lists + factor/frac + its*(matvec + apply/frac)."""
+    from oracle.oracle import Oracle, num_threads
+    t0 = time.perf_counter()
+    o = Oracle.from_problem(p)
+    t_lists = time.perf_counter() - t0
+    pi, _ = o.perm()
+    g = p.w[0][pi].copy()
+    t0 = time.perf_counter()
+    o.matvec(g)
+    t_mv = time.perf_counter() - t0
+    stride = 200
+    t0 = time.perf_counter()
+    o.rasm_setup_sample(stride)
+    t_fac = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.rasm_apply_sample(g)
+    t_ap = time.perf_counter() - t0
+    ncell = o.ncell
+    frac = (ncell // stride + 1) / ncell
+    est = t_lists + t_fac / frac + its * (t_mv + t_ap / frac)
+    return {
+        "value": est, "unit": "s", "cores": num_threads(), "kind": "port",
+        "sample": (f"N=4M lattice: lists+sort full ({t_lists:.2f}s), one full matvec ({t_mv:.2f}s), "
+                   f"block LU of every {stride}th cluster ({t_fac:.2f}s) and its apply ({t_ap:.3f}s), "
+                   f"scaled to all {ncell} clusters and {its} GMRES iterations"),
+        "t_lists_s": t_lists, "t_matvec_s": t_mv,
+        "pair_evals_per_s": None,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n-side", type=int, default=2000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel-reps", type=int, default=10)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1"}))
+        sys.exit(2)
+
+    import torch
+    from petrbf_amd import problems
+    from petrbf_amd.rbf import Context
+
+    dev = torch.device("cuda:0")
+    log(f"generating lattice {args.n_side}^2")
+    p = problems.lattice_2d_problem(args.n_side, seed=0x5EED0001)
+    x = torch.from_numpy(p.x).to(dev)
+    y = torch.from_numpy(p.y).to(dev)
+    w = torch.from_numpy(p.w[0]).to(dev)
+    gamma = torch.empty_like(w)
+    stream = torch.cuda.Stream(device=dev)
+    ctx = Context.from_problem(p)
+    ctx.set_stream(stream.cuda_stream)
+
+    def step():
+        ctx.set_points_dev(x, y)
+        return ctx.solve_dev(w, gamma, ksp="gmres", pc="rasm", rtol=1e-10, restart=30, max_it=1000)
+
+    for i in range(args.warmup):
+        inf = step()
+        log(f"warmup {i}: its {inf.its} reason {inf.reason} setup {inf.t_setup_s:.3f}s solve {inf.t_solve_s:.3f}s")
+    torch.cuda.synchronize()
+    infos = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        infos.append(step())
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    inf = infos[-1]
+    log(f"timed {args.steps} steps: {elapsed:.3f}s; its {inf.its} reason {inf.reason}")
+
+    # ---- per-kernel timing with HIP events on the library's stream
+    info = ctx.info()
+    ista, iend, _, _ = ctx.cells()
+    T = (iend - ista + 1).astype(np.int64)
+    tptr = ctx.trunc_ptr()
+    bptr = ctx.buffer_ptr()
+    S = np.diff(tptr)
+    nb = np.diff(bptr)
+    pairs = int((T * S).sum())
+    g_s = torch.randn(p.n, dtype=torch.float64, device=dev)
+    out = torch.empty_like(g_s)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(args.kernel_reps):
+        ctx.matvec_dev(g_s, out)
+    e1.record(stream)
+    e1.synchronize()
+    t_mv = e0.elapsed_time(e1) / 1e3 / args.kernel_reps
+    e0.record(stream)
+    for _ in range(args.kernel_reps):
+        ctx.rasm_apply_dev(g_s, out)
+    e1.record(stream)
+    e1.synchronize()
+    t_ap = e0.elapsed_time(e1) / 1e3 / args.kernel_reps
+    m_bytes = int((nb * T).sum()) * 8
+    apply_bytes = m_bytes + int(nb.sum()) * 4 + p.n * 8 * 2 + (info["ncell"] + 1) * 8 * 2
+    mv_flops = FLOP_PER_PAIR_2D * pairs
+    log(f"matvec {t_mv*1e3:.3f} ms ({pairs/t_mv:.3e} pairs/s, {mv_flops/t_mv/1e12:.2f} TFLOP/s); "
+        f"rasm apply {t_ap*1e3:.3f} ms ({apply_bytes/t_ap/1e9:.0f} GB/s)")
+
+    ms_per_step = elapsed / args.steps * 1e3
+    res = {
+        "metric": "N=4M 2D solve time to rtol 1e-10 (learned/FP32 vs plain); Gaussian pair-evals/s",
+        "value": ms_per_step / 1e3,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"2D jittered lattice {args.n_side}x{args.n_side} (N={p.n}), plain FP64 "
+                               "GMRES(30) + RASM (4x4-spacing cells, 2-cell buffer, 5 sigma truncation), rtol 1e-10",
+                   "mode": "plain", "n": p.n, "ncell": info["ncell"], "its": inf.its, "reason": inf.reason},
+        "solve": {"its": inf.its, "t_setup_s": inf.t_setup_s, "t_solve_s": inf.t_solve_s,
+                  "t_matvec_s": inf.t_matvec_s, "n_matvec": inf.n_matvec, "t_pc_s": inf.t_pc_s,
+                  "n_pc": inf.n_pc},
+        "pair_evals_per_s": pairs / t_mv,
+        "pairs_per_matvec": pairs,
+        "roofline": {"bound": "hbm", "kernel": "k_rasm_apply_fp64", "achieved": apply_bytes / t_ap / 1e9,
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": apply_bytes / t_ap / 1e9 / PEAK_HBM_GBS,
+                     "traffic": None, "avg_ms": t_ap * 1e3, "bytes_per_launch": apply_bytes},
+        "roofline_matvec": {"bound": "fp64-valu", "kernel": "k_matvec_wave", "achieved": mv_flops / t_mv / 1e12,
+                            "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                            "frac": mv_flops / t_mv / 1e12 / PEAK_FP64_TFLOPS, "avg_ms": t_mv * 1e3,
+                            "flops_per_launch": mv_flops},
+    }
+    if not args.no_cpu:
+        log("cpu baseline (oracle, bounded sample)")
+        res["cpu_baseline"] = cpu_baseline(p, inf.its)
+        log(f"cpu baseline est {res['cpu_baseline']['value']:.1f}s on {res['cpu_baseline']['cores']} threads")
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,162 @@
+/*
+ * petrbf.h -- C ABI of the MI355X (gfx950) Gaussian-RBF interpolation library.
+ *
+ * Drop-in boundary for barbagroup/petrbf's solve path.  Each entry point names
+ * the reference interface it replaces (paths relative to the reference tree).
+ * Plain pointers and sizes only; no PETSc, MPI or torch types.
+ *
+ * Conventions
+ *   - Every function returns a prbf_status (0 = PRBF_OK).  prbf_last_error()
+ *     gives the message of the last failure on that context.
+ *   - "sorted" arrays are in cluster order (particles counting-sorted by cell,
+ *     2d/get_cluster.h:77-138), the order of the reference's MatShell vectors;
+ *     "original" arrays are in the caller's order.
+ *   - Host-pointer functions copy in, run on the context's stream and
+ *     synchronise.  *_dev functions take device pointers, are ordered on the
+ *     context's stream and return without synchronising.
+ *   - The caller owns all memory it passes; the context owns its device
+ *     memory.  One context per host thread (thread-compatible, not
+ *     thread-safe), like the reference's BOTH context (2d/par.h:148-151).
+ */
+#ifndef PETRBF_H
+#define PETRBF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct prbf_ctx prbf_ctx;
+
+typedef enum {
+  PRBF_OK = 0,
+  PRBF_ERR_ARG = 1,      /* invalid argument or size */
+  PRBF_ERR_HIP = 2,      /* HIP runtime error (device fault, launch failure) */
+  PRBF_ERR_NOMEM = 3,    /* device allocation failed */
+  PRBF_ERR_STATE = 4,    /* call out of order (e.g. solve before set_points) */
+  PRBF_ERR_FACTOR = 5,   /* a RASM block was not positive definite */
+  PRBF_ERR_OUTSIDE = 6,  /* a point fell outside the cell grid */
+  PRBF_ERR_LIMIT = 7     /* a block exceeds the kernel's supported size */
+} prbf_status;
+
+/* Cluster geometry.  The reference takes nsigma_box, sigma_buffer and
+ * sigma_trunc as ints in units of sigma (2d/par.h:72-74,
+ * 2d/rbf_interpolation.cxx:25-26); they are reals here with the same meaning:
+ * cell side = nsigma_box*sigma + 1e-6 (2d/get_cluster.h:29), buffer box
+ * half-width = sigma_buffer*sigma/2 + 1e-6 (2d/get_buffer.h:11), truncation
+ * box half-width = sigma_trunc*sigma/2 + 1e-6 (2d/get_trunc.h:14). */
+typedef struct {
+  int dim;              /* 2 or 3 */
+  double sigma;
+  double nsigma_box;
+  double sigma_buffer;
+  double sigma_trunc;
+} prbf_params;
+
+typedef enum { PRBF_KSP_GMRES = 0, PRBF_KSP_FGMRES = 1 } prbf_ksp;
+typedef enum { PRBF_PC_NONE = 0, PRBF_PC_RASM_FP64 = 1 } prbf_pc;
+
+/* Solver options; prbf_solve_opts_default() gives the reference's run line
+ * (-ksp_type gmres default restart 30, -pc_type asm -sub_pc_type lu,
+ * 2d/Makefile:7) with rtol 1e-10 and max_it 1000. */
+typedef struct {
+  int ksp;          /* prbf_ksp: GMRES = left preconditioning (PETSc default) */
+  int pc;           /* prbf_pc */
+  int restart;      /* Krylov subspace size per cycle (30) */
+  int max_it;       /* iteration cap (1000) */
+  double rtol;      /* relative tolerance on the solver's residual norm */
+  double abstol;    /* 1e-50 */
+  double dtol;      /* 1e5 */
+} prbf_solve_opts;
+
+typedef struct {
+  int its;           /* iterations (KSPGetIterationNumber, 2d/rbf_interpolation.cxx:234) */
+  int reason;        /* PETSc KSPConvergedReason numbering: 2 rtol, 3 atol, <0 diverged */
+  double rnorm0;     /* initial residual norm of the convergence test */
+  double rnorm;      /* final residual norm of the convergence test */
+  double t_setup_s;  /* RASM setup wall time (0 if it was already set up) */
+  double t_solve_s;  /* Krylov iterations wall time */
+  double t_matvec_s; /* summed device time of the matvec kernels */
+  double t_pc_s;     /* summed device time of the preconditioner applies */
+  int n_matvec;
+  int n_pc;
+} prbf_solve_info;
+
+/* ---- lifetime -------------------------------------------------------- */
+void prbf_solve_opts_default(prbf_solve_opts* o);
+prbf_status prbf_create(prbf_ctx** ctx, const prbf_params* p);
+prbf_status prbf_destroy(prbf_ctx* ctx);
+const char* prbf_last_error(const prbf_ctx* ctx);
+/* Use an existing HIP stream (hipStream_t) for all work of this context. */
+prbf_status prbf_set_stream(prbf_ctx* ctx, void* stream);
+prbf_status prbf_synchronize(prbf_ctx* ctx);
+
+/* ---- kernel (a): cell list, sort, buffer and truncation lists --------
+ * Replaces Get_cluster::get_cluster (2d/get_cluster.h:15-276) and the list
+ * builders Get_buffer::get_buffer (2d/get_buffer.h:9-90) and
+ * Get_trunc::get_trunc (2d/get_trunc.h:9-87), built once on the device.
+ * Targets (i-set) and sources (j-set); xj == NULL makes the source set the
+ * target set (the solve, 2d/rbf_interpolation.cxx:195-201).  The grid spans
+ * the union bounding box (2d/vorticity_evaluation.cxx:59-70). */
+prbf_status prbf_set_points(prbf_ctx* ctx, int64_t ni, const double* xi, const double* yi, const double* zi,
+                            int64_t nj, const double* xj, const double* yj, const double* zj);
+prbf_status prbf_set_points_dev(prbf_ctx* ctx, int64_t ni, const double* xi, const double* yi, const double* zi,
+                                int64_t nj, const double* xj, const double* yj, const double* zj);
+
+/* info[0..15]: ncell, nx, ny, nz, nb_buffer, nb_trunc, buffer_nnz, trunc_nnz,
+ * max_buffer, max_core, max_trunc, ni, nj, 0, 0, 0.
+ * geom[0..5]: xmin, ymin, zmin, box, buffer half-width, trunc half-width. */
+prbf_status prbf_get_info(prbf_ctx* ctx, int64_t* info, double* geom);
+/* Sorted position -> original index (the reference's particle->ii / jj). */
+prbf_status prbf_get_permutation(prbf_ctx* ctx, int32_t* perm_i, int32_t* perm_j);
+/* Per-cell ranges in sorted order (cluster->ista/iend/jsta/jend). */
+prbf_status prbf_get_cells(prbf_ctx* ctx, int32_t* ista, int32_t* iend, int32_t* jsta, int32_t* jend);
+/* Buffer lists (cluster->idx, the PCASM `is`): CSR over cells, sorted i-set ids. */
+prbf_status prbf_get_buffer_csr(prbf_ctx* ctx, int64_t* ptr, int32_t* idx);
+/* Truncation lists (the sources get_trunc gathers): CSR over cells, sorted j-set ids. */
+prbf_status prbf_get_trunc_csr(prbf_ctx* ctx, int64_t* ptr, int32_t* idx);
+/* Sorted coordinates of the i-set (z may be NULL in 2D). */
+prbf_status prbf_get_sorted_points(prbf_ctx* ctx, double* x, double* y, double* z);
+
+/* ---- kernel (b): truncated Gaussian matvec ---------------------------
+ * MatShell MULT callback mymatmult (2d/matmult.cxx:6-51) and the GPU seam
+ * gpumatmult (2d/gpumatmult.cu:65-67): w = A~ g in sorted order, FP64.
+ * nrhs vectors stored back to back (lengths nj in, ni out). */
+prbf_status prbf_matvec(prbf_ctx* ctx, int nrhs, const double* g_sorted, double* w_sorted);
+prbf_status prbf_matvec_dev(prbf_ctx* ctx, int nrhs, const double* g_sorted, double* w_sorted);
+
+/* ---- kernel (c): evaluation at targets -------------------------------
+ * vorticity_evaluation (2d/vorticity_evaluation.cxx:26-299 with
+ * Get_vorticity, 2d/get_vorticity.h:9-27): w_i = sum_trunc g_j G(x_i - x_j),
+ * both in the caller's original order. */
+prbf_status prbf_evaluate(prbf_ctx* ctx, int nrhs, const double* gj, double* wi);
+prbf_status prbf_evaluate_dev(prbf_ctx* ctx, int nrhs, const double* gj, double* wi);
+
+/* ---- RASM preconditioner ----------------------------------------------
+ * MatShell GET_SUBMATRICES callback mysubmat (2d/matmult.cxx:53-104) + the
+ * PETSc dense sub-factorisation, for the subdomains set up by
+ * PCASMSetLocalSubdomains (2d/rbf_interpolation.cxx:213-229). */
+prbf_status prbf_rasm_setup(prbf_ctx* ctx);
+/* PCApply_ASM (restrict): z = sum_c P_core A_c^{-1} R_buffer r, sorted order. */
+prbf_status prbf_rasm_apply(prbf_ctx* ctx, const double* r_sorted, double* z_sorted);
+prbf_status prbf_rasm_apply_dev(prbf_ctx* ctx, const double* r_sorted, double* z_sorted);
+/* Core rows of A_c^{-1} for cluster c: M[b*T + t], b < n buffer slots, t < T
+ * core members; n and T returned.  Pass M == NULL to query the sizes. */
+prbf_status prbf_rasm_get_core_rows(prbf_ctx* ctx, int64_t c, double* M, int32_t* n, int32_t* T);
+
+/* ---- solve -------------------------------------------------------------
+ * rbf_interpolation (2d/rbf_interpolation.cxx:25-310): finds gamma with
+ * A~ gamma = w, KSPSolve with zero initial guess (:183-185,232).  w and
+ * gamma in original order.  hist (optional) receives the residual history
+ * of the convergence test (up to hist_cap entries). */
+prbf_status prbf_solve(prbf_ctx* ctx, const prbf_solve_opts* opts, const double* w, double* gamma,
+                       prbf_solve_info* info, double* hist, int hist_cap);
+prbf_status prbf_solve_dev(prbf_ctx* ctx, const prbf_solve_opts* opts, const double* w, double* gamma,
+                           prbf_solve_info* info, double* hist, int hist_cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PETRBF_H */

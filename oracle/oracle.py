@@ -54,6 +54,10 @@ def lib():
     L.orc_matvec.argtypes = [vp, i32, _dp, _dp]
     L.orc_rasm_setup.restype = i32
     L.orc_rasm_setup.argtypes = [vp]
+    L.orc_rasm_setup_sample.restype = i32
+    L.orc_rasm_setup_sample.argtypes = [vp, ll]
+    L.orc_rasm_apply_sample.restype = i32
+    L.orc_rasm_apply_sample.argtypes = [vp, _dp, _dp]
     L.orc_rasm_apply.restype = i32
     L.orc_rasm_apply.argtypes = [vp, _dp, _dp]
     L.orc_get_block.restype = i32
@@ -167,6 +171,16 @@ class Oracle:
         st = lib().orc_rasm_setup(self.h)
         if st != 0:
             raise RuntimeError(f"orc_rasm_setup failed ({st})")
+
+    def rasm_setup_sample(self, stride):
+        st = lib().orc_rasm_setup_sample(self.h, int(stride))
+        if st != 0:
+            raise RuntimeError(f"orc_rasm_setup_sample failed ({st})")
+
+    def rasm_apply_sample(self, r):
+        z = np.zeros(self.ni)
+        lib().orc_rasm_apply_sample(self.h, np.ascontiguousarray(r, dtype=np.float64), z)
+        return z
 
     def rasm_apply(self, r):
         z = np.zeros(self.ni)

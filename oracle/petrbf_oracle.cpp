@@ -59,6 +59,7 @@ struct Oracle {
   std::vector<double> fac;
   std::vector<int> piv;          // offset bptr[c]
   bool rasm_ready = false;
+  long long rasm_stride = 1;
 };
 
 inline long long cell_id(const Oracle& o, int ix, int iy, int iz) {
@@ -306,10 +307,11 @@ void block_of(const Oracle& o, long long c, double* A) {
   }
 }
 
-int rasm_setup(Oracle& o) {
+// stride > 1 factorises only clusters c with c % stride == 0 (a timing sample).
+int rasm_setup(Oracle& o, long long stride = 1) {
   o.fptr.assign(o.ncell + 1, 0);
   for (long long c = 0; c < o.ncell; ++c) {
-    long long n = o.bptr[c + 1] - o.bptr[c];
+    long long n = (c % stride == 0) ? o.bptr[c + 1] - o.bptr[c] : 0;
     o.fptr[c + 1] = o.fptr[c] + n * n;
   }
   o.fac.assign(o.fptr[o.ncell], 0.0);
@@ -318,18 +320,20 @@ int rasm_setup(Oracle& o) {
 #pragma omp parallel for schedule(dynamic, 16) reduction(+ : bad)
   for (long long c = 0; c < o.ncell; ++c) {
     int n = (int)(o.bptr[c + 1] - o.bptr[c]);
-    if (n == 0) continue;
+    if (n == 0 || c % stride != 0) continue;
     double* A = o.fac.data() + o.fptr[c];
     block_of(o, c, A);  // symmetric: row-major == column-major
     if (!lu_factor(n, A, o.piv.data() + o.bptr[c])) bad++;
   }
-  o.rasm_ready = (bad == 0);
+  o.rasm_ready = (bad == 0) && stride == 1;
+  o.rasm_stride = stride;
   return bad == 0 ? 0 : 5;
 }
 
 // PCASM, restrict variant: z = sum_c P_core(c) A_c^{-1} R_buffer(c) r.  Cores
 // partition the unknowns, so every z entry is written exactly once.
 void rasm_apply(const Oracle& o, const double* r, double* z) {
+  const long long stride = o.rasm_stride;
 #pragma omp parallel
   {
     std::vector<double> buf;
@@ -337,7 +341,7 @@ void rasm_apply(const Oracle& o, const double* r, double* z) {
     for (long long c = 0; c < o.ncell; ++c) {
       long long b0 = o.bptr[c];
       int n = (int)(o.bptr[c + 1] - b0);
-      if (n == 0) continue;
+      if (n == 0 || c % stride != 0) continue;
       buf.resize(n);
       for (int i = 0; i < n; ++i) buf[i] = r[o.bidx[b0 + i]];
       lu_solve(n, o.fac.data() + o.fptr[c], o.piv.data() + b0, buf.data());
@@ -725,6 +729,20 @@ int orc_rasm_setup(void* h) {
   Oracle& o = *static_cast<Oracle*>(h);
   if (!o.same_sets) return 4;
   return rasm_setup(o);
+}
+
+// Timing sample: factorise (and later apply) only every stride-th cluster.
+int orc_rasm_setup_sample(void* h, long long stride) {
+  Oracle& o = *static_cast<Oracle*>(h);
+  if (!o.same_sets || stride < 1) return 4;
+  return rasm_setup(o, stride);
+}
+
+int orc_rasm_apply_sample(void* h, const double* r, double* z) {
+  Oracle& o = *static_cast<Oracle*>(h);
+  if (o.fptr.empty()) return 4;
+  rasm_apply(o, r, z);
+  return 0;
 }
 
 int orc_rasm_apply(void* h, const double* r, double* z) {

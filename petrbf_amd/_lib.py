@@ -1,0 +1,94 @@
+"""ctypes binding of libpetrbf.so (the C ABI of include/petrbf.h).
+
+Loading fails loudly: there is no CPU fallback behind this module.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpetrbf.so")
+
+PRBF_KSP_GMRES, PRBF_KSP_FGMRES = 0, 1
+PRBF_PC_NONE, PRBF_PC_RASM_FP64 = 0, 1
+
+STATUS = {0: "PRBF_OK", 1: "PRBF_ERR_ARG", 2: "PRBF_ERR_HIP", 3: "PRBF_ERR_NOMEM",
+          4: "PRBF_ERR_STATE", 5: "PRBF_ERR_FACTOR", 6: "PRBF_ERR_OUTSIDE", 7: "PRBF_ERR_LIMIT"}
+
+# every symbol the header declares
+EXPORTS = [
+    "prbf_solve_opts_default", "prbf_create", "prbf_destroy", "prbf_last_error", "prbf_set_stream",
+    "prbf_synchronize", "prbf_set_points", "prbf_set_points_dev", "prbf_get_info",
+    "prbf_get_permutation", "prbf_get_cells", "prbf_get_buffer_csr", "prbf_get_trunc_csr",
+    "prbf_get_sorted_points", "prbf_matvec", "prbf_matvec_dev", "prbf_evaluate", "prbf_evaluate_dev",
+    "prbf_rasm_setup", "prbf_rasm_apply", "prbf_rasm_apply_dev", "prbf_rasm_get_core_rows",
+    "prbf_solve", "prbf_solve_dev",
+]
+
+
+class PrbfParams(C.Structure):
+    _fields_ = [("dim", C.c_int), ("sigma", C.c_double), ("nsigma_box", C.c_double),
+                ("sigma_buffer", C.c_double), ("sigma_trunc", C.c_double)]
+
+
+class PrbfSolveOpts(C.Structure):
+    _fields_ = [("ksp", C.c_int), ("pc", C.c_int), ("restart", C.c_int), ("max_it", C.c_int),
+                ("rtol", C.c_double), ("abstol", C.c_double), ("dtol", C.c_double)]
+
+
+class PrbfSolveInfo(C.Structure):
+    _fields_ = [("its", C.c_int), ("reason", C.c_int), ("rnorm0", C.c_double), ("rnorm", C.c_double),
+                ("t_setup_s", C.c_double), ("t_solve_s", C.c_double), ("t_matvec_s", C.c_double),
+                ("t_pc_s", C.c_double), ("n_matvec", C.c_int), ("n_pc", C.c_int)]
+
+
+class PrbfError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Load libpetrbf.so; raise if it is missing (build it with petrbf_amd/build.py)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python petrbf_amd/build.py`")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, dp = C.c_void_p, C.c_int, C.c_int64, C.c_void_p
+    ctx = C.c_void_p
+    L.prbf_solve_opts_default.restype = None
+    L.prbf_solve_opts_default.argtypes = [C.POINTER(PrbfSolveOpts)]
+    L.prbf_create.argtypes = [C.POINTER(ctx), C.POINTER(PrbfParams)]
+    L.prbf_destroy.argtypes = [ctx]
+    L.prbf_last_error.restype = C.c_char_p
+    L.prbf_last_error.argtypes = [ctx]
+    L.prbf_set_stream.argtypes = [ctx, vp]
+    L.prbf_synchronize.argtypes = [ctx]
+    for f in ("prbf_set_points", "prbf_set_points_dev"):
+        getattr(L, f).argtypes = [ctx, i64, dp, dp, dp, i64, dp, dp, dp]
+    L.prbf_get_info.argtypes = [ctx, dp, dp]
+    L.prbf_get_permutation.argtypes = [ctx, dp, dp]
+    L.prbf_get_cells.argtypes = [ctx, dp, dp, dp, dp]
+    L.prbf_get_buffer_csr.argtypes = [ctx, dp, dp]
+    L.prbf_get_trunc_csr.argtypes = [ctx, dp, dp]
+    L.prbf_get_sorted_points.argtypes = [ctx, dp, dp, dp]
+    for f in ("prbf_matvec", "prbf_matvec_dev", "prbf_evaluate", "prbf_evaluate_dev"):
+        getattr(L, f).argtypes = [ctx, i32, dp, dp]
+    L.prbf_rasm_setup.argtypes = [ctx]
+    for f in ("prbf_rasm_apply", "prbf_rasm_apply_dev"):
+        getattr(L, f).argtypes = [ctx, dp, dp]
+    L.prbf_rasm_get_core_rows.argtypes = [ctx, i64, dp, C.POINTER(i32), C.POINTER(i32)]
+    for f in ("prbf_solve", "prbf_solve_dev"):
+        getattr(L, f).argtypes = [ctx, C.POINTER(PrbfSolveOpts), dp, dp, C.POINTER(PrbfSolveInfo), dp, i32]
+    for f in EXPORTS:
+        if f not in ("prbf_solve_opts_default", "prbf_last_error"):
+            getattr(L, f).restype = C.c_int
+    _lib = L
+    return L

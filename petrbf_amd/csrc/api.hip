@@ -1,0 +1,824 @@
+// C ABI (include/petrbf.h): context, setup, matvec, RASM, Krylov drivers.
+//
+// Host side of the drop-in boundary.  The context plays the role of the
+// reference's BOTH{PARTICLE, CLUSTER} (2d/par.h:148-151) handed to the PETSc
+// MatShell callbacks; here it owns the device copies of the sorted particles,
+// cell ranges, buffer / truncation lists and RASM core rows.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/petrbf.h"
+#include "common.h"
+#include "kernels.h"
+
+using namespace prbf;
+
+namespace {
+
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void ensure(size_t count) {
+    if (count <= n && p) return;
+    release();
+    size_t bytes = std::max<size_t>(1, count) * sizeof(T);
+    hipError_t e = hipMalloc((void**)&p, bytes);
+    if (e != hipSuccess) {
+      p = nullptr;
+      (void)hipGetLastError();
+      throw ApiError(PRBF_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    n = std::max<size_t>(1, count);
+  }
+  operator T*() const { return p; }
+};
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct prbf_ctx {
+  prbf_params prm{};
+  Grid g{};
+  hipStream_t st = nullptr;
+  bool own_stream = false;
+  std::string err;
+  bool has_points = false, same = true;
+  long long ni = 0, nj = 0;
+  // sorted particles and cell structure
+  DBuf<double> xi, yi, zi, xj, yj, zj;
+  DBuf<int> perm_i, perm_j, ista, iend, jsta, jend;
+  DBuf<long long> bptr, tptr;
+  DBuf<int> bidx, tidx;
+  long long buffer_nnz = 0, trunc_nnz = 0;
+  int max_buffer = 0, max_core = 0, max_trunc = 0;
+  std::vector<long long> h_bptr;
+  std::vector<int> h_ista, h_iend;
+  // RASM
+  bool rasm_ready = false;
+  DBuf<long long> mptr;
+  DBuf<double> M;
+  std::vector<long long> h_mptr;
+  // scratch
+  DBuf<int> i_scr1, i_scr2, i_scr3, d_err;
+  DBuf<long long> l_scr;
+  DBuf<double> part, scal, v1, v2, v3;
+  double* pin = nullptr;  // pinned host scalars
+  // Krylov
+  DBuf<double> V, Z;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  const double* xj_() const { return same ? xi.p : xj.p; }
+  const double* yj_() const { return same ? yi.p : yj.p; }
+  const double* zj_() const { return same ? zi.p : zj.p; }
+  const int* jsta_() const { return same ? ista.p : jsta.p; }
+  const int* jend_() const { return same ? iend.p : jend.p; }
+  const int* perm_j_() const { return same ? perm_i.p : perm_j.p; }
+};
+
+namespace {
+
+prbf_status fail(prbf_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return (prbf_status)code;
+}
+
+template <class F>
+prbf_status guarded(prbf_ctx* c, F&& f) {
+  if (!c) return PRBF_ERR_ARG;
+  try {
+    f();
+    c->err.clear();
+    return PRBF_OK;
+  } catch (const ApiError& e) {
+    return fail(c, e.code, e.what);
+  } catch (const HipError& e) {
+    return fail(c, PRBF_ERR_HIP, e.what);
+  }
+}
+
+void sync(prbf_ctx* c) { PRBF_HIP(hipStreamSynchronize(c->st)); }
+
+void need(bool ok, int code, const char* msg) {
+  if (!ok) throw ApiError(code, msg);
+}
+
+// Grid parameters from the bounding box (2d/get_cluster.h:20-34), evaluated
+// on the host in the reference's operation order.
+Grid make_grid(const prbf_params& p, const double* lo, const double* hi) {
+#pragma clang fp contract(off)
+  Grid g{};
+  g.dim = p.dim;
+  double nb = p.nsigma_box;
+  g.nb_buffer = (int)ceil((p.sigma_buffer - nb + kEpsf) / 2 / nb);
+  g.nb_trunc = (int)ceil((p.sigma_trunc - nb + kEpsf) / 2 / nb);
+  g.box = p.nsigma_box * p.sigma + kEpsf;
+  double mn[3] = {0, 0, 0};
+  int nc[3] = {1, 1, 1};
+  for (int d = 0; d < p.dim; ++d) {
+    mn[d] = lo[d] - kEpsf;
+    double mx = hi[d] + kEpsf;
+    nc[d] = (int)ceil((mx - mn[d]) / g.box);
+  }
+  g.xmin = mn[0];
+  g.ymin = mn[1];
+  g.zmin = mn[2];
+  g.nx = nc[0];
+  g.ny = nc[1];
+  g.nz = nc[2];
+  g.ncell = (long long)nc[0] * nc[1] * nc[2];
+  g.buffer_len = p.sigma_buffer * p.sigma / 2 + kEpsf;
+  g.trunc_len = p.sigma_trunc * p.sigma / 2 + kEpsf;
+  return g;
+}
+
+void bbox_of(prbf_ctx* c, long long n, const double* x, const double* y, const double* z, double* lo, double* hi) {
+  c->part.ensure(6 * 256);
+  c->scal.ensure(128);
+  bbox(c->st, n, x, y, z, c->prm.dim, c->part, c->scal);
+  PRBF_HIP(hipMemcpyAsync(c->pin, c->scal, 6 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  sync(c);
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = c->pin[d];
+    hi[d] = c->pin[3 + d];
+  }
+}
+
+void build_lists(prbf_ctx* c, bool buffer) {
+  const Grid& g = c->g;
+  int reach = buffer ? g.nb_buffer : g.nb_trunc;
+  double len = buffer ? g.buffer_len : g.trunc_len;
+  const int* sta = buffer ? c->ista.p : c->jsta_();
+  const int* end = buffer ? c->iend.p : c->jend_();
+  const double* xs = buffer ? c->xi.p : c->xj_();
+  const double* ys = buffer ? c->yi.p : c->yj_();
+  const double* zs = buffer ? c->zi.p : c->zj_();
+  DBuf<long long>& ptr = buffer ? c->bptr : c->tptr;
+  DBuf<int>& idx = buffer ? c->bidx : c->tidx;
+  c->i_scr1.ensure(g.ncell);
+  ptr.ensure(g.ncell + 1);
+  list_count(c->st, g, reach, len, c->ista, c->iend, sta, end, xs, ys, zs, c->i_scr1);
+  scan_excl(c->st, g.ncell, c->i_scr1, ptr);
+  long long total = 0;
+  PRBF_HIP(hipMemcpyAsync(&total, ptr.p + g.ncell, sizeof(long long), hipMemcpyDeviceToHost, c->st));
+  sync(c);
+  idx.ensure(total);
+  list_fill(c->st, g, reach, len, c->ista, c->iend, sta, end, xs, ys, zs, ptr, idx);
+  (buffer ? c->buffer_nnz : c->trunc_nnz) = total;
+}
+
+void set_points_impl(prbf_ctx* c, long long ni, const double* xi, const double* yi, const double* zi,
+                     long long nj, const double* xj, const double* yj, const double* zj) {
+  const int dim = c->prm.dim;
+  need(ni > 0, PRBF_ERR_ARG, "ni must be positive");
+  need(xi && yi && (dim == 2 || zi), PRBF_ERR_ARG, "missing target coordinates");
+  need(ni < (1LL << 31), PRBF_ERR_ARG, "ni exceeds int32 indexing");
+  bool same = (xj == nullptr);
+  if (!same) {
+    need(nj > 0 && yj && (dim == 2 || zj), PRBF_ERR_ARG, "missing source coordinates");
+    need(nj < (1LL << 31), PRBF_ERR_ARG, "nj exceeds int32 indexing");
+  } else {
+    nj = ni;
+  }
+  c->has_points = false;
+  c->rasm_ready = false;
+  double lo[3], hi[3], lo2[3], hi2[3];
+  bbox_of(c, ni, xi, yi, zi, lo, hi);
+  if (!same) {
+    bbox_of(c, nj, xj, yj, zj, lo2, hi2);
+    for (int d = 0; d < 3; ++d) { lo[d] = std::min(lo[d], lo2[d]); hi[d] = std::max(hi[d], hi2[d]); }
+  }
+  for (int d = 0; d < dim; ++d) need(std::isfinite(lo[d]) && std::isfinite(hi[d]), PRBF_ERR_ARG, "non-finite coordinate");
+  c->g = make_grid(c->prm, lo, hi);
+  const Grid& g = c->g;
+  need(g.ncell > 0 && g.ncell < (1LL << 31), PRBF_ERR_LIMIT, "cell grid too large");
+  c->same = same;
+  c->ni = ni;
+  c->nj = nj;
+  c->d_err.ensure(1);
+  PRBF_HIP(hipMemsetAsync(c->d_err, 0, sizeof(int), c->st));
+  c->l_scr.ensure(g.ncell + 1);
+  c->i_scr1.ensure(std::max<long long>(ni, nj));
+  c->i_scr2.ensure(g.ncell);
+  c->i_scr3.ensure(g.ncell);
+  auto sort_set = [&](long long n, const double* x, const double* y, const double* z, DBuf<double>& xs,
+                      DBuf<double>& ys, DBuf<double>& zs, DBuf<int>& perm, DBuf<int>& sta, DBuf<int>& end) {
+    xs.ensure(n);
+    ys.ensure(n);
+    if (dim == 3) zs.ensure(n);
+    perm.ensure(n);
+    sta.ensure(g.ncell);
+    end.ensure(g.ncell);
+    sort_points(c->st, g, n, x, y, dim == 3 ? z : nullptr, c->i_scr1, c->i_scr2, c->l_scr, c->i_scr3, perm, sta,
+                end, xs, ys, dim == 3 ? zs.p : nullptr, c->d_err);
+  };
+  sort_set(ni, xi, yi, zi, c->xi, c->yi, c->zi, c->perm_i, c->ista, c->iend);
+  if (!same) sort_set(nj, xj, yj, zj, c->xj, c->yj, c->zj, c->perm_j, c->jsta, c->jend);
+  int herr = 0;
+  PRBF_HIP(hipMemcpyAsync(&herr, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->st));
+  sync(c);
+  need(herr == 0, PRBF_ERR_OUTSIDE, "a point fell outside the cell grid");
+  build_lists(c, true);
+  build_lists(c, false);
+  // host copies for chunking and size limits
+  c->h_bptr.resize(g.ncell + 1);
+  c->h_ista.resize(g.ncell);
+  c->h_iend.resize(g.ncell);
+  std::vector<long long> h_tptr(g.ncell + 1);
+  PRBF_HIP(hipMemcpyAsync(c->h_bptr.data(), c->bptr, sizeof(long long) * (g.ncell + 1), hipMemcpyDeviceToHost, c->st));
+  PRBF_HIP(hipMemcpyAsync(h_tptr.data(), c->tptr, sizeof(long long) * (g.ncell + 1), hipMemcpyDeviceToHost, c->st));
+  PRBF_HIP(hipMemcpyAsync(c->h_ista.data(), c->ista, sizeof(int) * g.ncell, hipMemcpyDeviceToHost, c->st));
+  PRBF_HIP(hipMemcpyAsync(c->h_iend.data(), c->iend, sizeof(int) * g.ncell, hipMemcpyDeviceToHost, c->st));
+  sync(c);
+  c->max_buffer = c->max_core = c->max_trunc = 0;
+  for (long long k = 0; k < g.ncell; ++k) {
+    c->max_buffer = std::max<int>(c->max_buffer, (int)(c->h_bptr[k + 1] - c->h_bptr[k]));
+    c->max_trunc = std::max<int>(c->max_trunc, (int)(h_tptr[k + 1] - h_tptr[k]));
+    c->max_core = std::max(c->max_core, c->h_iend[k] - c->h_ista[k] + 1);
+  }
+  c->has_points = true;
+}
+
+void matvec_impl(prbf_ctx* c, int nrhs, const double* g, double* w) {
+  need(c->has_points, PRBF_ERR_STATE, "set_points first");
+  need(nrhs >= 1 && nrhs <= 3, PRBF_ERR_ARG, "nrhs must be 1..3");
+  matvec(c->st, c->prm.dim, nrhs, c->g.ncell, c->ista, c->iend, c->tptr, c->tidx, c->xi, c->yi, c->zi,
+         c->xj_(), c->yj_(), c->zj_(), g, c->nj, w, c->ni, c->prm.sigma);
+  PRBF_HIP(hipGetLastError());
+}
+
+constexpr int kMaxLdsBytes = 160 * 1024 - 4096;
+
+void rasm_setup_impl(prbf_ctx* c) {
+  need(c->has_points, PRBF_ERR_STATE, "set_points first");
+  need(c->same, PRBF_ERR_STATE, "RASM needs the source set equal to the target set");
+  const Grid& g = c->g;
+  const long long nc = g.ncell;
+  c->h_mptr.assign(nc + 1, 0);
+  for (long long k = 0; k < nc; ++k) {
+    long long n = c->h_bptr[k + 1] - c->h_bptr[k];
+    long long T = n ? (c->h_iend[k] - c->h_ista[k] + 1) : 0;
+    c->h_mptr[k + 1] = c->h_mptr[k] + n * T;
+  }
+  size_t lds = sizeof(double) * ((size_t)3 * c->max_buffer + (size_t)std::max(16, c->max_core) * c->max_buffer);
+  need(lds <= (size_t)kMaxLdsBytes, PRBF_ERR_LIMIT, "RASM block too large for the single-workgroup factorisation");
+  c->mptr.ensure(nc + 1);
+  c->M.ensure(c->h_mptr[nc]);
+  PRBF_HIP(hipMemcpyAsync(c->mptr, c->h_mptr.data(), sizeof(long long) * (nc + 1), hipMemcpyHostToDevice, c->st));
+  // chunks bounded by a workspace budget
+  size_t free_b = 0, total_b = 0;
+  PRBF_HIP(hipMemGetInfo(&free_b, &total_b));
+  long long budget = (long long)std::min<size_t>(free_b / 3, (size_t)16 << 30) / (long long)sizeof(double);
+  long long maxsq = (long long)c->max_buffer * c->max_buffer;
+  need(budget >= maxsq, PRBF_ERR_NOMEM, "not enough device memory for one RASM block");
+  DBuf<double> W;
+  DBuf<long long> wptr;
+  std::vector<long long> h_wptr;
+  PRBF_HIP(hipMemsetAsync(c->d_err, 0, sizeof(int), c->st));
+  long long k0 = 0;
+  bool allocated = false;
+  while (k0 < nc) {
+    long long k1 = k0, used = 0;
+    h_wptr.clear();
+    while (k1 < nc) {
+      long long n = c->h_bptr[k1 + 1] - c->h_bptr[k1];
+      if (used + n * n > budget && k1 > k0) break;
+      h_wptr.push_back(used);
+      used += n * n;
+      ++k1;
+      if (k1 - k0 >= (1LL << 20)) break;
+    }
+    if (!allocated) {
+      // size the workspace once for the largest chunk this budget allows
+      W.ensure((size_t)std::min<long long>(budget, std::max<long long>(used, 1)));
+      wptr.ensure(std::min<long long>(nc, 1LL << 20) + 1);
+      allocated = true;
+    }
+    if ((size_t)used > W.n) W.ensure(used);
+    sync(c);  // previous chunk done before wptr is overwritten
+    PRBF_HIP(hipMemcpy(wptr, h_wptr.data(), sizeof(long long) * h_wptr.size(), hipMemcpyHostToDevice));
+    int nmax = 0, tmax = 0;
+    for (long long k = k0; k < k1; ++k) {
+      nmax = std::max<int>(nmax, (int)(c->h_bptr[k + 1] - c->h_bptr[k]));
+      tmax = std::max(tmax, c->h_iend[k] - c->h_ista[k] + 1);
+    }
+    rasm_factor_fp64(c->st, g.dim, k0, k1 - k0, nmax, tmax, c->bptr, c->bidx, c->ista, c->iend, c->xi, c->yi,
+                     c->zi, wptr, W, c->mptr, c->M, c->prm.sigma, c->d_err);
+    PRBF_HIP(hipGetLastError());
+    k0 = k1;
+  }
+  int herr = 0;
+  PRBF_HIP(hipMemcpyAsync(&herr, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->st));
+  sync(c);
+  need(herr == 0, PRBF_ERR_FACTOR, "a RASM block is not numerically positive definite");
+  c->rasm_ready = true;
+}
+
+void rasm_apply_impl(prbf_ctx* c, const double* r, double* z) {
+  need(c->rasm_ready, PRBF_ERR_STATE, "rasm_setup first");
+  rasm_apply_fp64(c->st, c->g.ncell, c->bptr, c->bidx, c->ista, c->iend, c->mptr, c->M, r, z, c->max_core);
+  PRBF_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- Krylov
+struct Conv {  // PETSc KSPConvergedDefault, preconditioned norm, zero guess
+  double rtol, abstol, dtol, rnorm0 = 0, ttol = 0;
+  int operator()(int n, double rnorm) {
+    if (n == 0) {
+      rnorm0 = rnorm;
+      ttol = std::max(rtol * rnorm0, abstol);
+    }
+    if (rnorm != rnorm) return -9;
+    if (rnorm <= ttol) return rnorm < abstol ? 3 : 2;
+    if (rnorm >= dtol * rnorm0) return -4;
+    return 0;
+  }
+};
+
+struct Timed {
+  prbf_ctx* c;
+  double* acc;
+  int* cnt;
+  Timed(prbf_ctx* c_, double* a, int* n) : c(c_), acc(a), cnt(n) { PRBF_HIP(hipEventRecord(c->ev0, c->st)); }
+  ~Timed() noexcept(false) {
+    PRBF_HIP(hipEventRecord(c->ev1, c->st));
+    PRBF_HIP(hipEventSynchronize(c->ev1));
+    float ms = 0;
+    PRBF_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *acc += ms * 1e-3;
+    ++*cnt;
+  }
+};
+
+// Restarted GMRES (left PC) or FGMRES (right PC), PETSc's KSPGMRESCycle
+// structure: classical Gram-Schmidt without refinement, Givens rotations on
+// the host, solution update at the end of each cycle.
+void krylov_impl(prbf_ctx* c, const prbf_solve_opts& o, const double* b, double* x, prbf_solve_info* info,
+                 double* hist, int hist_cap) {
+  const long long n = c->ni;
+  const int m = o.restart;
+  need(m >= 1 && m < 63, PRBF_ERR_ARG, "restart must be in 1..62");
+  need(o.pc == PRBF_PC_NONE || c->rasm_ready, PRBF_ERR_STATE, "rasm_setup first");
+  const bool flex = (o.ksp == PRBF_KSP_FGMRES);
+  c->V.ensure((size_t)(m + 1) * n);
+  if (flex) c->Z.ensure((size_t)m * n);
+  c->v1.ensure(n);
+  c->v2.ensure(n);
+  c->part.ensure(kRedBlocks * 64);
+  c->scal.ensure(256);
+  double* V = c->V;
+  auto Vj = [&](int j) { return V + (long long)j * n; };
+  double* tmp = c->v1;
+  double* tmp2 = c->v2;
+  double* dsc = c->scal;  // [0..63] dots, [64] norm^2, [128..191] y
+  double* pin = c->pin;
+  prbf_solve_info inf{};
+  auto A = [&](const double* v, double* y) {
+    Timed t(c, &inf.t_matvec_s, &inf.n_matvec);
+    matvec_impl(c, 1, v, y);
+  };
+  auto B = [&](const double* v, double* y) {
+    if (o.pc == PRBF_PC_NONE) {
+      PRBF_HIP(hipMemcpyAsync(y, v, sizeof(double) * n, hipMemcpyDeviceToDevice, c->st));
+      return;
+    }
+    Timed t(c, &inf.t_pc_s, &inf.n_pc);
+    rasm_apply_impl(c, v, y);
+  };
+  auto read_norm = [&]() {
+    PRBF_HIP(hipMemcpyAsync(pin, dsc + 64, sizeof(double), hipMemcpyDeviceToHost, c->st));
+    sync(c);
+    return std::sqrt(pin[0]);
+  };
+  std::vector<double> hh((size_t)(m + 1) * (m + 1), 0.0), cc(m + 1), ss(m + 1), rs(m + 2), nrs(m + 1);
+  auto HH = [&](int i, int j) -> double& { return hh[(size_t)j * (m + 1) + i]; };
+  Conv conv{o.rtol, o.abstol, o.dtol};
+  int nh = 0;
+  double last_res = 0.0;
+  auto log_res = [&](double r) {
+    if (hist && nh < hist_cap) hist[nh] = r;
+    ++nh;
+    last_res = r;
+  };
+  const double t0 = now_s();
+  fill(c->st, n, x, 0.0);
+  bool guess_zero = true;
+  int its = 0, itcount = 0, reason = 0;
+  while (!reason) {
+    // initial residual of the cycle
+    if (guess_zero) {
+      PRBF_HIP(hipMemcpyAsync(tmp, b, sizeof(double) * n, hipMemcpyDeviceToDevice, c->st));
+    } else {
+      A(x, tmp2);
+      xpay_sub(c->st, n, b, tmp2, tmp);
+    }
+    if (flex) {
+      PRBF_HIP(hipMemcpyAsync(Vj(0), tmp, sizeof(double) * n, hipMemcpyDeviceToDevice, c->st));
+    } else {
+      B(tmp, Vj(0));
+    }
+    norm2sq(c->st, n, Vj(0), c->part, dsc + 64);
+    double res = read_norm();
+    if (res != 0.0) scale(c->st, n, Vj(0), 1.0 / res);
+    rs[0] = res;
+    log_res(res);
+    if (res == 0.0) {
+      reason = 3;
+      break;
+    }
+    reason = conv(its, res);
+    int loc = 0;
+    bool hapend = false;
+    while (!reason && loc < m && its < o.max_it) {
+      if (flex) {
+        B(Vj(loc), c->Z.p + (long long)loc * n);
+        A(c->Z.p + (long long)loc * n, Vj(loc + 1));
+      } else {
+        A(Vj(loc), tmp);
+        B(tmp, Vj(loc + 1));
+      }
+      double* w = Vj(loc + 1);
+      mdot(c->st, n, w, V, n, loc + 1, c->part, dsc);
+      maxpy_norm2(c->st, n, w, V, n, loc + 1, dsc, -1.0, c->part, dsc + 64);
+      PRBF_HIP(hipMemcpyAsync(pin, dsc, sizeof(double) * 65, hipMemcpyDeviceToHost, c->st));
+      sync(c);
+      for (int j = 0; j <= loc; ++j) HH(j, loc) = pin[j];
+      double tt = std::sqrt(pin[64]);
+      if (tt != 0.0) scale(c->st, n, w, 1.0 / tt);
+      HH(loc + 1, loc) = tt;
+      double hapbnd = std::min(std::fabs(tt / rs[loc]), 1e-30);
+      if (tt < hapbnd) hapend = true;
+      for (int j = 1; j <= loc; ++j) {
+        double a0 = HH(j - 1, loc);
+        HH(j - 1, loc) = cc[j - 1] * a0 + ss[j - 1] * HH(j, loc);
+        HH(j, loc) = cc[j - 1] * HH(j, loc) - ss[j - 1] * a0;
+      }
+      if (!hapend) {
+        double a0 = HH(loc, loc), a1 = HH(loc + 1, loc);
+        double t = std::sqrt(a0 * a0 + a1 * a1);
+        if (t == 0.0) {
+          reason = -2;
+          break;
+        }
+        cc[loc] = a0 / t;
+        ss[loc] = a1 / t;
+        rs[loc + 1] = -(ss[loc] * rs[loc]);
+        rs[loc] = cc[loc] * rs[loc];
+        HH(loc, loc) = cc[loc] * a0 + ss[loc] * a1;
+        res = std::fabs(rs[loc + 1]);
+      } else {
+        res = 0.0;
+      }
+      ++loc;
+      ++its;
+      log_res(res);
+      reason = conv(its, res);
+      if (hapend && !reason) reason = 8;
+    }
+    int it = loc - 1;
+    if (it >= 0) {
+      nrs[it] = rs[it] / HH(it, it);
+      for (int ii = 1; ii <= it; ++ii) {
+        int k = it - ii;
+        double t = rs[k];
+        for (int j = k + 1; j <= it; ++j) t = t - HH(k, j) * nrs[j];
+        nrs[k] = t / HH(k, k);
+      }
+      for (int j = 0; j <= it; ++j) pin[128 + j] = nrs[j];
+      PRBF_HIP(hipMemcpyAsync(dsc + 128, pin + 128, sizeof(double) * (it + 1), hipMemcpyHostToDevice, c->st));
+      update_x(c->st, n, x, flex ? c->Z.p : V, n, it + 1, dsc + 128);
+      sync(c);  // pinned y consumed before the next cycle reuses it
+    }
+    itcount += loc;
+    if (itcount >= o.max_it && !reason) {
+      reason = -3;
+      break;
+    }
+    guess_zero = false;
+  }
+  sync(c);
+  inf.its = its;
+  inf.reason = reason;
+  inf.rnorm0 = conv.rnorm0;
+  inf.rnorm = last_res;
+  inf.t_solve_s = now_s() - t0;
+  if (info) *info = inf;
+}
+
+void solve_dev_impl(prbf_ctx* c, const prbf_solve_opts* opts, const double* w, double* gamma,
+                    prbf_solve_info* info, double* hist, int hist_cap) {
+  need(c->has_points && c->same, PRBF_ERR_STATE, "set_points (one set) first");
+  prbf_solve_opts o;
+  prbf_solve_opts_default(&o);
+  if (opts) o = *opts;
+  double t_setup = 0.0;
+  if (o.pc == PRBF_PC_RASM_FP64 && !c->rasm_ready) {
+    double t0 = now_s();
+    rasm_setup_impl(c);
+    t_setup = now_s() - t0;
+  }
+  const long long n = c->ni;
+  c->v3.ensure(2 * n);
+  double* bs = c->v3;
+  double* xs = c->v3.p + n;
+  gather(c->st, n, c->perm_i, w, bs);
+  std::vector<double> hbuf;
+  double* h = hist;
+  int cap = hist_cap;
+  if (!h) {
+    hbuf.resize(o.max_it + 64);
+    h = hbuf.data();
+    cap = (int)hbuf.size();
+  }
+  prbf_solve_info local{};
+  krylov_impl(c, o, bs, xs, &local, h, cap);
+  scatter(c->st, n, c->perm_i, xs, gamma);
+  local.t_setup_s = t_setup;
+  if (info) *info = local;
+}
+
+}  // namespace
+
+// ============================================================== C ABI
+extern "C" {
+
+void prbf_solve_opts_default(prbf_solve_opts* o) {
+  if (!o) return;
+  o->ksp = PRBF_KSP_GMRES;
+  o->pc = PRBF_PC_RASM_FP64;
+  o->restart = 30;
+  o->max_it = 1000;
+  o->rtol = 1e-10;
+  o->abstol = 1e-50;
+  o->dtol = 1e5;
+}
+
+prbf_status prbf_create(prbf_ctx** out, const prbf_params* p) {
+  if (!out || !p) return PRBF_ERR_ARG;
+  *out = nullptr;
+  if ((p->dim != 2 && p->dim != 3) || !(p->sigma > 0) || !(p->nsigma_box > 0) || !(p->sigma_buffer > 0) ||
+      !(p->sigma_trunc > 0))
+    return PRBF_ERR_ARG;
+  prbf_ctx* c = new prbf_ctx;
+  c->prm = *p;
+  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
+      hipHostMalloc((void**)&c->pin, 512 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    (void)hipGetLastError();
+    delete c;
+    return PRBF_ERR_HIP;
+  }
+  c->own_stream = true;
+  *out = c;
+  return PRBF_OK;
+}
+
+prbf_status prbf_destroy(prbf_ctx* c) {
+  if (!c) return PRBF_ERR_ARG;
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  if (c->pin) (void)hipHostFree(c->pin);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->own_stream && c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+  return PRBF_OK;
+}
+
+const char* prbf_last_error(const prbf_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+prbf_status prbf_set_stream(prbf_ctx* c, void* stream) {
+  return guarded(c, [&] {
+    sync(c);
+    if (c->own_stream) PRBF_HIP(hipStreamDestroy(c->st));
+    c->st = (hipStream_t)stream;
+    c->own_stream = false;
+  });
+}
+
+prbf_status prbf_synchronize(prbf_ctx* c) {
+  return guarded(c, [&] { sync(c); });
+}
+
+prbf_status prbf_set_points_dev(prbf_ctx* c, int64_t ni, const double* xi, const double* yi, const double* zi,
+                                int64_t nj, const double* xj, const double* yj, const double* zj) {
+  return guarded(c, [&] { set_points_impl(c, ni, xi, yi, zi, nj, xj, yj, zj); });
+}
+
+prbf_status prbf_set_points(prbf_ctx* c, int64_t ni, const double* xi, const double* yi, const double* zi,
+                            int64_t nj, const double* xj, const double* yj, const double* zj) {
+  return guarded(c, [&] {
+    need(ni > 0, PRBF_ERR_ARG, "ni must be positive");
+    const int dim = c->prm.dim;
+    bool same = (xj == nullptr);
+    long long nsrc = same ? 0 : nj;
+    need(same || nj > 0, PRBF_ERR_ARG, "nj must be positive");
+    DBuf<double> buf;
+    buf.ensure((size_t)3 * (ni + nsrc));
+    double* p = buf;
+    auto up = [&](double* dst, const double* src, long long n) {
+      if (src) PRBF_HIP(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyHostToDevice, c->st));
+    };
+    up(p, xi, ni);
+    up(p + ni, yi, ni);
+    if (dim == 3) { need(zi != nullptr, PRBF_ERR_ARG, "missing z"); up(p + 2 * ni, zi, ni); }
+    double* q = p + 3 * ni;
+    if (!same) {
+      up(q, xj, nsrc);
+      up(q + nsrc, yj, nsrc);
+      if (dim == 3) { need(zj != nullptr, PRBF_ERR_ARG, "missing z"); up(q + 2 * nsrc, zj, nsrc); }
+    }
+    set_points_impl(c, ni, p, p + ni, dim == 3 ? p + 2 * ni : nullptr, nj, same ? nullptr : q,
+                    same ? nullptr : q + nsrc, (!same && dim == 3) ? q + 2 * nsrc : nullptr);
+    sync(c);
+  });
+}
+
+prbf_status prbf_get_info(prbf_ctx* c, int64_t* info, double* geom) {
+  return guarded(c, [&] {
+    need(c->has_points, PRBF_ERR_STATE, "set_points first");
+    const Grid& g = c->g;
+    if (info) {
+      int64_t v[16] = {g.ncell, g.nx, g.ny, g.nz, g.nb_buffer, g.nb_trunc, c->buffer_nnz, c->trunc_nnz,
+                       c->max_buffer, c->max_core, c->max_trunc, c->ni, c->nj, 0, 0, 0};
+      std::memcpy(info, v, sizeof(v));
+    }
+    if (geom) {
+      double v[6] = {g.xmin, g.ymin, g.zmin, g.box, g.buffer_len, g.trunc_len};
+      std::memcpy(geom, v, sizeof(v));
+    }
+  });
+}
+
+static void d2h(prbf_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (dst && bytes) PRBF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->st));
+}
+
+prbf_status prbf_get_permutation(prbf_ctx* c, int32_t* pi, int32_t* pj) {
+  return guarded(c, [&] {
+    need(c->has_points, PRBF_ERR_STATE, "set_points first");
+    d2h(c, pi, c->perm_i, sizeof(int) * c->ni);
+    d2h(c, pj, c->perm_j_(), sizeof(int) * c->nj);
+    sync(c);
+  });
+}
+
+prbf_status prbf_get_cells(prbf_ctx* c, int32_t* ista, int32_t* iend, int32_t* jsta, int32_t* jend) {
+  return guarded(c, [&] {
+    need(c->has_points, PRBF_ERR_STATE, "set_points first");
+    size_t b = sizeof(int) * c->g.ncell;
+    d2h(c, ista, c->ista, b);
+    d2h(c, iend, c->iend, b);
+    d2h(c, jsta, c->jsta_(), b);
+    d2h(c, jend, c->jend_(), b);
+    sync(c);
+  });
+}
+
+prbf_status prbf_get_buffer_csr(prbf_ctx* c, int64_t* ptr, int32_t* idx) {
+  return guarded(c, [&] {
+    need(c->has_points, PRBF_ERR_STATE, "set_points first");
+    d2h(c, ptr, c->bptr, sizeof(long long) * (c->g.ncell + 1));
+    d2h(c, idx, c->bidx, sizeof(int) * c->buffer_nnz);
+    sync(c);
+  });
+}
+
+prbf_status prbf_get_trunc_csr(prbf_ctx* c, int64_t* ptr, int32_t* idx) {
+  return guarded(c, [&] {
+    need(c->has_points, PRBF_ERR_STATE, "set_points first");
+    d2h(c, ptr, c->tptr, sizeof(long long) * (c->g.ncell + 1));
+    d2h(c, idx, c->tidx, sizeof(int) * c->trunc_nnz);
+    sync(c);
+  });
+}
+
+prbf_status prbf_get_sorted_points(prbf_ctx* c, double* x, double* y, double* z) {
+  return guarded(c, [&] {
+    need(c->has_points, PRBF_ERR_STATE, "set_points first");
+    d2h(c, x, c->xi, sizeof(double) * c->ni);
+    d2h(c, y, c->yi, sizeof(double) * c->ni);
+    if (c->prm.dim == 3) d2h(c, z, c->zi, sizeof(double) * c->ni);
+    sync(c);
+  });
+}
+
+prbf_status prbf_matvec_dev(prbf_ctx* c, int nrhs, const double* g, double* w) {
+  return guarded(c, [&] { matvec_impl(c, nrhs, g, w); });
+}
+
+prbf_status prbf_matvec(prbf_ctx* c, int nrhs, const double* g, double* w) {
+  return guarded(c, [&] {
+    need(c->has_points, PRBF_ERR_STATE, "set_points first");
+    need(nrhs >= 1 && nrhs <= 3, PRBF_ERR_ARG, "nrhs must be 1..3");
+    DBuf<double> dg, dw;
+    dg.ensure((size_t)nrhs * c->nj);
+    dw.ensure((size_t)nrhs * c->ni);
+    PRBF_HIP(hipMemcpyAsync(dg, g, sizeof(double) * nrhs * c->nj, hipMemcpyHostToDevice, c->st));
+    matvec_impl(c, nrhs, dg, dw);
+    PRBF_HIP(hipMemcpyAsync(w, dw, sizeof(double) * nrhs * c->ni, hipMemcpyDeviceToHost, c->st));
+    sync(c);
+  });
+}
+
+static void evaluate_dev_impl(prbf_ctx* c, int nrhs, const double* gj, double* wi) {
+  need(c->has_points, PRBF_ERR_STATE, "set_points first");
+  need(nrhs >= 1 && nrhs <= 3, PRBF_ERR_ARG, "nrhs must be 1..3");
+  DBuf<double> gs, ws;
+  gs.ensure((size_t)nrhs * c->nj);
+  ws.ensure((size_t)nrhs * c->ni);
+  for (int r = 0; r < nrhs; ++r) gather(c->st, c->nj, c->perm_j_(), gj + (long long)r * c->nj, gs.p + (long long)r * c->nj);
+  matvec_impl(c, nrhs, gs, ws);
+  for (int r = 0; r < nrhs; ++r) scatter(c->st, c->ni, c->perm_i, ws.p + (long long)r * c->ni, wi + (long long)r * c->ni);
+  sync(c);  // scratch buffers are released on return
+}
+
+prbf_status prbf_evaluate_dev(prbf_ctx* c, int nrhs, const double* gj, double* wi) {
+  return guarded(c, [&] { evaluate_dev_impl(c, nrhs, gj, wi); });
+}
+
+prbf_status prbf_evaluate(prbf_ctx* c, int nrhs, const double* gj, double* wi) {
+  return guarded(c, [&] {
+    need(c->has_points, PRBF_ERR_STATE, "set_points first");
+    need(nrhs >= 1 && nrhs <= 3, PRBF_ERR_ARG, "nrhs must be 1..3");
+    DBuf<double> dg, dw;
+    dg.ensure((size_t)nrhs * c->nj);
+    dw.ensure((size_t)nrhs * c->ni);
+    PRBF_HIP(hipMemcpyAsync(dg, gj, sizeof(double) * nrhs * c->nj, hipMemcpyHostToDevice, c->st));
+    evaluate_dev_impl(c, nrhs, dg, dw);
+    PRBF_HIP(hipMemcpyAsync(wi, dw, sizeof(double) * nrhs * c->ni, hipMemcpyDeviceToHost, c->st));
+    sync(c);
+  });
+}
+
+prbf_status prbf_rasm_setup(prbf_ctx* c) {
+  return guarded(c, [&] { rasm_setup_impl(c); });
+}
+
+prbf_status prbf_rasm_apply_dev(prbf_ctx* c, const double* r, double* z) {
+  return guarded(c, [&] { rasm_apply_impl(c, r, z); });
+}
+
+prbf_status prbf_rasm_apply(prbf_ctx* c, const double* r, double* z) {
+  return guarded(c, [&] {
+    need(c->rasm_ready, PRBF_ERR_STATE, "rasm_setup first");
+    DBuf<double> dr, dz;
+    dr.ensure(c->ni);
+    dz.ensure(c->ni);
+    PRBF_HIP(hipMemcpyAsync(dr, r, sizeof(double) * c->ni, hipMemcpyHostToDevice, c->st));
+    rasm_apply_impl(c, dr, dz);
+    PRBF_HIP(hipMemcpyAsync(z, dz, sizeof(double) * c->ni, hipMemcpyDeviceToHost, c->st));
+    sync(c);
+  });
+}
+
+prbf_status prbf_rasm_get_core_rows(prbf_ctx* c, int64_t k, double* M, int32_t* n, int32_t* T) {
+  return guarded(c, [&] {
+    need(c->rasm_ready, PRBF_ERR_STATE, "rasm_setup first");
+    need(k >= 0 && k < c->g.ncell, PRBF_ERR_ARG, "cluster index out of range");
+    int nn = (int)(c->h_bptr[k + 1] - c->h_bptr[k]);
+    int tt = nn ? c->h_iend[k] - c->h_ista[k] + 1 : 0;
+    if (n) *n = nn;
+    if (T) *T = tt;
+    if (M && nn) {
+      d2h(c, M, c->M.p + c->h_mptr[k], sizeof(double) * (size_t)nn * tt);
+      sync(c);
+    }
+  });
+}
+
+prbf_status prbf_solve_dev(prbf_ctx* c, const prbf_solve_opts* opts, const double* w, double* gamma,
+                           prbf_solve_info* info, double* hist, int hist_cap) {
+  return guarded(c, [&] { solve_dev_impl(c, opts, w, gamma, info, hist, hist_cap); });
+}
+
+prbf_status prbf_solve(prbf_ctx* c, const prbf_solve_opts* opts, const double* w, double* gamma,
+                       prbf_solve_info* info, double* hist, int hist_cap) {
+  return guarded(c, [&] {
+    need(c->has_points && c->same, PRBF_ERR_STATE, "set_points (one set) first");
+    DBuf<double> dw, dg;
+    dw.ensure(c->ni);
+    dg.ensure(c->ni);
+    PRBF_HIP(hipMemcpyAsync(dw, w, sizeof(double) * c->ni, hipMemcpyHostToDevice, c->st));
+    solve_dev_impl(c, opts, dw, dg, info, hist, hist_cap);
+    PRBF_HIP(hipMemcpyAsync(gamma, dg, sizeof(double) * c->ni, hipMemcpyDeviceToHost, c->st));
+    sync(c);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,250 @@
+// Cell list, stable sort and cluster lists (kernel family (a)).
+//
+// Device restatement of 2d/get_cluster.h:20-166 (single rank), 2d/get_buffer.h:11-81
+// and 2d/get_trunc.h:14-78, with the 3D twins.  Results are bit-identical to the
+// reference's: cell ids use the same IEEE divide and floor, box tests the same
+// fabs(x - xc) < len in FP64, and list order is core first, then neighbour
+// cells jx-outer / jy-inner (3D: jx, jy, jz) with particles in sorted order.
+#include "common.h"
+#include "kernels.h"
+
+namespace prbf {
+
+// ---------------------------------------------------------------- bounding box
+__global__ void k_bbox_partial(long long n, const double* __restrict__ x, const double* __restrict__ y,
+                               const double* __restrict__ z, int dim, double* __restrict__ part) {
+  __shared__ double smin[3][256], smax[3][256];
+  int t = threadIdx.x;
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (long long p = (long long)blockIdx.x * blockDim.x + t; p < n; p += (long long)gridDim.x * blockDim.x) {
+    double v[3] = {x[p], y[p], dim == 3 ? z[p] : 0.0};
+    for (int d = 0; d < 3; ++d) { lo[d] = fmin(lo[d], v[d]); hi[d] = fmax(hi[d], v[d]); }
+  }
+  for (int d = 0; d < 3; ++d) { smin[d][t] = lo[d]; smax[d][t] = hi[d]; }
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s)
+      for (int d = 0; d < 3; ++d) {
+        smin[d][t] = fmin(smin[d][t], smin[d][t + s]);
+        smax[d][t] = fmax(smax[d][t], smax[d][t + s]);
+      }
+    __syncthreads();
+  }
+  if (t == 0)
+    for (int d = 0; d < 3; ++d) { part[blockIdx.x * 6 + d] = smin[d][0]; part[blockIdx.x * 6 + 3 + d] = smax[d][0]; }
+}
+
+__global__ void k_bbox_final(int nparts, const double* __restrict__ part, double* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int b = 0; b < nparts; ++b)
+    for (int d = 0; d < 3; ++d) { lo[d] = fmin(lo[d], part[b * 6 + d]); hi[d] = fmax(hi[d], part[b * 6 + 3 + d]); }
+  for (int d = 0; d < 3; ++d) { out[d] = lo[d]; out[3 + d] = hi[d]; }
+}
+
+void bbox(hipStream_t s, long long n, const double* x, const double* y, const double* z, int dim,
+          double* d_part /*>= 6*256*/, double* d_out /*6*/) {
+  int nb = (int)std::min<long long>(256, (n + 255) / 256);
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(256), 0, s, n, x, y, z, dim, d_part);
+  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, s, nb, d_part, d_out);
+}
+
+// ---------------------------------------------------------------- cell of each particle
+__global__ void k_cell_assign(long long n, const double* __restrict__ x, const double* __restrict__ y,
+                              const double* __restrict__ z, Grid g, int* __restrict__ cell,
+                              int* __restrict__ count, int* __restrict__ err) {
+  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  // (int)floor((x - xmin)/box), 2d/get_cluster.h:78-79 -- IEEE divide, no contraction.
+  double fx = floor((x[p] - g.xmin) / g.box);
+  double fy = floor((y[p] - g.ymin) / g.box);
+  double fz = g.dim == 3 ? floor((z[p] - g.zmin) / g.box) : 0.0;
+  // range test in FP64 first, so NaN and out-of-grid points are rejected
+  // before any conversion to int
+  if (!(fx >= 0.0 && fx < g.nx && fy >= 0.0 && fy < g.ny && fz >= 0.0 && fz < g.nz)) {
+    atomicOr(err, 1);
+    cell[p] = -1;
+    return;
+  }
+  int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+  int c = (int)cell_index(g, ix, iy, iz);
+  cell[p] = c;
+  atomicAdd(&count[c], 1);
+}
+
+// Exclusive scan of int counts into int64 offsets (single workgroup, chunked;
+// deterministic).  out has n+1 entries, out[n] = total.
+__global__ void k_scan_excl(long long n, const int* __restrict__ in, long long* __restrict__ out) {
+  __shared__ long long sh[1024];
+  __shared__ long long carry;
+  int t = threadIdx.x;
+  if (t == 0) carry = 0;
+  __syncthreads();
+  for (long long base = 0; base < n; base += 1024) {
+    long long v = (base + t < n) ? in[base + t] : 0;
+    sh[t] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      long long a = (t >= off) ? sh[t - off] : 0;
+      __syncthreads();
+      sh[t] += a;
+      __syncthreads();
+    }
+    if (base + t < n) out[base + t] = carry + sh[t] - v;
+    __syncthreads();
+    if (t == 1023) carry += sh[1023];
+    __syncthreads();
+  }
+  if (t == 0) out[n] = carry;
+}
+
+void scan_excl(hipStream_t s, long long n, const int* in, long long* out) {
+  hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, s, n, in, out);
+}
+
+// Unordered placement into cell slots, then per-cell ordering by original
+// index: the result equals the reference's stable counting sort
+// (2d/get_cluster.h:123-130: slots filled in increasing particle index).
+__global__ void k_cell_scatter(long long n, const int* __restrict__ cell, int* __restrict__ cursor,
+                               int* __restrict__ perm) {
+  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  int c = cell[p];
+  if (c < 0) return;  // rejected point; the caller reports the error
+  int s = atomicAdd(&cursor[c], 1);
+  perm[s] = (int)p;
+}
+
+__global__ void k_cell_sortseg(long long ncell, const long long* __restrict__ start, int* __restrict__ perm) {
+  long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  long long a = start[c], b = start[c + 1];
+  for (long long i = a + 1; i < b; ++i) {
+    int v = perm[i];
+    long long j = i - 1;
+    while (j >= a && perm[j] > v) { perm[j + 1] = perm[j]; --j; }
+    perm[j + 1] = v;
+  }
+}
+
+__global__ void k_cell_ranges(long long ncell, const long long* __restrict__ start, int* __restrict__ sta,
+                              int* __restrict__ end) {
+  long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  sta[c] = (int)start[c];
+  end[c] = (int)start[c + 1] - 1;  // iend = ista + count - 1 (2d/get_cluster.h:100-107)
+}
+
+__global__ void k_gather_sorted(long long n, const int* __restrict__ perm, const double* __restrict__ x,
+                                const double* __restrict__ y, const double* __restrict__ z,
+                                double* __restrict__ xs, double* __restrict__ ys, double* __restrict__ zs) {
+  long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int p = perm[s];
+  xs[s] = x[p];
+  ys[s] = y[p];
+  if (zs) zs[s] = z[p];
+}
+
+void sort_points(hipStream_t st, const Grid& g, long long n, const double* x, const double* y, const double* z,
+                 int* d_cell, int* d_count, long long* d_start, int* d_cursor, int* d_perm,
+                 int* d_sta, int* d_end, double* xs, double* ys, double* zs, int* d_err) {
+  int nbp = (int)((n + 255) / 256);
+  int nbc = (int)((g.ncell + 255) / 256);
+  PRBF_HIP(hipMemsetAsync(d_count, 0, sizeof(int) * g.ncell, st));
+  hipLaunchKernelGGL(k_cell_assign, dim3(nbp), dim3(256), 0, st, n, x, y, z, g, d_cell, d_count, d_err);
+  scan_excl(st, g.ncell, d_count, d_start);
+  hipLaunchKernelGGL(k_copy_ll_to_int, dim3(nbc), dim3(256), 0, st, g.ncell, d_start, d_cursor);
+  // slots of rejected points stay 0 (a valid index) until the caller reports the error
+  PRBF_HIP(hipMemsetAsync(d_perm, 0, sizeof(int) * n, st));
+  hipLaunchKernelGGL(k_cell_scatter, dim3(nbp), dim3(256), 0, st, n, d_cell, d_cursor, d_perm);
+  hipLaunchKernelGGL(k_cell_sortseg, dim3(nbc), dim3(256), 0, st, g.ncell, d_start, d_perm);
+  hipLaunchKernelGGL(k_cell_ranges, dim3(nbc), dim3(256), 0, st, g.ncell, d_start, d_sta, d_end);
+  hipLaunchKernelGGL(k_gather_sorted, dim3(nbp), dim3(256), 0, st, n, d_perm, x, y, z, xs, ys, zs);
+}
+
+__global__ void k_copy_ll_to_int(long long n, const long long* __restrict__ a, int* __restrict__ b) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (int)a[i];
+}
+
+// ---------------------------------------------------------------- cluster lists
+// Visits the list of cluster c in the reference order and calls emit(j) for
+// each member.  `core_sta/core_end` are the cluster's own particles of the
+// listed set; the candidate cells come from `sta/end` of the same set.
+template <int DIM, class Emit>
+__device__ inline void visit_list(const Grid& g, long long c, int reach, double len,
+                                  const int* __restrict__ tsta, const int* __restrict__ tend,
+                                  const int* __restrict__ sta, const int* __restrict__ end,
+                                  const double* __restrict__ xs, const double* __restrict__ ys,
+                                  const double* __restrict__ zs, Emit&& emit) {
+  if (tsta[c] > tend[c]) return;  // cluster without targets: empty list (get_buffer.h:79-81)
+  int ix, iy, iz;
+  cell_coords(g, c, &ix, &iy, &iz);
+  double xc = cell_centre(g.xmin, ix, g.box);
+  double yc = cell_centre(g.ymin, iy, g.box);
+  double zc = DIM == 3 ? cell_centre(g.zmin, iz, g.box) : 0.0;
+  for (int j = sta[c]; j <= end[c]; ++j) emit(j);
+  int x0 = max(0, ix - reach), x1 = min(g.nx - 1, ix + reach);
+  int y0 = max(0, iy - reach), y1 = min(g.ny - 1, iy + reach);
+  int z0 = DIM == 3 ? max(0, iz - reach) : 0, z1 = DIM == 3 ? min(g.nz - 1, iz + reach) : 0;
+  for (int jx = x0; jx <= x1; ++jx)
+    for (int jy = y0; jy <= y1; ++jy)
+      for (int jz = z0; jz <= z1; ++jz) {
+        if (jx == ix && jy == iy && jz == iz) continue;
+        long long jc = cell_index(g, jx, jy, jz);
+        for (int j = sta[jc]; j <= end[jc]; ++j) {
+          bool in = fabs(xs[j] - xc) < len && fabs(ys[j] - yc) < len;
+          if (DIM == 3) in = in && fabs(zs[j] - zc) < len;
+          if (in) emit(j);
+        }
+      }
+}
+
+template <int DIM>
+__global__ void k_list_count(Grid g, int reach, double len, const int* __restrict__ tsta,
+                             const int* __restrict__ tend, const int* __restrict__ sta,
+                             const int* __restrict__ end, const double* __restrict__ xs,
+                             const double* __restrict__ ys, const double* __restrict__ zs,
+                             int* __restrict__ count) {
+  long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.ncell) return;
+  int k = 0;
+  visit_list<DIM>(g, c, reach, len, tsta, tend, sta, end, xs, ys, zs, [&](int) { ++k; });
+  count[c] = k;
+}
+
+template <int DIM>
+__global__ void k_list_fill(Grid g, int reach, double len, const int* __restrict__ tsta,
+                            const int* __restrict__ tend, const int* __restrict__ sta,
+                            const int* __restrict__ end, const double* __restrict__ xs,
+                            const double* __restrict__ ys, const double* __restrict__ zs,
+                            const long long* __restrict__ ptr, int* __restrict__ idx) {
+  long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.ncell) return;
+  long long k = ptr[c];
+  visit_list<DIM>(g, c, reach, len, tsta, tend, sta, end, xs, ys, zs, [&](int j) { idx[k++] = j; });
+}
+
+void list_count(hipStream_t st, const Grid& g, int reach, double len, const int* tsta, const int* tend,
+                const int* sta, const int* end, const double* xs, const double* ys, const double* zs,
+                int* count) {
+  int nb = (int)((g.ncell + 127) / 128);
+  if (g.dim == 2)
+    hipLaunchKernelGGL(k_list_count<2>, dim3(nb), dim3(128), 0, st, g, reach, len, tsta, tend, sta, end, xs, ys, zs, count);
+  else
+    hipLaunchKernelGGL(k_list_count<3>, dim3(nb), dim3(128), 0, st, g, reach, len, tsta, tend, sta, end, xs, ys, zs, count);
+}
+
+void list_fill(hipStream_t st, const Grid& g, int reach, double len, const int* tsta, const int* tend,
+               const int* sta, const int* end, const double* xs, const double* ys, const double* zs,
+               const long long* ptr, int* idx) {
+  int nb = (int)((g.ncell + 127) / 128);
+  if (g.dim == 2)
+    hipLaunchKernelGGL(k_list_fill<2>, dim3(nb), dim3(128), 0, st, g, reach, len, tsta, tend, sta, end, xs, ys, zs, ptr, idx);
+  else
+    hipLaunchKernelGGL(k_list_fill<3>, dim3(nb), dim3(128), 0, st, g, reach, len, tsta, tend, sta, end, xs, ys, zs, ptr, idx);
+}
+
+}  // namespace prbf

@@ -1,0 +1,179 @@
+// Vector kernels of the Krylov solvers (GMRES / FGMRES), FP64.
+//
+// Reductions use a fixed partition of the index range into kRedBlocks slices
+// and combine the slice partials in slice order, so every dot product and norm
+// is identical from run to run.  The multi-dot reads w once for all k basis
+// vectors (classical Gram-Schmidt, the PETSc default orthogonalisation), and
+// the multi-AXPY fuses the norm of the updated vector into the same pass.
+#include "common.h"
+#include "kernels.h"
+
+namespace prbf {
+
+constexpr int kRT = 256;     // threads per reduction block
+constexpr int kMaxK = 64;    // max basis vectors per fused pass
+
+__device__ inline double block_sum(double v, double* sh) {
+  int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int s = kRT / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] += sh[t + s];
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ inline void slice(long long n, long long* a, long long* b) {
+  long long len = (n + kRedBlocks - 1) / kRedBlocks;
+  *a = min(n, (long long)blockIdx.x * len);
+  *b = min(n, *a + len);
+}
+
+// part[blk * kMaxK + j] = sum over the slice of w * V_j, for j < k
+__global__ __launch_bounds__(kRT) void k_mdot(long long n, const double* __restrict__ w,
+                                              const double* __restrict__ V, long long ldv, int k,
+                                              double* __restrict__ part) {
+  __shared__ double sh[kRT];
+  long long a, b;
+  slice(n, &a, &b);
+  for (int j0 = 0; j0 < k; j0 += 8) {
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int kk = min(8, k - j0);
+    for (long long i = a + threadIdx.x; i < b; i += kRT) {
+      double wi = w[i];
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (j < kk) acc[j] = fma(wi, V[(long long)(j0 + j) * ldv + i], acc[j]);
+    }
+    for (int j = 0; j < kk; ++j) {
+      double s = block_sum(acc[j], sh);
+      if (threadIdx.x == 0) part[(long long)blockIdx.x * kMaxK + j0 + j] = s;
+    }
+  }
+}
+
+__global__ void k_reduce_parts(int k, const double* __restrict__ part, double* __restrict__ out) {
+  int j = threadIdx.x;
+  if (j >= k) return;
+  double s = 0.0;
+  for (int b = 0; b < kRedBlocks; ++b) s += part[(long long)b * kMaxK + j];
+  out[j] = s;
+}
+
+void mdot(hipStream_t st, long long n, const double* w, const double* V, long long ldv, int k, double* part,
+          double* out) {
+  hipLaunchKernelGGL(k_mdot, dim3(kRedBlocks), dim3(kRT), 0, st, n, w, V, ldv, k, part);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(kMaxK), 0, st, k, part, out);
+}
+
+// w += sum_j alpha*coef[j] V_j, then part = partial sums of w^2
+__global__ __launch_bounds__(kRT) void k_maxpy_norm2(long long n, double* __restrict__ w,
+                                                     const double* __restrict__ V, long long ldv, int k,
+                                                     const double* __restrict__ coef, double alpha,
+                                                     double* __restrict__ part) {
+  __shared__ double sh[kRT];
+  __shared__ double cf[kMaxK];
+  if (threadIdx.x < k) cf[threadIdx.x] = alpha * coef[threadIdx.x];
+  __syncthreads();
+  long long a, b;
+  slice(n, &a, &b);
+  double acc = 0.0;
+  for (long long i = a + threadIdx.x; i < b; i += kRT) {
+    double s = w[i];
+    for (int j = 0; j < k; ++j) s = fma(cf[j], V[(long long)j * ldv + i], s);
+    w[i] = s;
+    acc = fma(s, s, acc);
+  }
+  double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[(long long)blockIdx.x * kMaxK] = s;
+}
+
+void maxpy_norm2(hipStream_t st, long long n, double* w, const double* V, long long ldv, int k,
+                 const double* coef, double alpha, double* part, double* out_norm2) {
+  hipLaunchKernelGGL(k_maxpy_norm2, dim3(kRedBlocks), dim3(kRT), 0, st, n, w, V, ldv, k, coef, alpha, part);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(kMaxK), 0, st, 1, part, out_norm2);
+}
+
+__global__ __launch_bounds__(kRT) void k_norm2sq(long long n, const double* __restrict__ v,
+                                                 double* __restrict__ part) {
+  __shared__ double sh[kRT];
+  long long a, b;
+  slice(n, &a, &b);
+  double acc = 0.0;
+  for (long long i = a + threadIdx.x; i < b; i += kRT) acc = fma(v[i], v[i], acc);
+  double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[(long long)blockIdx.x * kMaxK] = s;
+}
+
+void norm2sq(hipStream_t st, long long n, const double* v, double* part, double* out) {
+  hipLaunchKernelGGL(k_norm2sq, dim3(kRedBlocks), dim3(kRT), 0, st, n, v, part);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(kMaxK), 0, st, 1, part, out);
+}
+
+__global__ void k_scale(long long n, double* __restrict__ v, double s) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] *= s;
+}
+__global__ void k_scale_copy(long long n, const double* __restrict__ a, double* __restrict__ b, double s) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i] * s;
+}
+__global__ void k_sub(long long n, const double* __restrict__ a, const double* __restrict__ b,
+                      double* __restrict__ r) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) r[i] = a[i] - b[i];
+}
+__global__ void k_update_x(long long n, double* __restrict__ x, const double* __restrict__ V, long long ldv,
+                           int k, const double* __restrict__ y) {
+  __shared__ double cf[kMaxK];
+  if (threadIdx.x < k) cf[threadIdx.x] = y[threadIdx.x];
+  __syncthreads();
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int j = 0; j < k; ++j) s = fma(cf[j], V[(long long)j * ldv + i], s);
+  x[i] += s;
+}
+__global__ void k_gather(long long n, const int* __restrict__ perm, const double* __restrict__ src,
+                         double* __restrict__ dst) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[perm[i]];
+}
+__global__ void k_scatter(long long n, const int* __restrict__ perm, const double* __restrict__ src,
+                          double* __restrict__ dst) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[perm[i]] = src[i];
+}
+__global__ void k_fill(long long n, double* __restrict__ v, double a) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = a;
+}
+
+static inline dim3 g1(long long n) { return dim3((unsigned)((n + 255) / 256)); }
+
+void scale(hipStream_t st, long long n, double* v, double s) {
+  hipLaunchKernelGGL(k_scale, g1(n), dim3(256), 0, st, n, v, s);
+}
+void scale_copy(hipStream_t st, long long n, const double* src, double* dst, double s) {
+  hipLaunchKernelGGL(k_scale_copy, g1(n), dim3(256), 0, st, n, src, dst, s);
+}
+void xpay_sub(hipStream_t st, long long n, const double* b, const double* ax, double* r) {
+  hipLaunchKernelGGL(k_sub, g1(n), dim3(256), 0, st, n, b, ax, r);
+}
+void update_x(hipStream_t st, long long n, double* x, const double* V, long long ldv, int k, const double* y) {
+  hipLaunchKernelGGL(k_update_x, g1(n), dim3(256), 0, st, n, x, V, ldv, k, y);
+}
+void gather(hipStream_t st, long long n, const int* perm, const double* src, double* dst) {
+  hipLaunchKernelGGL(k_gather, g1(n), dim3(256), 0, st, n, perm, src, dst);
+}
+void scatter(hipStream_t st, long long n, const int* perm, const double* src, double* dst) {
+  hipLaunchKernelGGL(k_scatter, g1(n), dim3(256), 0, st, n, perm, src, dst);
+}
+void fill(hipStream_t st, long long n, double* v, double a) {
+  hipLaunchKernelGGL(k_fill, g1(n), dim3(256), 0, st, n, v, a);
+}
+
+}  // namespace prbf

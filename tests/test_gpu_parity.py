@@ -1,0 +1,196 @@
+"""GPU parity: the HIP library against the CPU oracle on the same seeded
+inputs.  Integer work (permutation, cell ranges, buffer and truncation lists)
+must be bit-identical; the FP64 matvec within 1e-12 relative; the plain solve
+(GMRES(30), left RASM) within 1e-10 relative in gamma, with the same
+iteration count, the same final residual to 1e-10 of the initial one, and
+the same analytic-field error to 1e-10."""
+
+import sys
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+from petrbf_amd.rbf import Context, rbf_interpolation, vorticity_evaluation
+from tests import _brute as B
+from tests.cases import all_cases, lattice
+
+pytestmark = pytest.mark.gpu
+
+CASES = all_cases()
+
+
+def _announce(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _pair(case):
+    dim, sigma, nb, sb, st, tgt, src = case
+    z = tgt[2] if dim == 3 else None
+    if src is None:
+        o = Oracle(dim, sigma, nb, sb, st, tgt[0], tgt[1], z)
+    else:
+        o = Oracle(dim, sigma, nb, sb, st, tgt[0], tgt[1], z, src[0], src[1], src[2] if dim == 3 else None)
+    ctx = Context(dim, sigma, nb, sb, st)
+    if src is None:
+        ctx.set_points(tgt[0], tgt[1], z)
+    else:
+        ctx.set_points(tgt[0], tgt[1], z, src[0], src[1], src[2] if dim == 3 else None)
+    return o, ctx
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_lists_bit_exact(name):
+    _announce(f"lists {name}")
+    o, ctx = _pair(CASES[name])
+    g = o.grid()
+    info = ctx.info()
+    assert (info["nx"], info["ny"], info["nz"]) == (g["nx"], g["ny"], g["nz"])
+    assert (info["nb_buffer"], info["nb_trunc"]) == (g["nb_buffer"], g["nb_trunc"])
+    assert info["box"] == g["box"] and info["xmin"] == g["xmin"] and info["ymin"] == g["ymin"]
+    for a, b in zip(ctx.permutation(), o.perm()):
+        np.testing.assert_array_equal(a, b)
+    for a, b in zip(ctx.cells(), o.cells()):
+        np.testing.assert_array_equal(a, b)
+    for a, b in zip(ctx.buffer_csr(), o.buffer_csr()):
+        np.testing.assert_array_equal(a, b)
+    for a, b in zip(ctx.trunc_csr(), o.trunc_csr()):
+        np.testing.assert_array_equal(a, b)
+    xs, ys, zs = ctx.sorted_points()
+    ox, oy, oz = o.sorted_coords()
+    np.testing.assert_array_equal(xs, ox)
+    np.testing.assert_array_equal(ys, oy)
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_matvec_parity(name):
+    _announce(f"matvec {name}")
+    o, ctx = _pair(CASES[name])
+    rng = np.random.default_rng(17)
+    for nrhs in (1, 3):
+        g = rng.standard_normal((nrhs, o.nj))
+        want = o.matvec(g)
+        got = ctx.matvec(g)
+        assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)
+    ctx.close()
+
+
+def test_evaluation_distinct_targets_matches_oracle_and_dense():
+    _announce("evaluation distinct targets")
+    dim, sigma, nb, sb, st, tgt, src = CASES["targets_sources"]
+    rng = np.random.default_rng(5)
+    gj = rng.standard_normal(src[0].shape[0])
+    got = vorticity_evaluation(tgt[0], tgt[1], src[0], src[1], gj, sigma, nb, sb, st)
+    ref = B.lists(dim, sigma, nb, sb, st, tgt, src)
+    A = B.dense_truncated(dim, sigma, ref)
+    w_sorted = (A @ gj[ref["perm_j"]].astype(np.longdouble)).astype(np.float64)
+    want = np.empty_like(w_sorted)
+    want[ref["perm_i"]] = w_sorted
+    assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)
+
+
+def test_rasm_core_rows_are_inverse_rows():
+    _announce("rasm core rows")
+    p = lattice(24)
+    ctx = Context.from_problem(p)
+    ctx.set_points(p.x, p.y)
+    ctx.rasm_setup()
+    x, y, _ = ctx.sorted_points()
+    ista, iend, _, _ = ctx.cells()
+    bl = B.csr_to_lists(*ctx.buffer_csr())
+    checked = 0
+    for c in range(ctx.ncell):
+        b = bl[c]
+        if b.size == 0:
+            continue
+        r2 = (x[b, None] - x[None, b]) ** 2 + (y[b, None] - y[None, b]) ** 2
+        A = np.exp(-r2 / (2 * p.sigma ** 2)) / (2 * np.pi * p.sigma ** 2)
+        T = iend[c] - ista[c] + 1
+        M = ctx.core_rows(c)
+        assert M.shape == (T, b.size)
+        # M A restricted to the core rows is the identity's core rows
+        R = M @ A
+        E = np.eye(b.size)[:T]
+        assert np.abs(R - E).max() < 1e-6, c
+        checked += 1
+    assert checked == int(((iend - ista) >= 0).sum())
+    ctx.close()
+
+
+def test_rasm_apply_matches_oracle_lu():
+    _announce("rasm apply")
+    p = lattice(40)
+    o = Oracle.from_problem(p)
+    o.rasm_setup()
+    ctx = Context.from_problem(p)
+    ctx.set_points(p.x, p.y)
+    ctx.rasm_setup()
+    rng = np.random.default_rng(3)
+    r = rng.standard_normal(p.n)
+    want = o.rasm_apply(r)
+    got = ctx.rasm_apply(r)
+    assert np.linalg.norm(got - want) <= 1e-8 * np.linalg.norm(want)
+    ctx.close()
+
+
+@pytest.mark.parametrize("n_side", [100, 160])
+def test_plain_solve_parity(n_side):
+    """The solve workload (100x100 is the N = 10^4 CPU configuration)."""
+    _announce(f"plain solve parity {n_side}")
+    p = lattice(n_side, seed=0x5EED0001)
+    o = Oracle.from_problem(p)
+    pi, _ = o.perm()
+    ref = o.solve_sorted(p.w[0][pi], rtol=1e-10, restart=30, max_it=1000)
+    g_cpu = np.empty(p.n)
+    g_cpu[pi] = ref["gamma"]
+    ctx = Context.from_problem(p)
+    ctx.set_points(p.x, p.y)
+    g_gpu, info = ctx.solve(p.w[0], ksp="gmres", pc="rasm", rtol=1e-10, restart=30, max_it=1000)
+    _announce(f"its gpu {info.its} cpu {ref['its']}; setup {info.t_setup_s:.3f}s solve {info.t_solve_s:.3f}s")
+    assert info.reason == 2 and ref["reason"] == 2
+    assert info.its == ref["its"]
+    assert np.linalg.norm(g_gpu - g_cpu) <= 1e-10 * np.linalg.norm(g_cpu)
+    r0 = ref["hist"][0]
+    assert abs(info.rnorm0 - r0) <= 1e-12 * r0
+    assert abs(info.rnorm - ref["hist"][-1]) <= 1e-10 * r0
+    # analytic-field error: evaluate the strengths at the particles (2d/main.cxx:125-133)
+    w_gpu = ctx.evaluate(g_gpu)
+    w_cpu = np.empty(p.n)
+    w_cpu[pi] = o.matvec(ref["gamma"])
+    e = p.w[0]
+    err_gpu = np.linalg.norm(w_gpu - e) / np.linalg.norm(e)
+    err_cpu = np.linalg.norm(w_cpu - e) / np.linalg.norm(e)
+    _announce(f"analytic-field error gpu {err_gpu:.3e} cpu {err_cpu:.3e}")
+    assert abs(err_gpu - err_cpu) <= 1e-10
+    assert np.linalg.norm(w_gpu - w_cpu) <= 1e-10 * np.linalg.norm(w_cpu)
+    ctx.close()
+
+
+def test_rbf_interpolation_entry_point_and_fgmres():
+    _announce("rbf_interpolation + fgmres")
+    p = lattice(60)
+    g, its = rbf_interpolation(p.x, p.y, p.w[0], p.sigma, p.nsigma_box, p.sigma_buffer, p.sigma_trunc)
+    g2, its2 = rbf_interpolation(p.x, p.y, p.w[0], p.sigma, p.nsigma_box, p.sigma_buffer, p.sigma_trunc,
+                                 ksp="fgmres")
+    assert its > 0 and its2 > 0
+    o = Oracle.from_problem(p)
+    pi, _ = o.perm()
+    for gamma in (g, g2):
+        w = np.empty(p.n)
+        w[pi] = o.matvec(gamma[pi])
+        assert np.linalg.norm(w - p.w[0]) <= 1e-8 * np.linalg.norm(p.w[0])
+
+
+def test_errors_are_reported_not_crashed():
+    _announce("error paths")
+    ctx = Context(2, 0.1, 3.2, 16.0, 13.2)
+    with pytest.raises(RuntimeError):
+        ctx.rasm_setup()                       # before set_points
+    with pytest.raises(RuntimeError):
+        ctx.set_points(np.array([0.0, np.nan]), np.array([0.0, 0.0]))
+    ctx.set_points(np.array([0.0, 0.5]), np.array([0.0, 0.1]))
+    with pytest.raises(RuntimeError):
+        ctx.matvec(np.zeros((4, 2)))           # nrhs out of range
+    ctx.close()
