@@ -75,7 +75,20 @@ def main():
     ap.add_argument("--n-side", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=10)
+    ap.add_argument("--mode", choices=["plain", "mixed", "both"], default="both")
+    ap.add_argument("--chol-waves", type=int, default=0, help="4 or 8 wavefronts per block factorisation")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a device copy of 1 GiB five times (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda:0").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8, "reps": 5}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1"}))
         sys.exit(2)
@@ -91,28 +104,48 @@ def main():
     y = torch.from_numpy(p.y).to(dev)
     w = torch.from_numpy(p.w[0]).to(dev)
     gamma = torch.empty_like(w)
+    wv = torch.empty_like(w)
     stream = torch.cuda.Stream(device=dev)
     ctx = Context.from_problem(p)
     ctx.set_stream(stream.cuda_stream)
+    if args.chol_waves:
+        ctx.set_tuning(chol_waves=args.chol_waves)
 
-    def step():
-        ctx.set_points_dev(x, y)
-        return ctx.solve_dev(w, gamma, ksp="gmres", pc="rasm", rtol=1e-10, restart=30, max_it=1000)
+    modes = {"plain": ("gmres", "rasm"), "mixed": ("fgmres", "rasm32")}
+    run = ["plain", "mixed"] if args.mode == "both" else [args.mode]
+    results = {}
+    for mode in run:
+        ksp, pc = modes[mode]
 
-    for i in range(args.warmup):
-        inf = step()
-        log(f"warmup {i}: its {inf.its} reason {inf.reason} setup {inf.t_setup_s:.3f}s solve {inf.t_solve_s:.3f}s")
-    torch.cuda.synchronize()
-    infos = []
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        infos.append(step())
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    inf = infos[-1]
-    log(f"timed {args.steps} steps: {elapsed:.3f}s; its {inf.its} reason {inf.reason}")
+        def step():
+            ctx.set_points_dev(x, y)
+            return ctx.solve_dev(w, gamma, ksp=ksp, pc=pc, rtol=1e-10, restart=30, max_it=1000)
 
-    # ---- per-kernel timing with HIP events on the library's stream
+        for i in range(args.warmup):
+            inf = step()
+            log(f"{mode} warmup {i}: its {inf.its} reason {inf.reason} setup {inf.t_setup_s:.3f}s "
+                f"solve {inf.t_solve_s:.3f}s fallback {inf.n_fallback}")
+        torch.cuda.synchronize()
+        infos = []
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            infos.append(step())
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        inf = infos[-1]
+        # true FP64 residual of the returned strengths (evaluation = the same operator)
+        ctx.evaluate_dev(gamma, wv)
+        torch.cuda.synchronize()
+        true_rel = float(torch.linalg.norm(wv - w) / torch.linalg.norm(w))
+        log(f"{mode}: {args.steps} steps {elapsed:.3f}s; its {inf.its} reason {inf.reason} true rel resid {true_rel:.3e}")
+        results[mode] = dict(ms_per_step=elapsed / args.steps * 1e3, its=inf.its, reason=inf.reason,
+                             t_setup_s=inf.t_setup_s, t_solve_s=inf.t_solve_s, t_matvec_s=inf.t_matvec_s,
+                             n_matvec=inf.n_matvec, t_pc_s=inf.t_pc_s, n_pc=inf.n_pc,
+                             n_fallback=inf.n_fallback, true_rel_residual=true_rel)
+
+    # ---- per-kernel timing with HIP events on the library's stream (FP64 RASM set up last if plain ran)
+    ctx.set_points_dev(x, y)
+    ctx.rasm_setup("rasm")
     info = ctx.info()
     ista, iend, _, _ = ctx.cells()
     T = (iend - ista + 1).astype(np.int64)
@@ -124,26 +157,31 @@ def main():
     g_s = torch.randn(p.n, dtype=torch.float64, device=dev)
     out = torch.empty_like(g_s)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    e0.record(stream)
-    for _ in range(args.kernel_reps):
-        ctx.matvec_dev(g_s, out)
-    e1.record(stream)
-    e1.synchronize()
-    t_mv = e0.elapsed_time(e1) / 1e3 / args.kernel_reps
-    e0.record(stream)
-    for _ in range(args.kernel_reps):
-        ctx.rasm_apply_dev(g_s, out)
-    e1.record(stream)
-    e1.synchronize()
-    t_ap = e0.elapsed_time(e1) / 1e3 / args.kernel_reps
-    m_bytes = int((nb * T).sum()) * 8
-    apply_bytes = m_bytes + int(nb.sum()) * 4 + p.n * 8 * 2 + (info["ncell"] + 1) * 8 * 2
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(args.kernel_reps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / 1e3 / args.kernel_reps
+
+    t_mv = timed(lambda: ctx.matvec_dev(g_s, out))
+    t_ap = timed(lambda: ctx.rasm_apply_dev(g_s, out))
+    ctx.rasm_setup("rasm32")
+    t_ap32 = timed(lambda: ctx.rasm_apply_dev(g_s, out))
+    m_elems = int((nb * T).sum())
+    side_bytes = int(nb.sum()) * 4 + p.n * 8 * 2 + (info["ncell"] + 1) * 8 * 2
+    apply_bytes = m_elems * 8 + side_bytes
+    apply32_bytes = m_elems * 4 + side_bytes
     mv_flops = FLOP_PER_PAIR_2D * pairs
     log(f"matvec {t_mv*1e3:.3f} ms ({pairs/t_mv:.3e} pairs/s, {mv_flops/t_mv/1e12:.2f} TFLOP/s); "
-        f"rasm apply {t_ap*1e3:.3f} ms ({apply_bytes/t_ap/1e9:.0f} GB/s)")
+        f"rasm apply fp64 {t_ap*1e3:.3f} ms ({apply_bytes/t_ap/1e9:.0f} GB/s), "
+        f"fp32 {t_ap32*1e3:.3f} ms ({apply32_bytes/t_ap32/1e9:.0f} GB/s)")
 
-    ms_per_step = elapsed / args.steps * 1e3
+    head = "mixed" if "mixed" in results else run[0]
+    ms_per_step = results[head]["ms_per_step"]
     res = {
         "metric": "N=4M 2D solve time to rtol 1e-10 (learned/FP32 vs plain); Gaussian pair-evals/s",
         "value": ms_per_step / 1e3,
@@ -155,27 +193,32 @@ def main():
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if head == "plain" else "f64 (Krylov, matvec) + f32 (RASM blocks)",
         "data": "synthetic",
-        "config": {"workload": f"2D jittered lattice {args.n_side}x{args.n_side} (N={p.n}), plain FP64 "
-                               "GMRES(30) + RASM (4x4-spacing cells, 2-cell buffer, 5 sigma truncation), rtol 1e-10",
-                   "mode": "plain", "n": p.n, "ncell": info["ncell"], "its": inf.its, "reason": inf.reason},
-        "solve": {"its": inf.its, "t_setup_s": inf.t_setup_s, "t_solve_s": inf.t_solve_s,
-                  "t_matvec_s": inf.t_matvec_s, "n_matvec": inf.n_matvec, "t_pc_s": inf.t_pc_s,
-                  "n_pc": inf.n_pc},
+        "config": {"workload": f"2D jittered lattice {args.n_side}x{args.n_side} (N={p.n}), 4x4-spacing cells, "
+                               "2-cell buffer, 5 sigma truncation, Lamb-Oseen RHS, rtol 1e-10",
+                   "mode": head, "n": p.n, "ncell": info["ncell"]},
+        "modes": results,
+        "speedup_mixed_vs_plain": (results["plain"]["ms_per_step"] / results["mixed"]["ms_per_step"]
+                                   if len(results) == 2 else None),
         "pair_evals_per_s": pairs / t_mv,
         "pairs_per_matvec": pairs,
-        "roofline": {"bound": "hbm", "kernel": "k_rasm_apply_fp64", "achieved": apply_bytes / t_ap / 1e9,
+        "roofline": {"bound": "hbm", "kernel": "k_rasm_apply<16,double>", "achieved": apply_bytes / t_ap / 1e9,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": apply_bytes / t_ap / 1e9 / PEAK_HBM_GBS,
                      "traffic": None, "avg_ms": t_ap * 1e3, "bytes_per_launch": apply_bytes},
+        "roofline_apply_fp32": {"bound": "hbm", "kernel": "k_rasm_apply<16,float>",
+                                "achieved": apply32_bytes / t_ap32 / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                "frac": apply32_bytes / t_ap32 / 1e9 / PEAK_HBM_GBS, "avg_ms": t_ap32 * 1e3,
+                                "bytes_per_launch": apply32_bytes},
         "roofline_matvec": {"bound": "fp64-valu", "kernel": "k_matvec_wave", "achieved": mv_flops / t_mv / 1e12,
                             "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                             "frac": mv_flops / t_mv / 1e12 / PEAK_FP64_TFLOPS, "avg_ms": t_mv * 1e3,
                             "flops_per_launch": mv_flops},
     }
+    inf_its = results.get("plain", results[head])["its"]
     if not args.no_cpu:
         log("cpu baseline (oracle, bounded sample)")
-        res["cpu_baseline"] = cpu_baseline(p, inf.its)
+        res["cpu_baseline"] = cpu_baseline(p, inf_its)
         log(f"cpu baseline est {res['cpu_baseline']['value']:.1f}s on {res['cpu_baseline']['cores']} threads")
     print(json.dumps(res), flush=True)
 

@@ -55,7 +55,11 @@ typedef struct {
 } prbf_params;
 
 typedef enum { PRBF_KSP_GMRES = 0, PRBF_KSP_FGMRES = 1 } prbf_ksp;
-typedef enum { PRBF_PC_NONE = 0, PRBF_PC_RASM_FP64 = 1 } prbf_pc;
+/* RASM_FP64: exact FP64 block solves (the reference's -sub_pc_type lu).
+ * RASM_FP32: the same blocks factorised and solved in FP32 on the matrix
+ * cores, applied with FP64 accumulation; meant for FGMRES. */
+typedef enum { PRBF_PC_NONE = 0, PRBF_PC_RASM_FP64 = 1, PRBF_PC_RASM_FP32 = 2 } prbf_pc;
+typedef enum { PRBF_TUNE_CHOL_WAVES = 1 } prbf_tune_key;
 
 /* Solver options; prbf_solve_opts_default() gives the reference's run line
  * (-ksp_type gmres default restart 30, -pc_type asm -sub_pc_type lu,
@@ -81,6 +85,7 @@ typedef struct {
   double t_pc_s;     /* summed device time of the preconditioner applies */
   int n_matvec;
   int n_pc;
+  int n_fallback;    /* RASM_FP32: blocks redone in FP64 after an FP32 breakdown */
 } prbf_solve_info;
 
 /* ---- lifetime -------------------------------------------------------- */
@@ -138,11 +143,17 @@ prbf_status prbf_evaluate_dev(prbf_ctx* ctx, int nrhs, const double* gj, double*
  * PETSc dense sub-factorisation, for the subdomains set up by
  * PCASMSetLocalSubdomains (2d/rbf_interpolation.cxx:213-229). */
 prbf_status prbf_rasm_setup(prbf_ctx* ctx);
+/* Same for a chosen prbf_pc (RASM_FP64 or RASM_FP32).  No reference
+ * counterpart for FP32: it is the mixed-precision variant of the same PC. */
+prbf_status prbf_rasm_setup_pc(prbf_ctx* ctx, int pc);
+/* Tuning knobs of the device kernels (PRBF_TUNE_CHOL_WAVES: 4 or 8
+ * wavefronts per block-factorisation workgroup).  No reference counterpart. */
+prbf_status prbf_set_tuning(prbf_ctx* ctx, int key, int value);
 /* PCApply_ASM (restrict): z = sum_c P_core A_c^{-1} R_buffer r, sorted order. */
 prbf_status prbf_rasm_apply(prbf_ctx* ctx, const double* r_sorted, double* z_sorted);
 prbf_status prbf_rasm_apply_dev(prbf_ctx* ctx, const double* r_sorted, double* z_sorted);
-/* Core rows of A_c^{-1} for cluster c: M[b*T + t], b < n buffer slots, t < T
- * core members; n and T returned.  Pass M == NULL to query the sizes. */
+/* Core rows of A_c^{-1} for cluster c: M[t*n + b], t < T core members,
+ * b < n buffer slots; n and T returned.  Pass M == NULL to query the sizes. */
 prbf_status prbf_rasm_get_core_rows(prbf_ctx* ctx, int64_t c, double* M, int32_t* n, int32_t* T);
 
 /* ---- solve -------------------------------------------------------------

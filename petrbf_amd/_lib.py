@@ -12,7 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpetrbf.so")
 
 PRBF_KSP_GMRES, PRBF_KSP_FGMRES = 0, 1
-PRBF_PC_NONE, PRBF_PC_RASM_FP64 = 0, 1
+PRBF_PC_NONE, PRBF_PC_RASM_FP64, PRBF_PC_RASM_FP32 = 0, 1, 2
+PRBF_TUNE_CHOL_WAVES = 1
 
 STATUS = {0: "PRBF_OK", 1: "PRBF_ERR_ARG", 2: "PRBF_ERR_HIP", 3: "PRBF_ERR_NOMEM",
           4: "PRBF_ERR_STATE", 5: "PRBF_ERR_FACTOR", 6: "PRBF_ERR_OUTSIDE", 7: "PRBF_ERR_LIMIT"}
@@ -23,7 +24,7 @@ EXPORTS = [
     "prbf_synchronize", "prbf_set_points", "prbf_set_points_dev", "prbf_get_info",
     "prbf_get_permutation", "prbf_get_cells", "prbf_get_buffer_csr", "prbf_get_trunc_csr",
     "prbf_get_sorted_points", "prbf_matvec", "prbf_matvec_dev", "prbf_evaluate", "prbf_evaluate_dev",
-    "prbf_rasm_setup", "prbf_rasm_apply", "prbf_rasm_apply_dev", "prbf_rasm_get_core_rows",
+    "prbf_rasm_setup", "prbf_rasm_setup_pc", "prbf_set_tuning", "prbf_rasm_apply", "prbf_rasm_apply_dev", "prbf_rasm_get_core_rows",
     "prbf_solve", "prbf_solve_dev",
 ]
 
@@ -41,7 +42,7 @@ class PrbfSolveOpts(C.Structure):
 class PrbfSolveInfo(C.Structure):
     _fields_ = [("its", C.c_int), ("reason", C.c_int), ("rnorm0", C.c_double), ("rnorm", C.c_double),
                 ("t_setup_s", C.c_double), ("t_solve_s", C.c_double), ("t_matvec_s", C.c_double),
-                ("t_pc_s", C.c_double), ("n_matvec", C.c_int), ("n_pc", C.c_int)]
+                ("t_pc_s", C.c_double), ("n_matvec", C.c_int), ("n_pc", C.c_int), ("n_fallback", C.c_int)]
 
 
 class PrbfError(RuntimeError):
@@ -82,6 +83,8 @@ def load():
     for f in ("prbf_matvec", "prbf_matvec_dev", "prbf_evaluate", "prbf_evaluate_dev"):
         getattr(L, f).argtypes = [ctx, i32, dp, dp]
     L.prbf_rasm_setup.argtypes = [ctx]
+    L.prbf_rasm_setup_pc.argtypes = [ctx, i32]
+    L.prbf_set_tuning.argtypes = [ctx, i32, i32]
     for f in ("prbf_rasm_apply", "prbf_rasm_apply_dev"):
         getattr(L, f).argtypes = [ctx, dp, dp]
     L.prbf_rasm_get_core_rows.argtypes = [ctx, i64, dp, C.POINTER(i32), C.POINTER(i32)]

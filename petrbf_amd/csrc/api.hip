@@ -82,6 +82,12 @@ struct prbf_ctx {
   double* pin = nullptr;  // pinned host scalars
   // Krylov
   DBuf<double> V, Z;
+  int chol_waves = 4;  // wavefronts per block-factorisation workgroup
+  int rasm_pc = 0;
+  int n_fallback = 0;
+  DBuf<float> M32;
+  DBuf<long long> fb;
+  DBuf<double> M64fb;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
   const double* xj_() const { return same ? xi.p : xj.p; }
@@ -263,79 +269,148 @@ void matvec_impl(prbf_ctx* c, int nrhs, const double* g, double* w) {
   PRBF_HIP(hipGetLastError());
 }
 
-constexpr int kMaxLdsBytes = 160 * 1024 - 4096;
+constexpr size_t kMaxLdsBytes = 160 * 1024;
 
-void rasm_setup_impl(prbf_ctx* c) {
+// Runs the block factorisation over all clusters in chunks bounded by a
+// workspace budget.  fp32: blocks in FP32 on the FP32 MFMA, core rows stored
+// in FP32, failed blocks listed in `failed` for an FP64 redo.
+template <typename Real>
+void factor_chunks(prbf_ctx* c, bool fp32, const std::vector<long long>& wsz, Real* Mout, std::vector<int>* failed) {
+  const Grid& g = c->g;
+  const long long nc = g.ncell;
+  const int nw = c->chol_waves;
+  size_t free_b = 0, total_b = 0;
+  PRBF_HIP(hipMemGetInfo(&free_b, &total_b));
+  long long budget = (long long)std::min<size_t>(free_b / 3, (size_t)8 << 30) / (long long)sizeof(Real);
+  long long maxw = *std::max_element(wsz.begin(), wsz.end());
+  need(budget >= maxw, PRBF_ERR_NOMEM, "not enough device memory for one RASM block");
+  DBuf<Real> W;
+  DBuf<long long> wptr;
+  DBuf<int> status;
+  if (fp32) {
+    status.ensure(nc);
+    PRBF_HIP(hipMemsetAsync(status, 0, sizeof(int) * nc, c->st));
+  }
+  std::vector<long long> h_wptr;
+  const long long kMaxChunk = 1LL << 20;
+  wptr.ensure(std::min<long long>(nc, kMaxChunk) + 1);
+  long long k0 = 0;
+  while (k0 < nc) {
+    long long k1 = k0, used = 0;
+    h_wptr.clear();
+    int nmax = 0, tmax = 0;
+    while (k1 < nc && k1 - k0 < kMaxChunk) {
+      if (used + wsz[k1] > budget && k1 > k0) break;
+      h_wptr.push_back(used);
+      used += wsz[k1];
+      nmax = std::max<int>(nmax, (int)(c->h_bptr[k1 + 1] - c->h_bptr[k1]));
+      tmax = std::max(tmax, c->h_iend[k1] - c->h_ista[k1] + 1);
+      ++k1;
+    }
+    W.ensure((size_t)std::max<long long>(used, 1));
+    sync(c);  // the previous chunk has consumed wptr before it is overwritten
+    PRBF_HIP(hipMemcpy(wptr, h_wptr.data(), sizeof(long long) * h_wptr.size(), hipMemcpyHostToDevice));
+    if (fp32)
+      block_chol_fp32(c->st, g.dim, nw, k0, k1 - k0, nmax, tmax, c->bptr, c->bidx, c->ista, c->iend, c->xi, c->yi,
+                      c->zi, wptr, (float*)W.p, c->mptr, (float*)Mout, c->prm.sigma, status);
+    else
+      block_chol_fp64(c->st, g.dim, nw, k0, nullptr, k1 - k0, nmax, tmax, c->bptr, c->bidx, c->ista, c->iend,
+                      c->xi, c->yi, c->zi, wptr, (double*)W.p, c->mptr, (double*)Mout, c->prm.sigma, c->d_err);
+    PRBF_HIP(hipGetLastError());
+    k0 = k1;
+  }
+  sync(c);
+  if (fp32 && failed) {
+    std::vector<int> h(nc);
+    PRBF_HIP(hipMemcpy(h.data(), status, sizeof(int) * nc, hipMemcpyDeviceToHost));
+    failed->clear();
+    for (long long k = 0; k < nc; ++k)
+      if (h[k]) failed->push_back((int)k);
+  }
+}
+
+void rasm_setup_impl(prbf_ctx* c, int pc) {
   need(c->has_points, PRBF_ERR_STATE, "set_points first");
   need(c->same, PRBF_ERR_STATE, "RASM needs the source set equal to the target set");
+  need(pc == PRBF_PC_RASM_FP64 || pc == PRBF_PC_RASM_FP32, PRBF_ERR_ARG, "unknown preconditioner");
+  const bool fp32 = (pc == PRBF_PC_RASM_FP32);
   const Grid& g = c->g;
   const long long nc = g.ncell;
   c->h_mptr.assign(nc + 1, 0);
+  std::vector<long long> wsz(nc, 0);
   for (long long k = 0; k < nc; ++k) {
     long long n = c->h_bptr[k + 1] - c->h_bptr[k];
     long long T = n ? (c->h_iend[k] - c->h_ista[k] + 1) : 0;
     c->h_mptr[k + 1] = c->h_mptr[k] + n * T;
+    wsz[k] = n ? block_workspace((int)n, (int)T) : 0;
   }
-  size_t lds = sizeof(double) * ((size_t)3 * c->max_buffer + (size_t)std::max(16, c->max_core) * c->max_buffer);
-  need(lds <= (size_t)kMaxLdsBytes, PRBF_ERR_LIMIT, "RASM block too large for the single-workgroup factorisation");
+  need(block_lds_bytes(fp32, c->max_buffer, c->max_core, c->chol_waves) <= kMaxLdsBytes, PRBF_ERR_LIMIT,
+       "RASM block too large for the LDS panel of the block factorisation");
+  c->rasm_ready = false;
   c->mptr.ensure(nc + 1);
-  c->M.ensure(c->h_mptr[nc]);
   PRBF_HIP(hipMemcpyAsync(c->mptr, c->h_mptr.data(), sizeof(long long) * (nc + 1), hipMemcpyHostToDevice, c->st));
-  // chunks bounded by a workspace budget
-  size_t free_b = 0, total_b = 0;
-  PRBF_HIP(hipMemGetInfo(&free_b, &total_b));
-  long long budget = (long long)std::min<size_t>(free_b / 3, (size_t)16 << 30) / (long long)sizeof(double);
-  long long maxsq = (long long)c->max_buffer * c->max_buffer;
-  need(budget >= maxsq, PRBF_ERR_NOMEM, "not enough device memory for one RASM block");
-  DBuf<double> W;
-  DBuf<long long> wptr;
-  std::vector<long long> h_wptr;
   PRBF_HIP(hipMemsetAsync(c->d_err, 0, sizeof(int), c->st));
-  long long k0 = 0;
-  bool allocated = false;
-  while (k0 < nc) {
-    long long k1 = k0, used = 0;
-    h_wptr.clear();
-    while (k1 < nc) {
-      long long n = c->h_bptr[k1 + 1] - c->h_bptr[k1];
-      if (used + n * n > budget && k1 > k0) break;
-      h_wptr.push_back(used);
-      used += n * n;
-      ++k1;
-      if (k1 - k0 >= (1LL << 20)) break;
-    }
-    if (!allocated) {
-      // size the workspace once for the largest chunk this budget allows
-      W.ensure((size_t)std::min<long long>(budget, std::max<long long>(used, 1)));
-      wptr.ensure(std::min<long long>(nc, 1LL << 20) + 1);
-      allocated = true;
-    }
-    if ((size_t)used > W.n) W.ensure(used);
-    sync(c);  // previous chunk done before wptr is overwritten
-    PRBF_HIP(hipMemcpy(wptr, h_wptr.data(), sizeof(long long) * h_wptr.size(), hipMemcpyHostToDevice));
+  c->n_fallback = 0;
+  if (!fp32) {
+    c->M.ensure(c->h_mptr[nc]);
+    factor_chunks<double>(c, false, wsz, c->M.p, nullptr);
+  } else {
+    c->M32.ensure(c->h_mptr[nc]);
+    std::vector<int> failed;
+    factor_chunks<float>(c, true, wsz, c->M32.p, &failed);
+    // FP64 redo of the blocks whose FP32 factorisation broke down
+    c->fb.ensure(nc);
+    std::vector<long long> h_fb(nc, -1), h_m64(nc + 1, 0);
+    long long tot = 0, wtot = 0;
     int nmax = 0, tmax = 0;
-    for (long long k = k0; k < k1; ++k) {
-      nmax = std::max<int>(nmax, (int)(c->h_bptr[k + 1] - c->h_bptr[k]));
-      tmax = std::max(tmax, c->h_iend[k] - c->h_ista[k] + 1);
+    std::vector<long long> h_w;
+    for (int k : failed) {
+      long long n = c->h_bptr[k + 1] - c->h_bptr[k];
+      long long T = c->h_iend[k] - c->h_ista[k] + 1;
+      h_fb[k] = tot;
+      h_m64[k] = tot;
+      tot += n * T;
+      h_w.push_back(wtot);
+      wtot += wsz[k];
+      nmax = std::max<int>(nmax, (int)n);
+      tmax = std::max<int>(tmax, (int)T);
     }
-    rasm_factor_fp64(c->st, g.dim, k0, k1 - k0, nmax, tmax, c->bptr, c->bidx, c->ista, c->iend, c->xi, c->yi,
-                     c->zi, wptr, W, c->mptr, c->M, c->prm.sigma, c->d_err);
-    PRBF_HIP(hipGetLastError());
-    k0 = k1;
+    c->n_fallback = (int)failed.size();
+    PRBF_HIP(hipMemcpyAsync(c->fb, h_fb.data(), sizeof(long long) * nc, hipMemcpyHostToDevice, c->st));
+    if (!failed.empty()) {
+      c->M64fb.ensure(tot);
+      DBuf<double> W;
+      DBuf<long long> wp, mp;
+      DBuf<int> cl;
+      W.ensure(wtot);
+      wp.ensure(h_w.size());
+      mp.ensure(nc + 1);
+      cl.ensure(failed.size());
+      PRBF_HIP(hipMemcpyAsync(wp, h_w.data(), sizeof(long long) * h_w.size(), hipMemcpyHostToDevice, c->st));
+      PRBF_HIP(hipMemcpyAsync(mp, h_m64.data(), sizeof(long long) * (nc + 1), hipMemcpyHostToDevice, c->st));
+      PRBF_HIP(hipMemcpyAsync(cl, failed.data(), sizeof(int) * failed.size(), hipMemcpyHostToDevice, c->st));
+      block_chol_fp64(c->st, g.dim, c->chol_waves, 0, cl, (long long)failed.size(), nmax, tmax, c->bptr, c->bidx,
+                      c->ista, c->iend, c->xi, c->yi, c->zi, wp, W, mp, c->M64fb, c->prm.sigma, c->d_err);
+      PRBF_HIP(hipGetLastError());
+      sync(c);
+    }
   }
   int herr = 0;
   PRBF_HIP(hipMemcpyAsync(&herr, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->st));
   sync(c);
   need(herr == 0, PRBF_ERR_FACTOR, "a RASM block is not numerically positive definite");
   c->rasm_ready = true;
+  c->rasm_pc = pc;
 }
 
 void rasm_apply_impl(prbf_ctx* c, const double* r, double* z) {
   need(c->rasm_ready, PRBF_ERR_STATE, "rasm_setup first");
-  rasm_apply_fp64(c->st, c->g.ncell, c->bptr, c->bidx, c->ista, c->iend, c->mptr, c->M, r, z, c->max_core);
+  if (c->rasm_pc == PRBF_PC_RASM_FP32)
+    rasm_apply_fp32(c->st, c->g.ncell, c->bptr, c->bidx, c->ista, c->iend, c->mptr, c->M32, c->fb, c->M64fb.p, r, z);
+  else
+    rasm_apply_fp64(c->st, c->g.ncell, c->bptr, c->bidx, c->ista, c->iend, c->mptr, c->M, r, z, c->max_core);
   PRBF_HIP(hipGetLastError());
 }
-
 // ---------------------------------------------------------------- Krylov
 struct Conv {  // PETSc KSPConvergedDefault, preconditioned norm, zero guess
   double rtol, abstol, dtol, rnorm0 = 0, ttol = 0;
@@ -528,9 +603,9 @@ void solve_dev_impl(prbf_ctx* c, const prbf_solve_opts* opts, const double* w, d
   prbf_solve_opts_default(&o);
   if (opts) o = *opts;
   double t_setup = 0.0;
-  if (o.pc == PRBF_PC_RASM_FP64 && !c->rasm_ready) {
+  if (o.pc != PRBF_PC_NONE && (!c->rasm_ready || c->rasm_pc != o.pc)) {
     double t0 = now_s();
-    rasm_setup_impl(c);
+    rasm_setup_impl(c, o.pc);
     t_setup = now_s() - t0;
   }
   const long long n = c->ni;
@@ -550,6 +625,7 @@ void solve_dev_impl(prbf_ctx* c, const prbf_solve_opts* opts, const double* w, d
   krylov_impl(c, o, bs, xs, &local, h, cap);
   scatter(c->st, n, c->perm_i, xs, gamma);
   local.t_setup_s = t_setup;
+  local.n_fallback = c->n_fallback;
   if (info) *info = local;
 }
 
@@ -767,7 +843,19 @@ prbf_status prbf_evaluate(prbf_ctx* c, int nrhs, const double* gj, double* wi) {
 }
 
 prbf_status prbf_rasm_setup(prbf_ctx* c) {
-  return guarded(c, [&] { rasm_setup_impl(c); });
+  return guarded(c, [&] { rasm_setup_impl(c, PRBF_PC_RASM_FP64); });
+}
+
+prbf_status prbf_rasm_setup_pc(prbf_ctx* c, int pc) {
+  return guarded(c, [&] { rasm_setup_impl(c, pc); });
+}
+
+prbf_status prbf_set_tuning(prbf_ctx* c, int key, int value) {
+  return guarded(c, [&] {
+    need(key == PRBF_TUNE_CHOL_WAVES, PRBF_ERR_ARG, "unknown tuning key");
+    need(value == 4 || value == 8, PRBF_ERR_ARG, "waves per block factorisation must be 4 or 8");
+    c->chol_waves = value;
+  });
 }
 
 prbf_status prbf_rasm_apply_dev(prbf_ctx* c, const double* r, double* z) {
@@ -789,7 +877,7 @@ prbf_status prbf_rasm_apply(prbf_ctx* c, const double* r, double* z) {
 
 prbf_status prbf_rasm_get_core_rows(prbf_ctx* c, int64_t k, double* M, int32_t* n, int32_t* T) {
   return guarded(c, [&] {
-    need(c->rasm_ready, PRBF_ERR_STATE, "rasm_setup first");
+    need(c->rasm_ready && c->rasm_pc == PRBF_PC_RASM_FP64, PRBF_ERR_STATE, "FP64 rasm_setup first");
     need(k >= 0 && k < c->g.ncell, PRBF_ERR_ARG, "cluster index out of range");
     int nn = (int)(c->h_bptr[k + 1] - c->h_bptr[k]);
     int tt = nn ? c->h_iend[k] - c->h_ista[k] + 1 : 0;

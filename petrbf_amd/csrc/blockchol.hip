@@ -1,0 +1,392 @@
+// Batched RASM block setup on the matrix cores: one workgroup per cluster
+// block, 16x16 tiles, MFMA for every O(n^3) and O(n^2 T) term.
+//
+// For cluster c with n buffer particles and T core particles the block is the
+// dense untruncated Gaussian over the buffer (2d/matmult.cxx:83-93).  PCASM's
+// restrict form (2d/rbf_interpolation.cxx:213-229) only ever reads the core
+// rows of A_c^{-1}, so the setup produces M_c = (A_c^{-1})[core, buffer]
+// (T x n, row-major) and the apply is a batched GEMV over M_c.
+//
+//   phase 0  order the buffer with the core LAST and padding (identity) up to
+//            NP = 16*ceil(n/16); write the lower tiles of A' to a workspace in
+//            the MFMA accumulator layout (one coalesced access per register);
+//   phase 1  right-looking tiled Cholesky A' = L L^T: the 16x16 diagonal tile
+//            by one wavefront (in LDS), L_kk^{-1} beside it, the panel as
+//            A_ik L_kk^{-T} on MFMA, then the trailing update
+//            A_ij -= L_ik L_jk^T on MFMA with the panel in LDS;
+//   phase 2  with the core last, L^{-1} E_core is zero above the core, so
+//            A'^{-1} E_core = L^{-T} (L^{-1} E_core) is a short forward and a
+//            full backward tile substitution, both MFMA with the
+//            L_II^{-1} tiles kept from phase 1;
+//   phase 3  scatter X to M_c in buffer order.
+// T = double is the plain path (exact FP64 block solves); T = float runs the
+// same kernel in FP32 on the FP32 MFMA (mixed precision), with a per-block
+// status so a block whose FP32 factorisation breaks down is redone in FP64.
+#include "common.h"
+#include "kernels.h"
+
+namespace prbf {
+
+template <typename T>
+struct MF;
+template <>
+struct MF<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  __device__ static inline v4 mma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct MF<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  __device__ static inline v4 mma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+constexpr int kTE = 256;   // elements per 16x16 tile
+constexpr int kLS = 17;    // LDS row stride (bank-conflict padding)
+constexpr int kLT = 16 * kLS;
+
+// Accumulator (C/D) layout of the 16x16x4 MFMA: lane l, register r holds
+// element (4*(l/16) + r, l%16).  Tiles are stored in that order, so element
+// (i, j) of a tile sits at cl(i, j) and register r of lane l at r*64 + l.
+__device__ inline int cl(int i, int j) { return (i & 3) * 64 + (i >> 2) * 16 + j; }
+__device__ inline int ltile(int I, int J) { return I * (I + 1) / 2 + J; }
+
+__device__ inline void tri_decode(int t, int* I, int* J) {
+  int i = (int)((sqrtf(8.0f * t + 1.0f) - 1.0f) * 0.5f);
+  while (i * (i + 1) / 2 > t) --i;
+  while ((i + 1) * (i + 2) / 2 <= t) ++i;
+  *I = i;
+  *J = t - i * (i + 1) / 2;
+}
+
+__device__ inline void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
+template <typename T>
+__device__ inline T gauss_entry(double dx, double dy, double dz, double inv_two_s2, double norm);
+template <>
+__device__ inline double gauss_entry<double>(double dx, double dy, double dz, double inv, double norm) {
+  return exp(-(dx * dx + dy * dy + dz * dz) * inv) * norm;
+}
+template <>
+__device__ inline float gauss_entry<float>(double dx, double dy, double dz, double inv, double norm) {
+  float a = (float)((dx * dx + dy * dy + dz * dz) * inv);
+  return __expf(-a) * (float)norm;
+}
+
+// Workspace of one block, in elements of T:
+//   lower tiles (nt = KT(KT+1)/2, 256 each) | L_II^{-1} (KT x 256, row-major) | X (NP x TP)
+__host__ __device__ inline long long blk_ws(int n, int T) {
+  int KT = (n + 15) / 16, NP = 16 * KT, TP = 16 * ((T + 15) / 16);
+  return (long long)(KT * (KT + 1) / 2 + KT) * kTE + (long long)NP * TP;
+}
+
+template <typename T>
+__host__ __device__ inline size_t blk_lds_bytes(int nmax, int tmax, int nw) {
+  int KT = (nmax + 15) / 16, NP = 16 * KT;
+  (void)tmax;
+  size_t pan = (size_t)KT * kLT * sizeof(T);
+  size_t crd = (size_t)3 * NP * sizeof(double);
+  size_t red = (size_t)nw * kTE * sizeof(T);
+  size_t region = pan > crd ? pan : crd;
+  region = region > red ? region : red;
+  region = (region + 15) & ~(size_t)15;
+  return region + (size_t)(nw + 2) * kLT * sizeof(T) + 16;
+}
+
+template <typename Real, int DIM, int NW>
+__global__ __launch_bounds__(NW * 64) void k_block_chol(
+    long long c0, const int* __restrict__ clist, const long long* __restrict__ bptr, const int* __restrict__ bidx,
+    const int* __restrict__ ista, const int* __restrict__ iend, const double* __restrict__ xs,
+    const double* __restrict__ ys, const double* __restrict__ zs, const long long* __restrict__ wptr,
+    Real* __restrict__ Wbase, const long long* __restrict__ mptr, Real* __restrict__ M, double inv_two_s2,
+    double norm, int* __restrict__ err, int* __restrict__ blk_status) {
+  typedef typename MF<Real>::v4 v4;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const long long c = clist ? (long long)clist[blockIdx.x] : c0 + blockIdx.x;
+  const long long b0 = bptr[c];
+  const int n = (int)(bptr[c + 1] - b0);
+  if (n == 0) return;
+  const int T = iend[c] - ista[c] + 1;
+  const int KT = (n + 15) / 16, NP = 16 * KT, TP = 16 * ((T + 15) / 16), TC = TP / 16;
+  const int nt = KT * (KT + 1) / 2;
+  const int ncore0 = n - T;
+  const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+  const int lr = lane & 15, lq = lane >> 4;
+
+  Real* Wb = Wbase + wptr[blockIdx.x];
+  Real* Wd = Wb + (long long)nt * kTE;       // L_II^{-1}, row-major
+  Real* Xg = Wd + (long long)KT * kTE;       // NP x TP
+
+  // LDS carve-up
+  size_t pan = (size_t)KT * kLT * sizeof(Real), crd = (size_t)3 * NP * sizeof(double),
+         red = (size_t)NW * kTE * sizeof(Real);
+  size_t region = pan > crd ? pan : crd;
+  region = region > red ? region : red;
+  region = (region + 15) & ~(size_t)15;
+  Real* P = (Real*)lds_raw;                     // panel: KT tiles, row-major stride 17
+  double* cx = (double*)lds_raw;          // phase 0 coordinates (same region)
+  double* cy = cx + NP;
+  double* cz = cy + NP;
+  Real* R = (Real*)lds_raw;                     // phase 2 cross-wave reduction (same region)
+  Real* S = (Real*)(lds_raw + region);          // per-wave scratch tile
+  Real* D = S + NW * kLT;                    // diagonal tile L_kk
+  Real* Li = D + kLT;                        // L_kk^{-1}
+  int* flag = (int*)(Li + kLT);
+  Real* Sw = S + wave * kLT;
+
+  // ---- phase 0: coordinates (core last) and the lower tiles of A'
+  for (int p = tid; p < n; p += NW * 64) {
+    int b = p < ncore0 ? p + T : p - ncore0;
+    int j = bidx[b0 + b];
+    cx[p] = xs[j];
+    cy[p] = ys[j];
+    cz[p] = DIM == 3 ? zs[j] : 0.0;
+  }
+  if (tid == 0) *flag = 0;
+  __syncthreads();
+  for (int t = wave; t < nt; t += NW) {
+    int I, J;
+    tri_decode(t, &I, &J);
+    Real* tile = Wb + (long long)t * kTE;
+    const int j = 16 * J + lr;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = 16 * I + 4 * lq + r;
+      Real v;
+      if (i < n && j < n)
+        v = gauss_entry<Real>(cx[i] - cx[j], cy[i] - cy[j], DIM == 3 ? cz[i] - cz[j] : 0.0, inv_two_s2, norm);
+      else
+        v = (i == j) ? Real(1) : Real(0);
+      tile[r * 64 + lane] = v;
+    }
+  }
+  __syncthreads();
+
+  // ---- phase 1: tiled right-looking Cholesky
+  for (int kk = 0; kk < KT; ++kk) {
+    if (wave == 0) {
+      Real* tkk = Wb + (long long)ltile(kk, kk) * kTE;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) D[(4 * lq + r) * kLS + lr] = tkk[r * 64 + lane];
+      wave_fence();
+      for (int p = 0; p < 16; ++p) {
+        Real d = D[p * kLS + p];
+        bool bad = !(d > Real(0));
+        Real s = bad ? Real(1) : sqrt(d);
+        Real rs = Real(1) / s;
+        wave_fence();
+        if (lane == 0 && bad) *flag = 1;
+        if (lane > p && lane < 16) D[lane * kLS + p] *= rs;
+        if (lane == p) D[p * kLS + p] = s;
+        wave_fence();
+        if (lane > p && lane < 16) {
+          Real lp = D[lane * kLS + p];
+          for (int q = p + 1; q <= lane; ++q) D[lane * kLS + q] -= lp * D[q * kLS + p];
+        }
+        wave_fence();
+      }
+      // zero the strict upper part, invert the lower triangle column by column
+      if (lane < 16)
+        for (int q = lane + 1; q < 16; ++q) D[lane * kLS + q] = Real(0);
+      wave_fence();
+      if (lane < 16) {
+        const int jcol = lane;
+        for (int i = 0; i < 16; ++i) {
+          Real x;
+          if (i < jcol) {
+            x = Real(0);
+          } else {
+            x = (i == jcol) ? Real(1) : Real(0);
+            for (int q = jcol; q < i; ++q) x -= D[i * kLS + q] * Li[q * kLS + jcol];
+            x /= D[i * kLS + i];
+          }
+          Li[i * kLS + jcol] = x;
+        }
+      }
+      wave_fence();
+#pragma unroll
+      for (int r = 0; r < 4; ++r) tkk[r * 64 + lane] = D[(4 * lq + r) * kLS + lr];
+      Real* wd = Wd + (long long)kk * kTE;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int e = r * 64 + lane;
+        wd[e] = Li[(e >> 4) * kLS + (e & 15)];
+      }
+    }
+    __syncthreads();
+    // panel: L_ik = A_ik L_kk^{-T}
+    for (int i = kk + 1 + wave; i < KT; i += NW) {
+      Real* tik = Wb + (long long)ltile(i, kk) * kTE;
+      Real a[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) a[r] = tik[r * 64 + lane];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Sw[(4 * lq + r) * kLS + lr] = a[r];
+      wave_fence();
+      v4 acc = {0, 0, 0, 0};
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        acc = MF<Real>::mma(Sw[lr * kLS + 4 * q + lq], Li[lr * kLS + 4 * q + lq], acc);
+      Real* pi = P + i * kLT;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        tik[r * 64 + lane] = acc[r];
+        pi[(4 * lq + r) * kLS + lr] = acc[r];
+      }
+      wave_fence();
+    }
+    __syncthreads();
+    // trailing update: A_ij -= L_ik L_jk^T for kk < j <= i < KT
+    const int m = KT - kk - 1;
+    const int ntr = m * (m + 1) / 2;
+    for (int u = wave; u < ntr; u += NW) {
+      int ii, jj;
+      tri_decode(u, &ii, &jj);
+      const int i = kk + 1 + ii, j = kk + 1 + jj;
+      Real* tij = Wb + (long long)ltile(i, j) * kTE;
+      v4 acc;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[r] = tij[r * 64 + lane];
+      const Real* pi = P + i * kLT;
+      const Real* pj = P + j * kLT;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        acc = MF<Real>::mma(-pi[lr * kLS + 4 * q + lq], pj[lr * kLS + 4 * q + lq], acc);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) tij[r * 64 + lane] = acc[r];
+    }
+    __syncthreads();
+  }
+  if (*flag) {
+    if (tid == 0) {
+      if (blk_status) blk_status[c] = 1;
+      else atomicOr(err, 2);
+    }
+    return;
+  }
+
+  // ---- phase 2: X = L^{-T} L^{-1} E_core (NP x TP, in place in Xg)
+  const int I0 = ncore0 / 16;
+  for (int tc = 0; tc < TC; ++tc) {
+    // forward: Y_I = L_II^{-1} (E_I - sum_{I0<=J<I} L_IJ Y_J), tile rows I >= I0
+    if (wave == 0) {
+      for (int I = I0; I < KT; ++I) {
+        v4 acc = {0, 0, 0, 0};
+        for (int J = I0; J < I; ++J) {
+          const Real* tij = Wb + (long long)ltile(I, J) * kTE;
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            acc = MF<Real>::mma(tij[cl(lr, 4 * q + lq)], Xg[(long long)(16 * J + 4 * q + lq) * TP + 16 * tc + lr], acc);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int row = 16 * I + 4 * lq + r, col = 16 * tc + lr;
+          Real e = (col < T && row == ncore0 + col) ? Real(1) : Real(0);
+          Sw[(4 * lq + r) * kLS + lr] = e - acc[r];
+        }
+        wave_fence();
+        const Real* li = Wd + (long long)I * kTE;
+        v4 y = {0, 0, 0, 0};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) y = MF<Real>::mma(li[lr * 16 + 4 * q + lq], Sw[(4 * q + lq) * kLS + lr], y);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Xg[(long long)(16 * I + 4 * lq + r) * TP + 16 * tc + lr] = y[r];
+        wave_fence();
+      }
+    }
+    __syncthreads();
+    // backward: X_I = L_II^{-T} (Y_I - sum_{J>I} L_JI^T X_J)
+    for (int I = KT - 1; I >= 0; --I) {
+      v4 acc = {0, 0, 0, 0};
+      for (int J = I + 1 + wave; J < KT; J += NW) {
+        const Real* tji = Wb + (long long)ltile(J, I) * kTE;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          acc = MF<Real>::mma(tji[lq * 64 + q * 16 + lr], Xg[(long long)(16 * J + 4 * q + lq) * TP + 16 * tc + lr], acc);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) R[wave * kTE + r * 64 + lane] = acc[r];
+      __syncthreads();
+      if (wave == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          Real s = R[r * 64 + lane];
+          for (int w = 1; w < NW; ++w) s += R[w * kTE + r * 64 + lane];
+          Real yv = (I >= I0) ? Xg[(long long)(16 * I + 4 * lq + r) * TP + 16 * tc + lr] : Real(0);
+          Sw[(4 * lq + r) * kLS + lr] = yv - s;
+        }
+        wave_fence();
+        const Real* li = Wd + (long long)I * kTE;
+        v4 x = {0, 0, 0, 0};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) x = MF<Real>::mma(li[(4 * q + lq) * 16 + lr], Sw[(4 * q + lq) * kLS + lr], x);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Xg[(long long)(16 * I + 4 * lq + r) * TP + 16 * tc + lr] = x[r];
+      }
+      __syncthreads();
+    }
+  }
+
+  // ---- phase 3: M_c[t][b] = X[pos(b)][t]
+  Real* Mc = M + mptr[c];
+  for (int e = tid; e < n * T; e += NW * 64) {
+    int t = e / n, b = e % n;
+    int p = b < T ? ncore0 + b : b - T;
+    Mc[e] = Xg[(long long)p * TP + t];
+  }
+  if (blk_status && tid == 0) blk_status[c] = 0;
+}
+
+template <typename T>
+static void launch_block_chol(hipStream_t st, int dim, int nw, long long c0, const int* clist, long long ncl,
+                              int nmax, int tmax, const long long* bptr, const int* bidx, const int* ista, const int* iend,
+                              const double* xs, const double* ys, const double* zs, const long long* wptr, T* W,
+                              const long long* mptr, T* M, double sigma, int* err, int* blk_status) {
+  Kern k = make_kern(sigma);
+  size_t lds = blk_lds_bytes<T>(nmax, tmax, nw);
+  static bool attr = false;
+  if (!attr) {
+    const void* fns[] = {(const void*)k_block_chol<T, 2, 4>, (const void*)k_block_chol<T, 3, 4>,
+                         (const void*)k_block_chol<T, 2, 8>, (const void*)k_block_chol<T, 3, 8>};
+    for (const void* f : fns)
+      PRBF_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    attr = true;
+  }
+  dim3 grid((unsigned)ncl);
+#define PRBF_BC(D, NWV)                                                                                   \
+  hipLaunchKernelGGL((k_block_chol<T, D, NWV>), grid, dim3(NWV * 64), lds, st, c0, clist, bptr, bidx, ista, iend, xs, \
+                     ys, zs, wptr, W, mptr, M, k.inv_two_s2, k.norm, err, blk_status)
+  if (dim == 2) {
+    if (nw == 8) PRBF_BC(2, 8); else PRBF_BC(2, 4);
+  } else {
+    if (nw == 8) PRBF_BC(3, 8); else PRBF_BC(3, 4);
+  }
+#undef PRBF_BC
+}
+
+long long block_workspace(int n, int T) { return blk_ws(n, T); }
+
+size_t block_lds_bytes(bool fp32, int nmax, int tmax, int nw) {
+  return fp32 ? blk_lds_bytes<float>(nmax, tmax, nw) : blk_lds_bytes<double>(nmax, tmax, nw);
+}
+
+void block_chol_fp64(hipStream_t st, int dim, int nw, long long c0, const int* clist, long long ncl, int nmax,
+                     int tmax, const long long* bptr, const int* bidx, const int* ista, const int* iend,
+                     const double* xs, const double* ys, const double* zs, const long long* wptr, double* W,
+                     const long long* mptr, double* M, double sigma, int* err) {
+  launch_block_chol<double>(st, dim, nw, c0, clist, ncl, nmax, tmax, bptr, bidx, ista, iend, xs, ys, zs, wptr, W,
+                            mptr, M, sigma, err, nullptr);
+}
+
+void block_chol_fp32(hipStream_t st, int dim, int nw, long long c0, long long ncl, int nmax, int tmax,
+                     const long long* bptr, const int* bidx, const int* ista, const int* iend, const double* xs,
+                     const double* ys, const double* zs, const long long* wptr, float* W, const long long* mptr,
+                     float* M, double sigma, int* blk_status) {
+  launch_block_chol<float>(st, dim, nw, c0, nullptr, ncl, nmax, tmax, bptr, bidx, ista, iend, xs, ys, zs, wptr, W,
+                           mptr, M, sigma, nullptr, blk_status);
+}
+
+}  // namespace prbf

@@ -25,19 +25,26 @@ void matvec(hipStream_t st, int dim, int nrhs, long long ncell, const int* tsta,
             const double* xj, const double* yj, const double* zj, const double* g, long long ldg, double* w,
             long long ldw, double sigma);
 
-// rasm.hip --------------------------------------------------------------------
-struct RasmChunk {
-  long long c0, ncl;        // clusters [c0, c0+ncl)
-  const long long* wptr;    // per-cluster workspace offsets (doubles), relative to W
-};
-// Factorises the blocks of one chunk and writes their core-inverse rows.
-void rasm_factor_fp64(hipStream_t st, int dim, long long c0, long long ncl, int nmax, int tmax,
-                      const long long* bptr, const int* bidx, const int* ista, const int* iend,
-                      const double* xs, const double* ys, const double* zs, const long long* wptr,
-                      double* W, const long long* mptr, double* M, double sigma, int* err);
+// blockchol.hip / rasm.hip ----------------------------------------------------
+// Workspace (elements) and LDS bytes of one block factorisation.
+long long block_workspace(int n, int T);
+size_t block_lds_bytes(bool fp32, int nmax, int tmax, int nw);
+// Factorises the blocks of clusters [c0, c0+ncl) and writes their core rows.
+// clist != nullptr: factorise clusters clist[0..ncl) instead of [c0, c0+ncl).
+void block_chol_fp64(hipStream_t st, int dim, int nw, long long c0, const int* clist, long long ncl, int nmax,
+                     int tmax, const long long* bptr, const int* bidx, const int* ista, const int* iend,
+                     const double* xs, const double* ys, const double* zs, const long long* wptr, double* W,
+                     const long long* mptr, double* M, double sigma, int* err);
+void block_chol_fp32(hipStream_t st, int dim, int nw, long long c0, long long ncl, int nmax, int tmax,
+                     const long long* bptr, const int* bidx, const int* ista, const int* iend, const double* xs,
+                     const double* ys, const double* zs, const long long* wptr, float* W, const long long* mptr,
+                     float* M, double sigma, int* blk_status);
 void rasm_apply_fp64(hipStream_t st, long long ncell, const long long* bptr, const int* bidx,
                      const int* ista, const int* iend, const long long* mptr, const double* M,
                      const double* r, double* z, int tmax);
+void rasm_apply_fp32(hipStream_t st, long long ncell, const long long* bptr, const int* bidx,
+                     const int* ista, const int* iend, const long long* mptr, const float* M,
+                     const long long* fb, const double* M64, const double* r, double* z);
 
 // krylov.hip ------------------------------------------------------------------
 constexpr int kRedBlocks = 1024;  // fixed partition => run-to-run identical sums

@@ -20,7 +20,8 @@ import numpy as np
 from . import _lib as L
 
 _KSP = {"gmres": L.PRBF_KSP_GMRES, "fgmres": L.PRBF_KSP_FGMRES}
-_PC = {"none": L.PRBF_PC_NONE, "rasm": L.PRBF_PC_RASM_FP64, "asm": L.PRBF_PC_RASM_FP64}
+_PC = {"none": L.PRBF_PC_NONE, "rasm": L.PRBF_PC_RASM_FP64, "asm": L.PRBF_PC_RASM_FP64,
+       "rasm32": L.PRBF_PC_RASM_FP32}
 
 
 def _f64(a):
@@ -48,6 +49,7 @@ class SolveInfo:
     t_pc_s: float
     n_matvec: int
     n_pc: int
+    n_fallback: int
     hist: np.ndarray
 
     @property
@@ -186,8 +188,12 @@ class Context:
         self._check(self._L.prbf_evaluate_dev(self._h, nrhs, _p(gj), _p(wi)))
 
     # ---- RASM
-    def rasm_setup(self):
-        self._check(self._L.prbf_rasm_setup(self._h))
+    def rasm_setup(self, pc="rasm"):
+        self._check(self._L.prbf_rasm_setup_pc(self._h, _PC[pc]))
+
+    def set_tuning(self, chol_waves=None):
+        if chol_waves is not None:
+            self._check(self._L.prbf_set_tuning(self._h, L.PRBF_TUNE_CHOL_WAVES, int(chol_waves)))
 
     def rasm_apply(self, r_sorted):
         r = _f64(r_sorted)
@@ -202,10 +208,10 @@ class Context:
         n = C.c_int()
         t = C.c_int()
         self._check(self._L.prbf_rasm_get_core_rows(self._h, int(c), None, C.byref(n), C.byref(t)))
-        M = np.zeros((n.value, t.value))
+        M = np.zeros((t.value, n.value))
         if n.value:
             self._check(self._L.prbf_rasm_get_core_rows(self._h, int(c), _p(M), C.byref(n), C.byref(t)))
-        return M.T.copy()  # (T, n): rows of A_c^{-1} for the core members
+        return M  # (T, n): rows of A_c^{-1} for the core members
 
     # ---- solve
     def _opts(self, ksp, pc, rtol, restart, max_it):
@@ -221,7 +227,7 @@ class Context:
     def _info(self, inf, hist):
         n = min(hist.shape[0], inf.its + 64)
         return SolveInfo(inf.its, inf.reason, inf.rnorm0, inf.rnorm, inf.t_setup_s, inf.t_solve_s,
-                         inf.t_matvec_s, inf.t_pc_s, inf.n_matvec, inf.n_pc, hist[:n])
+                         inf.t_matvec_s, inf.t_pc_s, inf.n_matvec, inf.n_pc, inf.n_fallback, hist[:n])
 
     def solve(self, w, ksp="gmres", pc="rasm", rtol=1e-10, restart=30, max_it=1000):
         """Strengths gamma (caller's order) with A~ gamma = w."""

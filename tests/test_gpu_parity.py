@@ -152,8 +152,10 @@ def test_plain_solve_parity(n_side):
     assert info.reason == 2 and ref["reason"] == 2
     assert info.its == ref["its"]
     assert np.linalg.norm(g_gpu - g_cpu) <= 1e-10 * np.linalg.norm(g_cpu)
+    # ||B w|| and the final estimate: the block solves differ in rounding
+    # (Cholesky core rows vs the reference's LU), ~kappa(block)*eps ~ 1e-11
     r0 = ref["hist"][0]
-    assert abs(info.rnorm0 - r0) <= 1e-12 * r0
+    assert abs(info.rnorm0 - r0) <= 1e-10 * r0
     assert abs(info.rnorm - ref["hist"][-1]) <= 1e-10 * r0
     # analytic-field error: evaluate the strengths at the particles (2d/main.cxx:125-133)
     w_gpu = ctx.evaluate(g_gpu)
@@ -193,4 +195,59 @@ def test_errors_are_reported_not_crashed():
     ctx.set_points(np.array([0.0, 0.5]), np.array([0.0, 0.1]))
     with pytest.raises(RuntimeError):
         ctx.matvec(np.zeros((4, 2)))           # nrhs out of range
+    ctx.close()
+
+
+def test_block_factorisation_waves_agree():
+    """The 4- and 8-wavefront variants of the block factorisation compute the
+    same core rows (only the order of the cross-wave sums differs)."""
+    _announce("chol waves 4 vs 8")
+    p = lattice(40)
+    rows = []
+    for nw in (4, 8):
+        ctx = Context.from_problem(p)
+        ctx.set_tuning(chol_waves=nw)
+        ctx.set_points(p.x, p.y)
+        ctx.rasm_setup()
+        rows.append([ctx.core_rows(c) for c in range(0, ctx.ncell, 7)])
+        ctx.close()
+    for a, b in zip(*rows):
+        assert np.abs(a - b).max() <= 1e-9 * max(np.abs(a).max(), 1e-300)
+
+
+def test_mixed_fp32_rasm_fgmres_reaches_fp64_residual():
+    """Chosen form: FP32 blocks on the matrix cores inside FP64 FGMRES(30).
+    The true FP64 residual reaches rtol 1e-10 and the analytic-field error
+    matches the plain solve's."""
+    _announce("mixed fp32 rasm + fgmres")
+    p = lattice(160, seed=0x5EED0001)
+    ctx = Context.from_problem(p)
+    ctx.set_points(p.x, p.y)
+    g32, inf32 = ctx.solve(p.w[0], ksp="fgmres", pc="rasm32", rtol=1e-10)
+    _announce(f"mixed: its {inf32.its} fallback blocks {inf32.n_fallback} setup {inf32.t_setup_s:.3f}s")
+    assert inf32.reason == 2
+    e = p.w[0]
+    res32 = np.linalg.norm(ctx.evaluate(g32) - e) / np.linalg.norm(e)
+    assert res32 <= 1.05e-10
+    g64, inf64 = ctx.solve(p.w[0], ksp="gmres", pc="rasm", rtol=1e-10)
+    res64 = np.linalg.norm(ctx.evaluate(g64) - e) / np.linalg.norm(e)
+    _announce(f"true residual mixed {res32:.3e} plain {res64:.3e}; its mixed {inf32.its} plain {inf64.its}")
+    assert res32 <= 2 * res64 + 1e-10
+    ctx.close()
+
+
+def test_mixed_core_rows_close_to_fp64():
+    _announce("fp32 apply vs fp64 apply")
+    p = lattice(40)
+    ctx = Context.from_problem(p)
+    ctx.set_points(p.x, p.y)
+    rng = np.random.default_rng(11)
+    r = rng.standard_normal(p.n)
+    ctx.rasm_setup("rasm")
+    z64 = ctx.rasm_apply(r)
+    ctx.rasm_setup("rasm32")
+    z32 = ctx.rasm_apply(r)
+    rel = np.linalg.norm(z32 - z64) / np.linalg.norm(z64)
+    _announce(f"fp32 vs fp64 RASM apply rel diff {rel:.2e}")
+    assert rel < 0.5
     ctx.close()
