@@ -29,12 +29,20 @@ namespace prbf {
 
 template <typename T>
 struct MF;
+// Both 16x16x4 MFMAs take A as lane l -> A[l%16][l/16] and B as
+// lane l -> B[l/16][l%16].  Their accumulators differ: register r of lane l
+// holds row 4*(l/16) + r for f32 but row 4*r + l/16 for f64 (column l%16 in
+// both).  row() gives that row; off(i, j) is where element (i, j) of a tile
+// sits when tiles are stored register-major (register r of lane l at r*64 + l,
+// one coalesced access per register).
 template <>
 struct MF<double> {
   typedef double v4 __attribute__((ext_vector_type(4)));
   __device__ static inline v4 mma(double a, double b, v4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  __device__ static inline int row(int r, int lq) { return 4 * r + lq; }
+  __device__ static inline int off(int i, int j) { return (i >> 2) * 64 + (i & 3) * 16 + j; }
 };
 template <>
 struct MF<float> {
@@ -42,16 +50,14 @@ struct MF<float> {
   __device__ static inline v4 mma(float a, float b, v4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  __device__ static inline int row(int r, int lq) { return 4 * lq + r; }
+  __device__ static inline int off(int i, int j) { return (i & 3) * 64 + (i >> 2) * 16 + j; }
 };
 
 constexpr int kTE = 256;   // elements per 16x16 tile
 constexpr int kLS = 17;    // LDS row stride (bank-conflict padding)
 constexpr int kLT = 16 * kLS;
 
-// Accumulator (C/D) layout of the 16x16x4 MFMA: lane l, register r holds
-// element (4*(l/16) + r, l%16).  Tiles are stored in that order, so element
-// (i, j) of a tile sits at cl(i, j) and register r of lane l at r*64 + l.
-__device__ inline int cl(int i, int j) { return (i & 3) * 64 + (i >> 2) * 16 + j; }
 __device__ inline int ltile(int I, int J) { return I * (I + 1) / 2 + J; }
 
 __device__ inline void tri_decode(int t, int* I, int* J) {
@@ -154,7 +160,7 @@ __global__ __launch_bounds__(NW * 64) void k_block_chol(
     const int j = 16 * J + lr;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int i = 16 * I + 4 * lq + r;
+      const int i = 16 * I + MF<Real>::row(r, lq);
       Real v;
       if (i < n && j < n)
         v = gauss_entry<Real>(cx[i] - cx[j], cy[i] - cy[j], DIM == 3 ? cz[i] - cz[j] : 0.0, inv_two_s2, norm);
@@ -170,7 +176,7 @@ __global__ __launch_bounds__(NW * 64) void k_block_chol(
     if (wave == 0) {
       Real* tkk = Wb + (long long)ltile(kk, kk) * kTE;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) D[(4 * lq + r) * kLS + lr] = tkk[r * 64 + lane];
+      for (int r = 0; r < 4; ++r) D[MF<Real>::row(r, lq) * kLS + lr] = tkk[r * 64 + lane];
       wave_fence();
       for (int p = 0; p < 16; ++p) {
         Real d = D[p * kLS + p];
@@ -208,7 +214,7 @@ __global__ __launch_bounds__(NW * 64) void k_block_chol(
       }
       wave_fence();
 #pragma unroll
-      for (int r = 0; r < 4; ++r) tkk[r * 64 + lane] = D[(4 * lq + r) * kLS + lr];
+      for (int r = 0; r < 4; ++r) tkk[r * 64 + lane] = D[MF<Real>::row(r, lq) * kLS + lr];
       Real* wd = Wd + (long long)kk * kTE;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -224,7 +230,7 @@ __global__ __launch_bounds__(NW * 64) void k_block_chol(
 #pragma unroll
       for (int r = 0; r < 4; ++r) a[r] = tik[r * 64 + lane];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) Sw[(4 * lq + r) * kLS + lr] = a[r];
+      for (int r = 0; r < 4; ++r) Sw[MF<Real>::row(r, lq) * kLS + lr] = a[r];
       wave_fence();
       v4 acc = {0, 0, 0, 0};
 #pragma unroll
@@ -234,7 +240,7 @@ __global__ __launch_bounds__(NW * 64) void k_block_chol(
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         tik[r * 64 + lane] = acc[r];
-        pi[(4 * lq + r) * kLS + lr] = acc[r];
+        pi[MF<Real>::row(r, lq) * kLS + lr] = acc[r];
       }
       wave_fence();
     }
@@ -279,13 +285,13 @@ __global__ __launch_bounds__(NW * 64) void k_block_chol(
           const Real* tij = Wb + (long long)ltile(I, J) * kTE;
 #pragma unroll
           for (int q = 0; q < 4; ++q)
-            acc = MF<Real>::mma(tij[cl(lr, 4 * q + lq)], Xg[(long long)(16 * J + 4 * q + lq) * TP + 16 * tc + lr], acc);
+            acc = MF<Real>::mma(tij[MF<Real>::off(lr, 4 * q + lq)], Xg[(long long)(16 * J + 4 * q + lq) * TP + 16 * tc + lr], acc);
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          int row = 16 * I + 4 * lq + r, col = 16 * tc + lr;
+          int row = 16 * I + MF<Real>::row(r, lq), col = 16 * tc + lr;
           Real e = (col < T && row == ncore0 + col) ? Real(1) : Real(0);
-          Sw[(4 * lq + r) * kLS + lr] = e - acc[r];
+          Sw[MF<Real>::row(r, lq) * kLS + lr] = e - acc[r];
         }
         wave_fence();
         const Real* li = Wd + (long long)I * kTE;
@@ -293,7 +299,7 @@ __global__ __launch_bounds__(NW * 64) void k_block_chol(
 #pragma unroll
         for (int q = 0; q < 4; ++q) y = MF<Real>::mma(li[lr * 16 + 4 * q + lq], Sw[(4 * q + lq) * kLS + lr], y);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) Xg[(long long)(16 * I + 4 * lq + r) * TP + 16 * tc + lr] = y[r];
+        for (int r = 0; r < 4; ++r) Xg[(long long)(16 * I + MF<Real>::row(r, lq)) * TP + 16 * tc + lr] = y[r];
         wave_fence();
       }
     }
@@ -305,7 +311,7 @@ __global__ __launch_bounds__(NW * 64) void k_block_chol(
         const Real* tji = Wb + (long long)ltile(J, I) * kTE;
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-          acc = MF<Real>::mma(tji[lq * 64 + q * 16 + lr], Xg[(long long)(16 * J + 4 * q + lq) * TP + 16 * tc + lr], acc);
+          acc = MF<Real>::mma(tji[MF<Real>::off(4 * q + lq, lr)], Xg[(long long)(16 * J + 4 * q + lq) * TP + 16 * tc + lr], acc);
       }
 #pragma unroll
       for (int r = 0; r < 4; ++r) R[wave * kTE + r * 64 + lane] = acc[r];
@@ -315,8 +321,8 @@ __global__ __launch_bounds__(NW * 64) void k_block_chol(
         for (int r = 0; r < 4; ++r) {
           Real s = R[r * 64 + lane];
           for (int w = 1; w < NW; ++w) s += R[w * kTE + r * 64 + lane];
-          Real yv = (I >= I0) ? Xg[(long long)(16 * I + 4 * lq + r) * TP + 16 * tc + lr] : Real(0);
-          Sw[(4 * lq + r) * kLS + lr] = yv - s;
+          Real yv = (I >= I0) ? Xg[(long long)(16 * I + MF<Real>::row(r, lq)) * TP + 16 * tc + lr] : Real(0);
+          Sw[MF<Real>::row(r, lq) * kLS + lr] = yv - s;
         }
         wave_fence();
         const Real* li = Wd + (long long)I * kTE;
@@ -324,7 +330,7 @@ __global__ __launch_bounds__(NW * 64) void k_block_chol(
 #pragma unroll
         for (int q = 0; q < 4; ++q) x = MF<Real>::mma(li[(4 * q + lq) * 16 + lr], Sw[(4 * q + lq) * kLS + lr], x);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) Xg[(long long)(16 * I + 4 * lq + r) * TP + 16 * tc + lr] = x[r];
+        for (int r = 0; r < 4; ++r) Xg[(long long)(16 * I + MF<Real>::row(r, lq)) * TP + 16 * tc + lr] = x[r];
       }
       __syncthreads();
     }
