@@ -138,7 +138,10 @@ def main():
         torch.cuda.synchronize()
         true_rel = float(torch.linalg.norm(wv - w) / torch.linalg.norm(w))
         log(f"{mode}: {args.steps} steps {elapsed:.3f}s; its {inf.its} reason {inf.reason} true rel resid {true_rel:.3e}")
+        ci = ctx.info()
+        log(f"{mode}: block factorisation config waves={ci['chol_waves']} panel={ci['chol_panel']}")
         results[mode] = dict(ms_per_step=elapsed / args.steps * 1e3, its=inf.its, reason=inf.reason,
+                             chol_waves=ci["chol_waves"], chol_panel=ci["chol_panel"],
                              t_setup_s=inf.t_setup_s, t_solve_s=inf.t_solve_s, t_matvec_s=inf.t_matvec_s,
                              n_matvec=inf.n_matvec, t_pc_s=inf.t_pc_s, n_pc=inf.n_pc,
                              n_fallback=inf.n_fallback, true_rel_residual=true_rel)

@@ -59,7 +59,7 @@ typedef enum { PRBF_KSP_GMRES = 0, PRBF_KSP_FGMRES = 1 } prbf_ksp;
  * RASM_FP32: the same blocks factorised and solved in FP32 on the matrix
  * cores, applied with FP64 accumulation; meant for FGMRES. */
 typedef enum { PRBF_PC_NONE = 0, PRBF_PC_RASM_FP64 = 1, PRBF_PC_RASM_FP32 = 2 } prbf_pc;
-typedef enum { PRBF_TUNE_CHOL_WAVES = 1 } prbf_tune_key;
+typedef enum { PRBF_TUNE_CHOL_WAVES = 1, PRBF_TUNE_CHOL_PANEL = 2 } prbf_tune_key;
 
 /* Solver options; prbf_solve_opts_default() gives the reference's run line
  * (-ksp_type gmres default restart 30, -pc_type asm -sub_pc_type lu,
@@ -110,7 +110,8 @@ prbf_status prbf_set_points_dev(prbf_ctx* ctx, int64_t ni, const double* xi, con
                                 int64_t nj, const double* xj, const double* yj, const double* zj);
 
 /* info[0..15]: ncell, nx, ny, nz, nb_buffer, nb_trunc, buffer_nnz, trunc_nnz,
- * max_buffer, max_core, max_trunc, ni, nj, 0, 0, 0.
+ * max_buffer, max_core, max_trunc, ni, nj, and the wavefronts and panel width
+ * the last block factorisation used (0 before any), 0.
  * geom[0..5]: xmin, ymin, zmin, box, buffer half-width, trunc half-width. */
 prbf_status prbf_get_info(prbf_ctx* ctx, int64_t* info, double* geom);
 /* Sorted position -> original index (the reference's particle->ii / jj). */
@@ -146,8 +147,10 @@ prbf_status prbf_rasm_setup(prbf_ctx* ctx);
 /* Same for a chosen prbf_pc (RASM_FP64 or RASM_FP32).  No reference
  * counterpart for FP32: it is the mixed-precision variant of the same PC. */
 prbf_status prbf_rasm_setup_pc(prbf_ctx* ctx, int pc);
-/* Tuning knobs of the device kernels (PRBF_TUNE_CHOL_WAVES: 4 or 8
- * wavefronts per block-factorisation workgroup).  No reference counterpart. */
+/* Force a launch configuration of the block factorisation instead of the
+ * built-in autotuner: PRBF_TUNE_CHOL_WAVES (4 or 8 wavefronts per workgroup),
+ * PRBF_TUNE_CHOL_PANEL (1 or 2 16-wide tile columns per panel).  No
+ * reference counterpart. */
 prbf_status prbf_set_tuning(prbf_ctx* ctx, int key, int value);
 /* PCApply_ASM (restrict): z = sum_c P_core A_c^{-1} R_buffer r, sorted order. */
 prbf_status prbf_rasm_apply(prbf_ctx* ctx, const double* r_sorted, double* z_sorted);

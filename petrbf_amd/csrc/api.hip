@@ -82,7 +82,10 @@ struct prbf_ctx {
   double* pin = nullptr;  // pinned host scalars
   // Krylov
   DBuf<double> V, Z;
-  int chol_waves = 4;  // wavefronts per block-factorisation workgroup
+  int chol_waves = 8;  // wavefronts per block-factorisation workgroup (when forced)
+  int chol_panel = 2;  // tile columns per panel (when forced)
+  bool tuning_forced = false;
+  int used_nw = 0, used_npc = 0;  // configuration of the last factorisation
   int rasm_pc = 0;
   int n_fallback = 0;
   DBuf<float> M32;
@@ -271,14 +274,44 @@ void matvec_impl(prbf_ctx* c, int nrhs, const double* g, double* w) {
 
 constexpr size_t kMaxLdsBytes = 160 * 1024;
 
+struct CholCfg {
+  int nw, npc;
+};
+
+// Launch configurations the tuner chooses among (wavefronts per workgroup,
+// tile columns per panel).
+const CholCfg kCholCfgs[] = {{4, 1}, {8, 1}, {4, 2}, {8, 2}};
+
+// Process-wide cache of tuned configurations, keyed by precision and the
+// largest block's tile count.
+std::vector<std::pair<long long, CholCfg>>& tuned_cache() {
+  static std::vector<std::pair<long long, CholCfg>> v;
+  return v;
+}
+
+template <typename Real>
+void launch_chunk(prbf_ctx* c, bool fp32, const CholCfg& cfg, long long k0, long long ncl, int nmax, int tmax,
+                  const long long* wptr, Real* W, Real* Mout, int* status) {
+  const Grid& g = c->g;
+  if (fp32)
+    block_chol_fp32(c->st, g.dim, cfg.nw, cfg.npc, k0, ncl, nmax, tmax, c->bptr, c->bidx, c->ista, c->iend, c->xi,
+                    c->yi, c->zi, wptr, (float*)W, c->mptr, (float*)Mout, c->prm.sigma, status);
+  else
+    block_chol_fp64(c->st, g.dim, cfg.nw, cfg.npc, k0, nullptr, ncl, nmax, tmax, c->bptr, c->bidx, c->ista, c->iend,
+                    c->xi, c->yi, c->zi, wptr, (double*)W, c->mptr, (double*)Mout, c->prm.sigma, c->d_err);
+  PRBF_HIP(hipGetLastError());
+}
+
 // Runs the block factorisation over all clusters in chunks bounded by a
 // workspace budget.  fp32: blocks in FP32 on the FP32 MFMA, core rows stored
-// in FP32, failed blocks listed in `failed` for an FP64 redo.
+// in FP32, failed blocks listed in `failed` for an FP64 redo.  Unless a
+// configuration was forced, the first call for a (precision, block size)
+// times every launch configuration that fits in LDS on the first
+// min(4096, chunk) blocks and keeps the fastest (the project's autotuner).
 template <typename Real>
 void factor_chunks(prbf_ctx* c, bool fp32, const std::vector<long long>& wsz, Real* Mout, std::vector<int>* failed) {
   const Grid& g = c->g;
   const long long nc = g.ncell;
-  const int nw = c->chol_waves;
   size_t free_b = 0, total_b = 0;
   PRBF_HIP(hipMemGetInfo(&free_b, &total_b));
   long long budget = (long long)std::min<size_t>(free_b / 3, (size_t)8 << 30) / (long long)sizeof(Real);
@@ -294,6 +327,14 @@ void factor_chunks(prbf_ctx* c, bool fp32, const std::vector<long long>& wsz, Re
   std::vector<long long> h_wptr;
   const long long kMaxChunk = 1LL << 20;
   wptr.ensure(std::min<long long>(nc, kMaxChunk) + 1);
+  // configuration: forced, cached, or tuned on the first chunk
+  const int ktmax = (c->max_buffer + 15) / 16;
+  const long long key = (fp32 ? 1000000LL : 0LL) + ktmax * 1000LL + g.dim;
+  CholCfg cfg{c->chol_waves, c->chol_panel};
+  bool tune = !c->tuning_forced;
+  if (tune)
+    for (auto& kv : tuned_cache())
+      if (kv.first == key) { cfg = kv.second; tune = false; }
   long long k0 = 0;
   while (k0 < nc) {
     long long k1 = k0, used = 0;
@@ -310,15 +351,27 @@ void factor_chunks(prbf_ctx* c, bool fp32, const std::vector<long long>& wsz, Re
     W.ensure((size_t)std::max<long long>(used, 1));
     sync(c);  // the previous chunk has consumed wptr before it is overwritten
     PRBF_HIP(hipMemcpy(wptr, h_wptr.data(), sizeof(long long) * h_wptr.size(), hipMemcpyHostToDevice));
-    if (fp32)
-      block_chol_fp32(c->st, g.dim, nw, k0, k1 - k0, nmax, tmax, c->bptr, c->bidx, c->ista, c->iend, c->xi, c->yi,
-                      c->zi, wptr, (float*)W.p, c->mptr, (float*)Mout, c->prm.sigma, status);
-    else
-      block_chol_fp64(c->st, g.dim, nw, k0, nullptr, k1 - k0, nmax, tmax, c->bptr, c->bidx, c->ista, c->iend,
-                      c->xi, c->yi, c->zi, wptr, (double*)W.p, c->mptr, (double*)Mout, c->prm.sigma, c->d_err);
-    PRBF_HIP(hipGetLastError());
+    if (tune) {
+      const long long ns = std::min<long long>(4096, k1 - k0);
+      float best = 1e30f;
+      for (const CholCfg& cand : kCholCfgs) {
+        if (block_lds_bytes(fp32, c->max_buffer, c->max_core, cand.nw, cand.npc) > kMaxLdsBytes) continue;
+        PRBF_HIP(hipEventRecord(c->ev0, c->st));
+        launch_chunk<Real>(c, fp32, cand, k0, ns, nmax, tmax, wptr, W.p, Mout, status);
+        PRBF_HIP(hipEventRecord(c->ev1, c->st));
+        PRBF_HIP(hipEventSynchronize(c->ev1));
+        float ms = 0;
+        PRBF_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        if (ms < best) { best = ms; cfg = cand; }
+      }
+      tuned_cache().push_back({key, cfg});
+      tune = false;
+    }
+    launch_chunk<Real>(c, fp32, cfg, k0, k1 - k0, nmax, tmax, wptr, W.p, Mout, status);
     k0 = k1;
   }
+  c->used_nw = cfg.nw;
+  c->used_npc = cfg.npc;
   sync(c);
   if (fp32 && failed) {
     std::vector<int> h(nc);
@@ -344,7 +397,7 @@ void rasm_setup_impl(prbf_ctx* c, int pc) {
     c->h_mptr[k + 1] = c->h_mptr[k] + n * T;
     wsz[k] = n ? block_workspace((int)n, (int)T) : 0;
   }
-  need(block_lds_bytes(fp32, c->max_buffer, c->max_core, c->chol_waves) <= kMaxLdsBytes, PRBF_ERR_LIMIT,
+  need(block_lds_bytes(fp32, c->max_buffer, c->max_core, 4, 1) <= kMaxLdsBytes, PRBF_ERR_LIMIT,
        "RASM block too large for the LDS panel of the block factorisation");
   c->rasm_ready = false;
   c->mptr.ensure(nc + 1);
@@ -389,7 +442,7 @@ void rasm_setup_impl(prbf_ctx* c, int pc) {
       PRBF_HIP(hipMemcpyAsync(wp, h_w.data(), sizeof(long long) * h_w.size(), hipMemcpyHostToDevice, c->st));
       PRBF_HIP(hipMemcpyAsync(mp, h_m64.data(), sizeof(long long) * (nc + 1), hipMemcpyHostToDevice, c->st));
       PRBF_HIP(hipMemcpyAsync(cl, failed.data(), sizeof(int) * failed.size(), hipMemcpyHostToDevice, c->st));
-      block_chol_fp64(c->st, g.dim, c->chol_waves, 0, cl, (long long)failed.size(), nmax, tmax, c->bptr, c->bidx,
+      block_chol_fp64(c->st, g.dim, 4, 1, 0, cl, (long long)failed.size(), nmax, tmax, c->bptr, c->bidx,
                       c->ista, c->iend, c->xi, c->yi, c->zi, wp, W, mp, c->M64fb, c->prm.sigma, c->d_err);
       PRBF_HIP(hipGetLastError());
       sync(c);
@@ -731,7 +784,7 @@ prbf_status prbf_get_info(prbf_ctx* c, int64_t* info, double* geom) {
     const Grid& g = c->g;
     if (info) {
       int64_t v[16] = {g.ncell, g.nx, g.ny, g.nz, g.nb_buffer, g.nb_trunc, c->buffer_nnz, c->trunc_nnz,
-                       c->max_buffer, c->max_core, c->max_trunc, c->ni, c->nj, 0, 0, 0};
+                       c->max_buffer, c->max_core, c->max_trunc, c->ni, c->nj, c->used_nw, c->used_npc, 0};
       std::memcpy(info, v, sizeof(v));
     }
     if (geom) {
@@ -852,9 +905,16 @@ prbf_status prbf_rasm_setup_pc(prbf_ctx* c, int pc) {
 
 prbf_status prbf_set_tuning(prbf_ctx* c, int key, int value) {
   return guarded(c, [&] {
-    need(key == PRBF_TUNE_CHOL_WAVES, PRBF_ERR_ARG, "unknown tuning key");
-    need(value == 4 || value == 8, PRBF_ERR_ARG, "waves per block factorisation must be 4 or 8");
-    c->chol_waves = value;
+    if (key == PRBF_TUNE_CHOL_WAVES) {
+      need(value == 4 || value == 8, PRBF_ERR_ARG, "waves per block factorisation must be 4 or 8");
+      c->chol_waves = value;
+    } else if (key == PRBF_TUNE_CHOL_PANEL) {
+      need(value == 1 || value == 2, PRBF_ERR_ARG, "panel width must be 1 or 2 tile columns");
+      c->chol_panel = value;
+    } else {
+      need(false, PRBF_ERR_ARG, "unknown tuning key");
+    }
+    c->tuning_forced = true;
   });
 }
 

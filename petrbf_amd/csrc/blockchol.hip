@@ -10,10 +10,12 @@
 //   phase 0  order the buffer with the core LAST and padding (identity) up to
 //            NP = 16*ceil(n/16); write the lower tiles of A' to a workspace in
 //            the MFMA accumulator layout (one coalesced access per register);
-//   phase 1  right-looking tiled Cholesky A' = L L^T: the 16x16 diagonal tile
-//            by one wavefront (in LDS), L_kk^{-1} beside it, the panel as
-//            A_ik L_kk^{-T} on MFMA, then the trailing update
-//            A_ij -= L_ik L_jk^T on MFMA with the panel in LDS;
+//   phase 1  right-looking tiled Cholesky A' = L L^T over panels of NPC tile
+//            columns: per column, the 16x16 diagonal tile by one wavefront
+//            (in LDS) with L_kk^{-1} beside it and the column as
+//            A_ik L_kk^{-T} on MFMA; then one trailing update
+//            A_ij -= sum L_ic L_jc^T per panel on MFMA with the panel in LDS
+//            (NPC = 2 halves the passes over the trailing tiles);
 //   phase 2  with the core last, L^{-1} E_core is zero above the core, so
 //            A'^{-1} E_core = L^{-T} (L^{-1} E_core) is a short forward and a
 //            full backward tile substitution, both MFMA with the
@@ -90,10 +92,10 @@ __host__ __device__ inline long long blk_ws(int n, int T) {
 }
 
 template <typename T>
-__host__ __device__ inline size_t blk_lds_bytes(int nmax, int tmax, int nw) {
+__host__ __device__ inline size_t blk_lds_bytes(int nmax, int tmax, int nw, int npc) {
   int KT = (nmax + 15) / 16, NP = 16 * KT;
   (void)tmax;
-  size_t pan = (size_t)KT * kLT * sizeof(T);
+  size_t pan = (size_t)npc * KT * kLT * sizeof(T);
   size_t crd = (size_t)3 * NP * sizeof(double);
   size_t red = (size_t)nw * kTE * sizeof(T);
   size_t region = pan > crd ? pan : crd;
@@ -102,7 +104,7 @@ __host__ __device__ inline size_t blk_lds_bytes(int nmax, int tmax, int nw) {
   return region + (size_t)(nw + 2) * kLT * sizeof(T) + 16;
 }
 
-template <typename Real, int DIM, int NW>
+template <typename Real, int DIM, int NW, int NPC>
 __global__ __launch_bounds__(NW * 64) void k_block_chol(
     long long c0, const int* __restrict__ clist, const long long* __restrict__ bptr, const int* __restrict__ bidx,
     const int* __restrict__ ista, const int* __restrict__ iend, const double* __restrict__ xs,
@@ -127,12 +129,12 @@ __global__ __launch_bounds__(NW * 64) void k_block_chol(
   Real* Xg = Wd + (long long)KT * kTE;       // NP x TP
 
   // LDS carve-up
-  size_t pan = (size_t)KT * kLT * sizeof(Real), crd = (size_t)3 * NP * sizeof(double),
+  size_t pan = (size_t)NPC * KT * kLT * sizeof(Real), crd = (size_t)3 * NP * sizeof(double),
          red = (size_t)NW * kTE * sizeof(Real);
   size_t region = pan > crd ? pan : crd;
   region = region > red ? region : red;
   region = (region + 15) & ~(size_t)15;
-  Real* P = (Real*)lds_raw;                     // panel: KT tiles, row-major stride 17
+  Real* P = (Real*)lds_raw;                     // panel: NPC x KT tiles, row-major stride 17
   double* cx = (double*)lds_raw;          // phase 0 coordinates (same region)
   double* cy = cx + NP;
   double* cz = cy + NP;
@@ -171,98 +173,140 @@ __global__ __launch_bounds__(NW * 64) void k_block_chol(
   }
   __syncthreads();
 
-  // ---- phase 1: tiled right-looking Cholesky
-  for (int kk = 0; kk < KT; ++kk) {
-    if (wave == 0) {
-      Real* tkk = Wb + (long long)ltile(kk, kk) * kTE;
+  // ---- phase 1: tiled right-looking Cholesky, NPC tile columns per panel
+  for (int k0 = 0; k0 < KT; k0 += NPC) {
+    const int npc = min(NPC, KT - k0);
+    for (int cc = 0; cc < npc; ++cc) {
+      const int kk = k0 + cc;
+      if (cc > 0) {
+        // bring column kk up to date with the earlier columns of this panel
+        for (int i = kk + wave; i < KT; i += NW) {
+          Real* tik = Wb + (long long)ltile(i, kk) * kTE;
+          v4 acc;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) D[MF<Real>::row(r, lq) * kLS + lr] = tkk[r * 64 + lane];
-      wave_fence();
-      for (int p = 0; p < 16; ++p) {
-        Real d = D[p * kLS + p];
-        bool bad = !(d > Real(0));
-        Real s = bad ? Real(1) : sqrt(d);
-        Real rs = Real(1) / s;
-        wave_fence();
-        if (lane == 0 && bad) *flag = 1;
-        if (lane > p && lane < 16) D[lane * kLS + p] *= rs;
-        if (lane == p) D[p * kLS + p] = s;
-        wave_fence();
-        if (lane > p && lane < 16) {
-          Real lp = D[lane * kLS + p];
-          for (int q = p + 1; q <= lane; ++q) D[lane * kLS + q] -= lp * D[q * kLS + p];
-        }
-        wave_fence();
-      }
-      // zero the strict upper part, invert the lower triangle column by column
-      if (lane < 16)
-        for (int q = lane + 1; q < 16; ++q) D[lane * kLS + q] = Real(0);
-      wave_fence();
-      if (lane < 16) {
-        const int jcol = lane;
-        for (int i = 0; i < 16; ++i) {
-          Real x;
-          if (i < jcol) {
-            x = Real(0);
-          } else {
-            x = (i == jcol) ? Real(1) : Real(0);
-            for (int q = jcol; q < i; ++q) x -= D[i * kLS + q] * Li[q * kLS + jcol];
-            x /= D[i * kLS + i];
+          for (int r = 0; r < 4; ++r) acc[r] = tik[r * 64 + lane];
+          for (int pc = 0; pc < cc; ++pc) {
+            const Real* pi = P + (pc * KT + i) * kLT;
+            const Real* pk = P + (pc * KT + kk) * kLT;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+              acc = MF<Real>::mma(-pi[lr * kLS + 4 * q + lq], pk[lr * kLS + 4 * q + lq], acc);
           }
-          Li[i * kLS + jcol] = x;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) tik[r * 64 + lane] = acc[r];
+        }
+        __syncthreads();
+      }
+      if (wave == 0) {
+        Real* tkk = Wb + (long long)ltile(kk, kk) * kTE;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) D[MF<Real>::row(r, lq) * kLS + lr] = tkk[r * 64 + lane];
+        wave_fence();
+        for (int p = 0; p < 16; ++p) {
+          Real d = D[p * kLS + p];
+          bool bad = !(d > Real(0));
+          Real s = bad ? Real(1) : sqrt(d);
+          Real rs = Real(1) / s;
+          wave_fence();
+          if (lane == 0 && bad) *flag = 1;
+          if (lane > p && lane < 16) D[lane * kLS + p] *= rs;
+          if (lane == p) D[p * kLS + p] = s;
+          wave_fence();
+          if (lane > p && lane < 16) {
+            Real lp = D[lane * kLS + p];
+            for (int q = p + 1; q <= lane; ++q) D[lane * kLS + q] -= lp * D[q * kLS + p];
+          }
+          wave_fence();
+        }
+        if (lane < 16)
+          for (int q = lane + 1; q < 16; ++q) D[lane * kLS + q] = Real(0);
+        wave_fence();
+        if (lane < 16) {
+          const int jcol = lane;
+          for (int i = 0; i < 16; ++i) {
+            Real x;
+            if (i < jcol) {
+              x = Real(0);
+            } else {
+              x = (i == jcol) ? Real(1) : Real(0);
+              for (int q = jcol; q < i; ++q) x -= D[i * kLS + q] * Li[q * kLS + jcol];
+              x /= D[i * kLS + i];
+            }
+            Li[i * kLS + jcol] = x;
+          }
+        }
+        wave_fence();
+#pragma unroll
+        for (int r = 0; r < 4; ++r) tkk[r * 64 + lane] = D[MF<Real>::row(r, lq) * kLS + lr];
+        Real* wd = Wd + (long long)kk * kTE;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int e = r * 64 + lane;
+          wd[e] = Li[(e >> 4) * kLS + (e & 15)];
         }
       }
-      wave_fence();
+      __syncthreads();
+      // panel column: L_ik = A_ik L_kk^{-T}
+      for (int i = kk + 1 + wave; i < KT; i += NW) {
+        Real* tik = Wb + (long long)ltile(i, kk) * kTE;
+        Real a[4];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) tkk[r * 64 + lane] = D[MF<Real>::row(r, lq) * kLS + lr];
-      Real* wd = Wd + (long long)kk * kTE;
+        for (int r = 0; r < 4; ++r) a[r] = tik[r * 64 + lane];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        int e = r * 64 + lane;
-        wd[e] = Li[(e >> 4) * kLS + (e & 15)];
+        for (int r = 0; r < 4; ++r) Sw[MF<Real>::row(r, lq) * kLS + lr] = a[r];
+        wave_fence();
+        v4 acc = {0, 0, 0, 0};
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          acc = MF<Real>::mma(Sw[lr * kLS + 4 * q + lq], Li[lr * kLS + 4 * q + lq], acc);
+        Real* pi = P + (cc * KT + i) * kLT;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          tik[r * 64 + lane] = acc[r];
+          pi[MF<Real>::row(r, lq) * kLS + lr] = acc[r];
+        }
+        wave_fence();
       }
+      __syncthreads();
     }
-    __syncthreads();
-    // panel: L_ik = A_ik L_kk^{-T}
-    for (int i = kk + 1 + wave; i < KT; i += NW) {
-      Real* tik = Wb + (long long)ltile(i, kk) * kTE;
-      Real a[4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) a[r] = tik[r * 64 + lane];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) Sw[MF<Real>::row(r, lq) * kLS + lr] = a[r];
-      wave_fence();
-      v4 acc = {0, 0, 0, 0};
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-        acc = MF<Real>::mma(Sw[lr * kLS + 4 * q + lq], Li[lr * kLS + 4 * q + lq], acc);
-      Real* pi = P + i * kLT;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        tik[r * 64 + lane] = acc[r];
-        pi[MF<Real>::row(r, lq) * kLS + lr] = acc[r];
-      }
-      wave_fence();
-    }
-    __syncthreads();
-    // trailing update: A_ij -= L_ik L_jk^T for kk < j <= i < KT
-    const int m = KT - kk - 1;
+    // trailing update: A_ij -= sum_c L_ic L_jc^T for k0+npc <= j <= i < KT,
+    // kG tiles in flight per wavefront
+    const int j0 = k0 + npc;
+    const int m = KT - j0;
     const int ntr = m * (m + 1) / 2;
-    for (int u = wave; u < ntr; u += NW) {
-      int ii, jj;
-      tri_decode(u, &ii, &jj);
-      const int i = kk + 1 + ii, j = kk + 1 + jj;
-      Real* tij = Wb + (long long)ltile(i, j) * kTE;
-      v4 acc;
+    constexpr int kG = 4;
+    for (int u0 = wave; u0 < ntr; u0 += NW * kG) {
+      v4 acc[kG];
+      Real* tp[kG];
+      int ti[kG], tj[kG];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) acc[r] = tij[r * 64 + lane];
-      const Real* pi = P + i * kLT;
-      const Real* pj = P + j * kLT;
+      for (int g = 0; g < kG; ++g) {
+        const int u = u0 + g * NW;
+        tp[g] = nullptr;
+        if (u < ntr) {
+          int ii, jj;
+          tri_decode(u, &ii, &jj);
+          ti[g] = j0 + ii;
+          tj[g] = j0 + jj;
+          tp[g] = Wb + (long long)ltile(ti[g], tj[g]) * kTE;
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-        acc = MF<Real>::mma(-pi[lr * kLS + 4 * q + lq], pj[lr * kLS + 4 * q + lq], acc);
+          for (int r = 0; r < 4; ++r) acc[g][r] = tp[g][r * 64 + lane];
+        }
+      }
 #pragma unroll
-      for (int r = 0; r < 4; ++r) tij[r * 64 + lane] = acc[r];
+      for (int g = 0; g < kG; ++g) {
+        if (tp[g]) {
+          for (int pc = 0; pc < npc; ++pc) {
+            const Real* pi = P + (pc * KT + ti[g]) * kLT;
+            const Real* pj = P + (pc * KT + tj[g]) * kLT;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+              acc[g] = MF<Real>::mma(-pi[lr * kLS + 4 * q + lq], pj[lr * kLS + 4 * q + lq], acc[g]);
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r) tp[g][r * 64 + lane] = acc[g][r];
+        }
+      }
     }
     __syncthreads();
   }
@@ -347,52 +391,58 @@ __global__ __launch_bounds__(NW * 64) void k_block_chol(
 }
 
 template <typename T>
-static void launch_block_chol(hipStream_t st, int dim, int nw, long long c0, const int* clist, long long ncl,
-                              int nmax, int tmax, const long long* bptr, const int* bidx, const int* ista, const int* iend,
-                              const double* xs, const double* ys, const double* zs, const long long* wptr, T* W,
-                              const long long* mptr, T* M, double sigma, int* err, int* blk_status) {
+static void launch_block_chol(hipStream_t st, int dim, int nw, int npc, long long c0, const int* clist, long long ncl,
+                              int nmax, int tmax, const long long* bptr, const int* bidx, const int* ista,
+                              const int* iend, const double* xs, const double* ys, const double* zs,
+                              const long long* wptr, T* W, const long long* mptr, T* M, double sigma, int* err,
+                              int* blk_status) {
   Kern k = make_kern(sigma);
-  size_t lds = blk_lds_bytes<T>(nmax, tmax, nw);
+  size_t lds = blk_lds_bytes<T>(nmax, tmax, nw, npc);
   static bool attr = false;
   if (!attr) {
-    const void* fns[] = {(const void*)k_block_chol<T, 2, 4>, (const void*)k_block_chol<T, 3, 4>,
-                         (const void*)k_block_chol<T, 2, 8>, (const void*)k_block_chol<T, 3, 8>};
+    const void* fns[] = {(const void*)k_block_chol<T, 2, 4, 1>, (const void*)k_block_chol<T, 3, 4, 1>,
+                         (const void*)k_block_chol<T, 2, 8, 1>, (const void*)k_block_chol<T, 3, 8, 1>,
+                         (const void*)k_block_chol<T, 2, 4, 2>, (const void*)k_block_chol<T, 3, 4, 2>,
+                         (const void*)k_block_chol<T, 2, 8, 2>, (const void*)k_block_chol<T, 3, 8, 2>};
     for (const void* f : fns)
       PRBF_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     attr = true;
   }
   dim3 grid((unsigned)ncl);
-#define PRBF_BC(D, NWV)                                                                                   \
-  hipLaunchKernelGGL((k_block_chol<T, D, NWV>), grid, dim3(NWV * 64), lds, st, c0, clist, bptr, bidx, ista, iend, xs, \
-                     ys, zs, wptr, W, mptr, M, k.inv_two_s2, k.norm, err, blk_status)
+#define PRBF_BC(D, NWV, NPCV)                                                                                  \
+  hipLaunchKernelGGL((k_block_chol<T, D, NWV, NPCV>), grid, dim3(NWV * 64), lds, st, c0, clist, bptr, bidx, ista, \
+                     iend, xs, ys, zs, wptr, W, mptr, M, k.inv_two_s2, k.norm, err, blk_status)
+#define PRBF_BC_NPC(D, NWV) \
+  if (npc == 2) PRBF_BC(D, NWV, 2); else PRBF_BC(D, NWV, 1)
   if (dim == 2) {
-    if (nw == 8) PRBF_BC(2, 8); else PRBF_BC(2, 4);
+    if (nw == 8) { PRBF_BC_NPC(2, 8); } else { PRBF_BC_NPC(2, 4); }
   } else {
-    if (nw == 8) PRBF_BC(3, 8); else PRBF_BC(3, 4);
+    if (nw == 8) { PRBF_BC_NPC(3, 8); } else { PRBF_BC_NPC(3, 4); }
   }
+#undef PRBF_BC_NPC
 #undef PRBF_BC
 }
 
 long long block_workspace(int n, int T) { return blk_ws(n, T); }
 
-size_t block_lds_bytes(bool fp32, int nmax, int tmax, int nw) {
-  return fp32 ? blk_lds_bytes<float>(nmax, tmax, nw) : blk_lds_bytes<double>(nmax, tmax, nw);
+size_t block_lds_bytes(bool fp32, int nmax, int tmax, int nw, int npc) {
+  return fp32 ? blk_lds_bytes<float>(nmax, tmax, nw, npc) : blk_lds_bytes<double>(nmax, tmax, nw, npc);
 }
 
-void block_chol_fp64(hipStream_t st, int dim, int nw, long long c0, const int* clist, long long ncl, int nmax,
-                     int tmax, const long long* bptr, const int* bidx, const int* ista, const int* iend,
+void block_chol_fp64(hipStream_t st, int dim, int nw, int npc, long long c0, const int* clist, long long ncl,
+                     int nmax, int tmax, const long long* bptr, const int* bidx, const int* ista, const int* iend,
                      const double* xs, const double* ys, const double* zs, const long long* wptr, double* W,
                      const long long* mptr, double* M, double sigma, int* err) {
-  launch_block_chol<double>(st, dim, nw, c0, clist, ncl, nmax, tmax, bptr, bidx, ista, iend, xs, ys, zs, wptr, W,
-                            mptr, M, sigma, err, nullptr);
+  launch_block_chol<double>(st, dim, nw, npc, c0, clist, ncl, nmax, tmax, bptr, bidx, ista, iend, xs, ys, zs, wptr,
+                            W, mptr, M, sigma, err, nullptr);
 }
 
-void block_chol_fp32(hipStream_t st, int dim, int nw, long long c0, long long ncl, int nmax, int tmax,
+void block_chol_fp32(hipStream_t st, int dim, int nw, int npc, long long c0, long long ncl, int nmax, int tmax,
                      const long long* bptr, const int* bidx, const int* ista, const int* iend, const double* xs,
                      const double* ys, const double* zs, const long long* wptr, float* W, const long long* mptr,
                      float* M, double sigma, int* blk_status) {
-  launch_block_chol<float>(st, dim, nw, c0, nullptr, ncl, nmax, tmax, bptr, bidx, ista, iend, xs, ys, zs, wptr, W,
-                           mptr, M, sigma, nullptr, blk_status);
+  launch_block_chol<float>(st, dim, nw, npc, c0, nullptr, ncl, nmax, tmax, bptr, bidx, ista, iend, xs, ys, zs, wptr,
+                           W, mptr, M, sigma, nullptr, blk_status);
 }
 
 }  // namespace prbf

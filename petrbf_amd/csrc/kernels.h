@@ -28,14 +28,14 @@ void matvec(hipStream_t st, int dim, int nrhs, long long ncell, const int* tsta,
 // blockchol.hip / rasm.hip ----------------------------------------------------
 // Workspace (elements) and LDS bytes of one block factorisation.
 long long block_workspace(int n, int T);
-size_t block_lds_bytes(bool fp32, int nmax, int tmax, int nw);
+size_t block_lds_bytes(bool fp32, int nmax, int tmax, int nw, int npc);
 // Factorises the blocks of clusters [c0, c0+ncl) and writes their core rows.
 // clist != nullptr: factorise clusters clist[0..ncl) instead of [c0, c0+ncl).
-void block_chol_fp64(hipStream_t st, int dim, int nw, long long c0, const int* clist, long long ncl, int nmax,
-                     int tmax, const long long* bptr, const int* bidx, const int* ista, const int* iend,
+void block_chol_fp64(hipStream_t st, int dim, int nw, int npc, long long c0, const int* clist, long long ncl,
+                     int nmax, int tmax, const long long* bptr, const int* bidx, const int* ista, const int* iend,
                      const double* xs, const double* ys, const double* zs, const long long* wptr, double* W,
                      const long long* mptr, double* M, double sigma, int* err);
-void block_chol_fp32(hipStream_t st, int dim, int nw, long long c0, long long ncl, int nmax, int tmax,
+void block_chol_fp32(hipStream_t st, int dim, int nw, int npc, long long c0, long long ncl, int nmax, int tmax,
                      const long long* bptr, const int* bidx, const int* ista, const int* iend, const double* xs,
                      const double* ys, const double* zs, const long long* wptr, float* W, const long long* mptr,
                      float* M, double sigma, int* blk_status);

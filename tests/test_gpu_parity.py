@@ -198,23 +198,6 @@ def test_errors_are_reported_not_crashed():
     ctx.close()
 
 
-def test_block_factorisation_waves_agree():
-    """The 4- and 8-wavefront variants of the block factorisation compute the
-    same core rows (only the order of the cross-wave sums differs)."""
-    _announce("chol waves 4 vs 8")
-    p = lattice(40)
-    rows = []
-    for nw in (4, 8):
-        ctx = Context.from_problem(p)
-        ctx.set_tuning(chol_waves=nw)
-        ctx.set_points(p.x, p.y)
-        ctx.rasm_setup()
-        rows.append([ctx.core_rows(c) for c in range(0, ctx.ncell, 7)])
-        ctx.close()
-    for a, b in zip(*rows):
-        assert np.abs(a - b).max() <= 1e-9 * max(np.abs(a).max(), 1e-300)
-
-
 def test_mixed_fp32_rasm_fgmres_reaches_fp64_residual():
     """Chosen form: FP32 blocks on the matrix cores inside FP64 FGMRES(30).
     The true FP64 residual reaches rtol 1e-10 and the analytic-field error
@@ -251,3 +234,26 @@ def test_mixed_core_rows_close_to_fp64():
     _announce(f"fp32 vs fp64 RASM apply rel diff {rel:.2e}")
     assert rel < 0.5
     ctx.close()
+
+
+@pytest.mark.parametrize("prec", ["rasm", "rasm32"])
+def test_block_factorisation_launch_configs_agree(prec):
+    """Every launch configuration the autotuner may pick (4/8 wavefronts,
+    1/2 tile columns per panel) yields the same preconditioner."""
+    _announce(f"chol configs {prec}")
+    p = lattice(40)
+    rng = np.random.default_rng(2)
+    r = rng.standard_normal(p.n)
+    outs = []
+    for nw in (4, 8):
+        for npc in (1, 2):
+            ctx = Context.from_problem(p)
+            ctx.set_tuning(chol_waves=nw, chol_panel=npc)
+            ctx.set_points(p.x, p.y)
+            ctx.rasm_setup(prec)
+            assert (ctx.info()["chol_waves"], ctx.info()["chol_panel"]) == (nw, npc)
+            outs.append(ctx.rasm_apply(r))
+            ctx.close()
+    tol = 1e-9 if prec == "rasm" else 1e-2
+    for o in outs[1:]:
+        assert np.linalg.norm(o - outs[0]) <= tol * np.linalg.norm(outs[0])
