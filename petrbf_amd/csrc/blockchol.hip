@@ -19,8 +19,9 @@
 //   phase 2  with the core last, L^{-1} E_core is zero above the core, so
 //            A'^{-1} E_core = L^{-T} (L^{-1} E_core) is a short forward and a
 //            full backward tile substitution, both MFMA with the
-//            L_II^{-1} tiles kept from phase 1;
-//   phase 3  scatter X to M_c in buffer order.
+//            L_II^{-1} tiles kept from phase 1, 16 right-hand sides at a
+//            time held in LDS;
+//   phase 3  scatter each 16-column slice of X to M_c in buffer order.
 // T = double is the plain path (exact FP64 block solves); T = float runs the
 // same kernel in FP32 on the FP32 MFMA (mixed precision), with a per-block
 // status so a block whose FP32 factorisation breaks down is redone in FP64.
@@ -72,6 +73,17 @@ __device__ inline void tri_decode(int t, int* I, int* J) {
 
 __device__ inline void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
 
+// Value of v in lane l (l uniform) via readlane.
+__device__ inline double bcast(double v, int l) {
+  long long b = __double_as_longlong(v);
+  int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+  int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ inline float bcast(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
 template <typename T>
 __device__ inline T gauss_entry(double dx, double dy, double dz, double inv_two_s2, double norm);
 template <>
@@ -85,10 +97,11 @@ __device__ inline float gauss_entry<float>(double dx, double dy, double dz, doub
 }
 
 // Workspace of one block, in elements of T:
-//   lower tiles (nt = KT(KT+1)/2, 256 each) | L_II^{-1} (KT x 256, row-major) | X (NP x TP)
+//   lower tiles (nt = KT(KT+1)/2, 256 each) | L_II^{-1} (KT x 256, row-major)
 __host__ __device__ inline long long blk_ws(int n, int T) {
-  int KT = (n + 15) / 16, NP = 16 * KT, TP = 16 * ((T + 15) / 16);
-  return (long long)(KT * (KT + 1) / 2 + KT) * kTE + (long long)NP * TP;
+  int KT = (n + 15) / 16;
+  (void)T;
+  return (long long)(KT * (KT + 1) / 2 + KT) * kTE;
 }
 
 template <typename T>
@@ -97,7 +110,7 @@ __host__ __device__ inline size_t blk_lds_bytes(int nmax, int tmax, int nw, int 
   (void)tmax;
   size_t pan = (size_t)npc * KT * kLT * sizeof(T);
   size_t crd = (size_t)3 * NP * sizeof(double);
-  size_t red = (size_t)nw * kTE * sizeof(T);
+  size_t red = (size_t)NP * kLS * sizeof(T);  // phase-2 right-hand sides
   size_t region = pan > crd ? pan : crd;
   region = region > red ? region : red;
   region = (region + 15) & ~(size_t)15;
@@ -118,7 +131,7 @@ __global__ __launch_bounds__(NW * 64) void k_block_chol(
   const int n = (int)(bptr[c + 1] - b0);
   if (n == 0) return;
   const int T = iend[c] - ista[c] + 1;
-  const int KT = (n + 15) / 16, NP = 16 * KT, TP = 16 * ((T + 15) / 16), TC = TP / 16;
+  const int KT = (n + 15) / 16, NP = 16 * KT, TC = (T + 15) / 16;
   const int nt = KT * (KT + 1) / 2;
   const int ncore0 = n - T;
   const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
@@ -126,11 +139,10 @@ __global__ __launch_bounds__(NW * 64) void k_block_chol(
 
   Real* Wb = Wbase + wptr[blockIdx.x];
   Real* Wd = Wb + (long long)nt * kTE;       // L_II^{-1}, row-major
-  Real* Xg = Wd + (long long)KT * kTE;       // NP x TP
 
   // LDS carve-up
   size_t pan = (size_t)NPC * KT * kLT * sizeof(Real), crd = (size_t)3 * NP * sizeof(double),
-         red = (size_t)NW * kTE * sizeof(Real);
+         red = (size_t)NP * kLS * sizeof(Real);
   size_t region = pan > crd ? pan : crd;
   region = region > red ? region : red;
   region = (region + 15) & ~(size_t)15;
@@ -138,7 +150,6 @@ __global__ __launch_bounds__(NW * 64) void k_block_chol(
   double* cx = (double*)lds_raw;          // phase 0 coordinates (same region)
   double* cy = cx + NP;
   double* cz = cy + NP;
-  Real* R = (Real*)lds_raw;                     // phase 2 cross-wave reduction (same region)
   Real* S = (Real*)(lds_raw + region);          // per-wave scratch tile
   Real* D = S + NW * kLT;                    // diagonal tile L_kk
   Real* Li = D + kLT;                        // L_kk^{-1}
@@ -202,38 +213,45 @@ __global__ __launch_bounds__(NW * 64) void k_block_chol(
 #pragma unroll
         for (int r = 0; r < 4; ++r) D[MF<Real>::row(r, lq) * kLS + lr] = tkk[r * 64 + lane];
         wave_fence();
+        // Cholesky of the tile with row i in the registers of lane i; the
+        // pivot and column values are broadcast with readlane.
+        Real rw[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) rw[q] = lane < 16 ? D[lane * kLS + q] : Real(0);
+        bool badany = false;
+#pragma unroll
         for (int p = 0; p < 16; ++p) {
-          Real d = D[p * kLS + p];
-          bool bad = !(d > Real(0));
-          Real s = bad ? Real(1) : sqrt(d);
-          Real rs = Real(1) / s;
-          wave_fence();
-          if (lane == 0 && bad) *flag = 1;
-          if (lane > p && lane < 16) D[lane * kLS + p] *= rs;
-          if (lane == p) D[p * kLS + p] = s;
-          wave_fence();
-          if (lane > p && lane < 16) {
-            Real lp = D[lane * kLS + p];
-            for (int q = p + 1; q <= lane; ++q) D[lane * kLS + q] -= lp * D[q * kLS + p];
+          const Real piv = bcast(rw[p], p);
+          const bool bad = !(piv > Real(0));
+          badany |= bad;
+          const Real sp = bad ? Real(1) : sqrt(piv);
+          const Real rs = Real(1) / sp;
+          if (lane == p) rw[p] = sp;
+          if (lane > p) rw[p] *= rs;
+#pragma unroll
+          for (int j = p + 1; j < 16; ++j) {
+            const Real ljp = bcast(rw[p], j);
+            if (lane >= j) rw[j] -= rw[p] * ljp;
           }
-          wave_fence();
         }
-        if (lane < 16)
-          for (int q = lane + 1; q < 16; ++q) D[lane * kLS + q] = Real(0);
-        wave_fence();
+        if (lane == 0 && badany) *flag = 1;
         if (lane < 16) {
-          const int jcol = lane;
-          for (int i = 0; i < 16; ++i) {
-            Real x;
-            if (i < jcol) {
-              x = Real(0);
-            } else {
-              x = (i == jcol) ? Real(1) : Real(0);
-              for (int q = jcol; q < i; ++q) x -= D[i * kLS + q] * Li[q * kLS + jcol];
-              x /= D[i * kLS + i];
-            }
-            Li[i * kLS + jcol] = x;
-          }
+#pragma unroll
+          for (int q = 0; q < 16; ++q) D[lane * kLS + q] = q <= lane ? rw[q] : Real(0);
+        }
+        wave_fence();
+        // column `lane` of L^{-1} by forward substitution; L read as LDS broadcasts
+        Real xi[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          Real acc2 = (i == lane) ? Real(1) : Real(0);
+#pragma unroll
+          for (int q = 0; q < i; ++q) acc2 -= D[i * kLS + q] * xi[q];
+          xi[i] = (i < lane) ? Real(0) : acc2 / D[i * kLS + i];
+        }
+        if (lane < 16) {
+#pragma unroll
+          for (int i = 0; i < 16; ++i) Li[i * kLS + lane] = xi[i];
         }
         wave_fence();
 #pragma unroll
@@ -318,8 +336,11 @@ __global__ __launch_bounds__(NW * 64) void k_block_chol(
     return;
   }
 
-  // ---- phase 2: X = L^{-T} L^{-1} E_core (NP x TP, in place in Xg)
+  // ---- phase 2: X = L^{-T} L^{-1} E_core, 16 right-hand sides at a time,
+  // X in LDS (row stride 17) in the region the panel used
+  Real* X = (Real*)lds_raw;
   const int I0 = ncore0 / 16;
+  Real* Mc = M + mptr[c];
   for (int tc = 0; tc < TC; ++tc) {
     // forward: Y_I = L_II^{-1} (E_I - sum_{I0<=J<I} L_IJ Y_J), tile rows I >= I0
     if (wave == 0) {
@@ -329,7 +350,7 @@ __global__ __launch_bounds__(NW * 64) void k_block_chol(
           const Real* tij = Wb + (long long)ltile(I, J) * kTE;
 #pragma unroll
           for (int q = 0; q < 4; ++q)
-            acc = MF<Real>::mma(tij[MF<Real>::off(lr, 4 * q + lq)], Xg[(long long)(16 * J + 4 * q + lq) * TP + 16 * tc + lr], acc);
+            acc = MF<Real>::mma(tij[MF<Real>::off(lr, 4 * q + lq)], X[(16 * J + 4 * q + lq) * kLS + lr], acc);
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -343,49 +364,54 @@ __global__ __launch_bounds__(NW * 64) void k_block_chol(
 #pragma unroll
         for (int q = 0; q < 4; ++q) y = MF<Real>::mma(li[lr * 16 + 4 * q + lq], Sw[(4 * q + lq) * kLS + lr], y);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) Xg[(long long)(16 * I + MF<Real>::row(r, lq)) * TP + 16 * tc + lr] = y[r];
+        for (int r = 0; r < 4; ++r) X[(16 * I + MF<Real>::row(r, lq)) * kLS + lr] = y[r];
         wave_fence();
       }
     }
     __syncthreads();
-    // backward: X_I = L_II^{-T} (Y_I - sum_{J>I} L_JI^T X_J)
+    // backward: X_I = L_II^{-T} (Y_I - sum_{J>I} L_JI^T X_J); the sum is split
+    // over the wavefronts and combined by wavefront 0 through the scratch tiles
     for (int I = KT - 1; I >= 0; --I) {
       v4 acc = {0, 0, 0, 0};
       for (int J = I + 1 + wave; J < KT; J += NW) {
         const Real* tji = Wb + (long long)ltile(J, I) * kTE;
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-          acc = MF<Real>::mma(tji[MF<Real>::off(4 * q + lq, lr)], Xg[(long long)(16 * J + 4 * q + lq) * TP + 16 * tc + lr], acc);
+          acc = MF<Real>::mma(tji[MF<Real>::off(4 * q + lq, lr)], X[(16 * J + 4 * q + lq) * kLS + lr], acc);
       }
 #pragma unroll
-      for (int r = 0; r < 4; ++r) R[wave * kTE + r * 64 + lane] = acc[r];
+      for (int r = 0; r < 4; ++r) Sw[r * 64 + lane] = acc[r];
       __syncthreads();
       if (wave == 0) {
+        Real rv[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          Real s = R[r * 64 + lane];
-          for (int w = 1; w < NW; ++w) s += R[w * kTE + r * 64 + lane];
-          Real yv = (I >= I0) ? Xg[(long long)(16 * I + MF<Real>::row(r, lq)) * TP + 16 * tc + lr] : Real(0);
-          Sw[MF<Real>::row(r, lq) * kLS + lr] = yv - s;
+          Real sum = S[r * 64 + lane];
+          for (int w = 1; w < NW; ++w) sum += S[w * kLT + r * 64 + lane];
+          Real yv = (I >= I0) ? X[(16 * I + MF<Real>::row(r, lq)) * kLS + lr] : Real(0);
+          rv[r] = yv - sum;
         }
+        wave_fence();
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Sw[MF<Real>::row(r, lq) * kLS + lr] = rv[r];
         wave_fence();
         const Real* li = Wd + (long long)I * kTE;
         v4 x = {0, 0, 0, 0};
 #pragma unroll
         for (int q = 0; q < 4; ++q) x = MF<Real>::mma(li[(4 * q + lq) * 16 + lr], Sw[(4 * q + lq) * kLS + lr], x);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) Xg[(long long)(16 * I + MF<Real>::row(r, lq)) * TP + 16 * tc + lr] = x[r];
+        for (int r = 0; r < 4; ++r) X[(16 * I + MF<Real>::row(r, lq)) * kLS + lr] = x[r];
       }
       __syncthreads();
     }
-  }
-
-  // ---- phase 3: M_c[t][b] = X[pos(b)][t]
-  Real* Mc = M + mptr[c];
-  for (int e = tid; e < n * T; e += NW * 64) {
-    int t = e / n, b = e % n;
-    int p = b < T ? ncore0 + b : b - T;
-    Mc[e] = Xg[(long long)p * TP + t];
+    // ---- phase 3: M_c[t][b] = X[pos(b)][t - 16 tc] for the core members of this chunk
+    const int t0 = 16 * tc, tcn = min(16, T - t0);
+    for (int e = tid; e < n * tcn; e += NW * 64) {
+      int t = e / n, b = e % n;
+      int p = b < T ? ncore0 + b : b - T;
+      Mc[(long long)(t0 + t) * n + b] = X[p * kLS + t];
+    }
+    __syncthreads();
   }
   if (blk_status && tid == 0) blk_status[c] = 0;
 }
