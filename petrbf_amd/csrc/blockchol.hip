@@ -88,7 +88,7 @@ template <typename T>
 __device__ inline T gauss_entry(double dx, double dy, double dz, double inv_two_s2, double norm);
 template <>
 __device__ inline double gauss_entry<double>(double dx, double dy, double dz, double inv, double norm) {
-  return exp(-(dx * dx + dy * dy + dz * dz) * inv) * norm;
+  return exp_nonpos(-(dx * dx + dy * dy + dz * dz) * inv) * norm;
 }
 template <>
 __device__ inline float gauss_entry<float>(double dx, double dy, double dz, double inv, double norm) {
@@ -264,26 +264,35 @@ __global__ __launch_bounds__(NW * 64) void k_block_chol(
         }
       }
       __syncthreads();
-      // panel column: L_ik = A_ik L_kk^{-T}
-      for (int i = kk + 1 + wave; i < KT; i += NW) {
-        Real* tik = Wb + (long long)ltile(i, kk) * kTE;
-        Real a[4];
+      // panel column: L_ik = A_ik L_kk^{-T}; the A operand is read straight
+      // from the stored tile, all of a wavefront's tiles issued before use
+      for (int i0 = kk + 1 + wave; i0 < KT; i0 += NW * 4) {
+        Real av[4][4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) a[r] = tik[r * 64 + lane];
+        for (int g = 0; g < 4; ++g) {
+          const int i = i0 + g * NW;
+          if (i < KT) {
+            const Real* tik = Wb + (long long)ltile(i, kk) * kTE;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) Sw[MF<Real>::row(r, lq) * kLS + lr] = a[r];
-        wave_fence();
-        v4 acc = {0, 0, 0, 0};
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-          acc = MF<Real>::mma(Sw[lr * kLS + 4 * q + lq], Li[lr * kLS + 4 * q + lq], acc);
-        Real* pi = P + (cc * KT + i) * kLT;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          tik[r * 64 + lane] = acc[r];
-          pi[MF<Real>::row(r, lq) * kLS + lr] = acc[r];
+            for (int q = 0; q < 4; ++q) av[g][q] = tik[MF<Real>::off(lr, 4 * q + lq)];
+          }
         }
-        wave_fence();
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          const int i = i0 + g * NW;
+          if (i < KT) {
+            v4 acc = {0, 0, 0, 0};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc = MF<Real>::mma(av[g][q], Li[lr * kLS + 4 * q + lq], acc);
+            Real* tik = Wb + (long long)ltile(i, kk) * kTE;
+            Real* pi = P + (cc * KT + i) * kLT;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              tik[r * 64 + lane] = acc[r];
+              pi[MF<Real>::row(r, lq) * kLS + lr] = acc[r];
+            }
+          }
+        }
       }
       __syncthreads();
     }
@@ -292,7 +301,7 @@ __global__ __launch_bounds__(NW * 64) void k_block_chol(
     const int j0 = k0 + npc;
     const int m = KT - j0;
     const int ntr = m * (m + 1) / 2;
-    constexpr int kG = 4;
+    constexpr int kG = 8;
     for (int u0 = wave; u0 < ntr; u0 += NW * kG) {
       v4 acc[kG];
       Real* tp[kG];
@@ -373,11 +382,26 @@ __global__ __launch_bounds__(NW * 64) void k_block_chol(
     // over the wavefronts and combined by wavefront 0 through the scratch tiles
     for (int I = KT - 1; I >= 0; --I) {
       v4 acc = {0, 0, 0, 0};
-      for (int J = I + 1 + wave; J < KT; J += NW) {
-        const Real* tji = Wb + (long long)ltile(J, I) * kTE;
+      for (int J0 = I + 1 + wave; J0 < KT; J0 += NW * 4) {
+        Real av[4][4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-          acc = MF<Real>::mma(tji[MF<Real>::off(4 * q + lq, lr)], X[(16 * J + 4 * q + lq) * kLS + lr], acc);
+        for (int g = 0; g < 4; ++g) {
+          const int J = J0 + g * NW;
+          if (J < KT) {
+            const Real* tji = Wb + (long long)ltile(J, I) * kTE;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) av[g][q] = tji[MF<Real>::off(4 * q + lq, lr)];
+          }
+        }
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          const int J = J0 + g * NW;
+          if (J < KT) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+              acc = MF<Real>::mma(av[g][q], X[(16 * J + 4 * q + lq) * kLS + lr], acc);
+          }
+        }
       }
 #pragma unroll
       for (int r = 0; r < 4; ++r) Sw[r * 64 + lane] = acc[r];

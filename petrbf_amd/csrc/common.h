@@ -70,6 +70,32 @@ inline Kern make_kern(double sigma) {
   return k;
 }
 
+// exp(x) for -745 < x <= 0 in FP64: Cody-Waite reduction x = k ln2 + r with a
+// two-part ln2 (the high part has 33 significant bits, so k*ln2_hi is exact
+// for |k| < 2^20), |r| <= ln2/2, then the degree-13 Taylor polynomial of e^r
+// (truncation error < 1e-17 relative) and a scaling by 2^k.  No range checks:
+// every caller passes -r^2/(2 sigma^2) of points inside a cluster's boxes.
+__device__ inline double exp_nonpos(double x) {
+  const double kf = rint(x * 1.4426950408889634);
+  double r = fma(-kf, 0.6931471804855391, x);
+  r = fma(-kf, 7.440617110012397e-11, r);
+  double p = 1.6059043836821613e-10;
+  p = fma(p, r, 2.08767569878681e-09);
+  p = fma(p, r, 2.505210838544172e-08);
+  p = fma(p, r, 2.755731922398589e-07);
+  p = fma(p, r, 2.7557319223985893e-06);
+  p = fma(p, r, 2.48015873015873e-05);
+  p = fma(p, r, 0.0001984126984126984);
+  p = fma(p, r, 0.001388888888888889);
+  p = fma(p, r, 0.008333333333333333);
+  p = fma(p, r, 0.041666666666666664);
+  p = fma(p, r, 0.16666666666666666);
+  p = fma(p, r, 0.5);
+  p = fma(p, r, 1.0);
+  p = fma(p, r, 1.0);
+  return ldexp(p, (int)kf);
+}
+
 }  // namespace prbf
 
 #define PRBF_HIP(call)                                                         \

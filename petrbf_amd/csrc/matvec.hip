@@ -5,10 +5,10 @@
 // as 2d/matmult.cxx:28-43 (MatShell MULT) and 2d/get_vorticity.h:11-26
 // (evaluation at a distinct target set).  One wavefront per cluster: the
 // cluster's truncation list (the same for all of its targets,
-// 2d/get_trunc.h:63) is staged in LDS as SoA FP64 tiles; each lane owns one
-// target and a residue class of the source tile, accumulates in FP64 with
+// 2d/get_trunc.h:63) is staged in LDS as SoA FP64 tiles; each lane owns two
+// targets and a residue class of the source tile, accumulates in FP64 with
 // FMA, and the per-target partial sums are combined in a fixed order, so the
-// result is deterministic.  FP64 VALU-bound: one exp per pair.
+// result is deterministic.  FP64 VALU-bound: one exp (exp_nonpos) per pair.
 #include "common.h"
 #include "kernels.h"
 
@@ -16,6 +16,9 @@ namespace prbf {
 
 constexpr int kTile = 256;  // sources staged per LDS tile
 
+// Lanes own two targets each (t and t + TP) and a residue class q of the
+// staged sources, so every source read from LDS feeds two pair evaluations;
+// the source loop is unrolled by two for independent exps.
 template <int DIM, int NRHS>
 __global__ __launch_bounds__(64) void k_matvec_wave(
     long long ncell, const int* __restrict__ tsta, const int* __restrict__ tend,
@@ -26,28 +29,35 @@ __global__ __launch_bounds__(64) void k_matvec_wave(
     double inv_two_s2, double norm) {
   __shared__ double sx[kTile], sy[kTile], sz[DIM == 3 ? kTile : 1];
   __shared__ double sg[NRHS][kTile];
-  __shared__ double part[NRHS][64];
+  __shared__ double part[NRHS][128];
   const long long c = blockIdx.x;
   const int lane = threadIdx.x;
   const int t0 = tsta[c], t1 = tend[c];
   if (t0 > t1) return;
   const long long s0 = tptr[c], s1 = tptr[c + 1];
   const int T = t1 - t0 + 1;
-  for (int tb = 0; tb < T; tb += 64) {
-    const int Tc = min(64, T - tb);
-    const int Q = 64 / Tc;             // source phases per target
-    const int t = lane % Tc, q = lane / Tc;
+  const double nc = -inv_two_s2;
+  for (int tb = 0; tb < T; tb += 128) {
+    const int Tc = min(128, T - tb);
+    const int TP = (Tc + 1) / 2;       // target pairs
+    const int Q = 64 / TP;             // source phases
+    const int pr = lane % TP, q = lane / TP;
     const bool active = q < Q;
-    const int it = t0 + tb + t;
-    double px = 0, py = 0, pz = 0;
+    const int ta = tb + pr, tb2 = tb + pr + TP;
+    const bool hasb = tb2 < tb + Tc;
+    double ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
     if (active) {
-      px = xi[it];
-      py = yi[it];
-      if (DIM == 3) pz = zi[it];
+      ax = xi[t0 + ta];
+      ay = yi[t0 + ta];
+      if (DIM == 3) az = zi[t0 + ta];
+      const int tbb = hasb ? tb2 : ta;
+      bx = xi[t0 + tbb];
+      by = yi[t0 + tbb];
+      if (DIM == 3) bz = zi[t0 + tbb];
     }
-    double acc[NRHS];
+    double accA[NRHS], accB[NRHS];
 #pragma unroll
-    for (int r = 0; r < NRHS; ++r) acc[r] = 0.0;
+    for (int r = 0; r < NRHS; ++r) accA[r] = accB[r] = 0.0;
     for (long long sb = s0; sb < s1; sb += kTile) {
       const int n = (int)min<long long>(kTile, s1 - sb);
       __syncthreads();
@@ -61,32 +71,71 @@ __global__ __launch_bounds__(64) void k_matvec_wave(
       }
       __syncthreads();
       if (active) {
-        for (int k = q; k < n; k += Q) {
-          double dx = px - sx[k];
-          double dy = py - sy[k];
-          double r2 = dx * dx + dy * dy;
+        int k = q;
+        for (; k + Q < n; k += 2 * Q) {
+          const int k2 = k + Q;
+          const double x1 = sx[k], y1 = sy[k], x2 = sx[k2], y2 = sy[k2];
+          double dxa1 = ax - x1, dya1 = ay - y1, dxb1 = bx - x1, dyb1 = by - y1;
+          double dxa2 = ax - x2, dya2 = ay - y2, dxb2 = bx - x2, dyb2 = by - y2;
+          double ra1 = fma(dxa1, dxa1, dya1 * dya1), rb1 = fma(dxb1, dxb1, dyb1 * dyb1);
+          double ra2 = fma(dxa2, dxa2, dya2 * dya2), rb2 = fma(dxb2, dxb2, dyb2 * dyb2);
           if (DIM == 3) {
-            double dz = pz - sz[k];
-            r2 += dz * dz;
+            const double z1 = sz[k], z2 = sz[k2];
+            ra1 = fma(az - z1, az - z1, ra1);
+            rb1 = fma(bz - z1, bz - z1, rb1);
+            ra2 = fma(az - z2, az - z2, ra2);
+            rb2 = fma(bz - z2, bz - z2, rb2);
           }
-          double e = exp(-r2 * inv_two_s2);
+          const double ea1 = exp_nonpos(ra1 * nc), eb1 = exp_nonpos(rb1 * nc);
+          const double ea2 = exp_nonpos(ra2 * nc), eb2 = exp_nonpos(rb2 * nc);
 #pragma unroll
-          for (int r = 0; r < NRHS; ++r) acc[r] = fma(sg[r][k], e, acc[r]);
+          for (int r = 0; r < NRHS; ++r) {
+            const double g1 = sg[r][k], g2 = sg[r][k2];
+            accA[r] = fma(g1, ea1, accA[r]);
+            accB[r] = fma(g1, eb1, accB[r]);
+            accA[r] = fma(g2, ea2, accA[r]);
+            accB[r] = fma(g2, eb2, accB[r]);
+          }
+        }
+        if (k < n) {
+          const double x1 = sx[k], y1 = sy[k];
+          double ra1 = fma(ax - x1, ax - x1, (ay - y1) * (ay - y1));
+          double rb1 = fma(bx - x1, bx - x1, (by - y1) * (by - y1));
+          if (DIM == 3) {
+            const double z1 = sz[k];
+            ra1 = fma(az - z1, az - z1, ra1);
+            rb1 = fma(bz - z1, bz - z1, rb1);
+          }
+          const double ea1 = exp_nonpos(ra1 * nc), eb1 = exp_nonpos(rb1 * nc);
+#pragma unroll
+          for (int r = 0; r < NRHS; ++r) {
+            accA[r] = fma(sg[r][k], ea1, accA[r]);
+            accB[r] = fma(sg[r][k], eb1, accB[r]);
+          }
         }
       }
     }
     __syncthreads();
+    // partials: part[r][qq*2TP + p] for target p (< 2TP); combined in phase order
 #pragma unroll
-    for (int r = 0; r < NRHS; ++r) part[r][lane] = active ? acc[r] : 0.0;
+    for (int r = 0; r < NRHS; ++r) {
+      if (active) {
+        part[r][q * 2 * TP + pr] = accA[r];
+        part[r][q * 2 * TP + TP + pr] = accB[r];
+      }
+    }
     __syncthreads();
     if (lane < Tc) {
+      // target `lane` sits in slot `lane` (pair lane for lane < TP, else the B half)
+      const int slot = lane;
 #pragma unroll
       for (int r = 0; r < NRHS; ++r) {
         double s = 0.0;
-        for (int qq = 0; qq < Q; ++qq) s += part[r][qq * Tc + lane];
+        for (int qq = 0; qq < Q; ++qq) s += part[r][qq * 2 * TP + slot];
         w[r * ldw + t0 + tb + lane] = s * norm;
       }
     }
+    __syncthreads();
   }
 }
 
