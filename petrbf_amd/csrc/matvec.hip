@@ -125,14 +125,13 @@ __global__ __launch_bounds__(64) void k_matvec_wave(
       }
     }
     __syncthreads();
-    if (lane < Tc) {
-      // target `lane` sits in slot `lane` (pair lane for lane < TP, else the B half)
-      const int slot = lane;
+    // target tt sits in slot tt (pair tt for tt < TP, else the B half of pair tt - TP)
+    for (int tt = lane; tt < Tc; tt += 64) {
 #pragma unroll
       for (int r = 0; r < NRHS; ++r) {
         double s = 0.0;
-        for (int qq = 0; qq < Q; ++qq) s += part[r][qq * 2 * TP + slot];
-        w[r * ldw + t0 + tb + lane] = s * norm;
+        for (int qq = 0; qq < Q; ++qq) s += part[r][qq * 2 * TP + tt];
+        w[r * ldw + t0 + tb + tt] = s * norm;
       }
     }
     __syncthreads();

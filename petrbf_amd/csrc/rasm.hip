@@ -25,6 +25,23 @@ namespace prbf {
 constexpr int kApplyWaves = 4;  // clusters per workgroup, one per wavefront
 constexpr int kKB = 8;          // buffer slots per lane per pass (n <= 512 in one pass)
 
+// acc[t] += sum over this lane's slots of M[t0+t][b] * r[b] for one pass
+template <int TMAX, typename MT>
+__device__ inline void rows_times(const MT* __restrict__ Mc, int n, int t0, int tc, int bb, int lane,
+                                  const double (&rb)[kKB], double (&acc)[TMAX]) {
+#pragma unroll
+  for (int t = 0; t < TMAX; ++t) {
+    if (t < tc) {
+      const long long row = (long long)(t0 + t) * n;
+#pragma unroll
+      for (int k = 0; k < kKB; ++k) {
+        const int b = bb + lane + 64 * k;
+        if (b < n) acc[t] = fma((double)Mc[row + b], rb[k], acc[t]);
+      }
+    }
+  }
+}
+
 template <int TMAX, typename MT>
 __global__ __launch_bounds__(64 * kApplyWaves) void k_rasm_apply(
     long long ncell, const long long* __restrict__ bptr, const int* __restrict__ bidx,
@@ -59,20 +76,10 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_rasm_apply(
       }
 #pragma unroll
       for (int k = 0; k < kKB; ++k) rb[k] = idx[k] >= 0 ? r[idx[k]] : 0.0;
-#pragma unroll
-      for (int t = 0; t < TMAX; ++t) {
-        if (t < tc) {
-          const long long row = (long long)(t0 + t) * n;
-#pragma unroll
-          for (int k = 0; k < kKB; ++k) {
-            const int b = bb + lane + 64 * k;
-            if (b < n) {
-              const double m = Md ? Md[row + b] : (double)Mc[row + b];
-              acc[t] = fma(m, rb[k], acc[t]);
-            }
-          }
-        }
-      }
+      if (Md)
+        rows_times<TMAX, double>(Md, n, t0, tc, bb, lane, rb, acc);
+      else
+        rows_times<TMAX, MT>(Mc, n, t0, tc, bb, lane, rb, acc);
     }
 #pragma unroll
     for (int t = 0; t < TMAX; ++t) rw[t][lane] = acc[t];
