@@ -205,57 +205,62 @@ __global__ __launch_bounds__(NW * 64) void k_block_chol(
           }
 #pragma unroll
           for (int r = 0; r < 4; ++r) tik[r * 64 + lane] = acc[r];
+          if (i == kk) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) D[MF<Real>::row(r, lq) * kLS + lr] = acc[r];
+          }
         }
         __syncthreads();
       }
       if (wave == 0) {
-        Real* tkk = Wb + (long long)ltile(kk, kk) * kTE;
+        if (kk == 0) {  // later diagonal tiles are left in D by the update that finished them
+          const Real* tkk = Wb;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) D[MF<Real>::row(r, lq) * kLS + lr] = tkk[r * 64 + lane];
+          for (int r = 0; r < 4; ++r) D[MF<Real>::row(r, lq) * kLS + lr] = tkk[r * 64 + lane];
+        }
         wave_fence();
-        // Cholesky of the tile with row i in the registers of lane i; the
-        // pivot and column values are broadcast with readlane.
-        Real rw[16];
+        // L_kk^{-1} directly: symmetric elimination on [A_kk | I] with row i in
+        // the registers of lane i; step p scales row p by 1/sqrt(pivot) and
+        // removes column p from the rows below.  The left half becomes L^T,
+        // the right half L^{-1}; only the latter is kept.  Broadcasts of row
+        // p are readlanes.
+        Real ra[16], re[16];
 #pragma unroll
-        for (int q = 0; q < 16; ++q) rw[q] = lane < 16 ? D[lane * kLS + q] : Real(0);
+        for (int q = 0; q < 16; ++q) {
+          ra[q] = lane < 16 ? D[lane * kLS + q] : Real(0);
+          re[q] = (q == lane) ? Real(1) : Real(0);
+        }
         bool badany = false;
 #pragma unroll
         for (int p = 0; p < 16; ++p) {
-          const Real piv = bcast(rw[p], p);
+          const Real piv = bcast(ra[p], p);
           const bool bad = !(piv > Real(0));
           badany |= bad;
-          const Real sp = bad ? Real(1) : sqrt(piv);
-          const Real rs = Real(1) / sp;
-          if (lane == p) rw[p] = sp;
-          if (lane > p) rw[p] *= rs;
+          const Real rinv = bad ? Real(1) : rsqrt(piv);
+          const Real m = ra[p] * rinv;
+          if (lane == p) {
 #pragma unroll
-          for (int j = p + 1; j < 16; ++j) {
-            const Real ljp = bcast(rw[p], j);
-            if (lane >= j) rw[j] -= rw[p] * ljp;
+            for (int q = p; q < 16; ++q) ra[q] *= rinv;
+#pragma unroll
+            for (int q = 0; q <= p; ++q) re[q] *= rinv;
+          }
+#pragma unroll
+          for (int q = p + 1; q < 16; ++q) {
+            const Real v = bcast(ra[q], p);
+            if (lane > p) ra[q] -= m * v;
+          }
+#pragma unroll
+          for (int q = 0; q <= p; ++q) {
+            const Real v = bcast(re[q], p);
+            if (lane > p) re[q] -= m * v;
           }
         }
         if (lane == 0 && badany) *flag = 1;
         if (lane < 16) {
 #pragma unroll
-          for (int q = 0; q < 16; ++q) D[lane * kLS + q] = q <= lane ? rw[q] : Real(0);
+          for (int q = 0; q < 16; ++q) Li[lane * kLS + q] = q <= lane ? re[q] : Real(0);
         }
         wave_fence();
-        // column `lane` of L^{-1} by forward substitution; L read as LDS broadcasts
-        Real xi[16];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-          Real acc2 = (i == lane) ? Real(1) : Real(0);
-#pragma unroll
-          for (int q = 0; q < i; ++q) acc2 -= D[i * kLS + q] * xi[q];
-          xi[i] = (i < lane) ? Real(0) : acc2 / D[i * kLS + i];
-        }
-        if (lane < 16) {
-#pragma unroll
-          for (int i = 0; i < 16; ++i) Li[i * kLS + lane] = xi[i];
-        }
-        wave_fence();
-#pragma unroll
-        for (int r = 0; r < 4; ++r) tkk[r * 64 + lane] = D[MF<Real>::row(r, lq) * kLS + lr];
         Real* wd = Wd + (long long)kk * kTE;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -332,6 +337,10 @@ __global__ __launch_bounds__(NW * 64) void k_block_chol(
           }
 #pragma unroll
           for (int r = 0; r < 4; ++r) tp[g][r * 64 + lane] = acc[g][r];
+          if (ti[g] == j0 && tj[g] == j0) {  // next diagonal tile: hand it over in D
+#pragma unroll
+            for (int r = 0; r < 4; ++r) D[MF<Real>::row(r, lq) * kLS + lr] = acc[g][r];
+          }
         }
       }
     }
