@@ -169,6 +169,17 @@ prbf_status prbf_solve(prbf_ctx* ctx, const prbf_solve_opts* opts, const double*
 prbf_status prbf_solve_dev(prbf_ctx* ctx, const prbf_solve_opts* opts, const double* w, double* gamma,
                            prbf_solve_info* info, double* hist, int hist_cap);
 
+/* ---- library entry of the 3D tree ---------------------------------------
+ * RBFInterpolation (3d/wrapper.cxx:20-105), for dim 2 or 3: FP32 grid
+ * (xg, yg, zg) with values wg in; if the grid is not the particle set
+ * (|xg[0] - xp[0]| > 1e-6, 3d/wrapper.cxx:93) the field of the particles
+ * (xp, yp, zp, strengths gp) is evaluated at the grid first; the grid
+ * strengths solving A~ g = w are written over wg.  No context: errors are
+ * reported by prbf_last_error(NULL). */
+prbf_status prbf_rbf_interpolation_f32(const prbf_params* p, int64_t ng, const float* xg, const float* yg,
+                                       const float* zg, float* wg, int64_t np, const float* xp, const float* yp,
+                                       const float* zp, const float* gp, const prbf_solve_opts* opts, int* its);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,7 +25,7 @@ EXPORTS = [
     "prbf_get_permutation", "prbf_get_cells", "prbf_get_buffer_csr", "prbf_get_trunc_csr",
     "prbf_get_sorted_points", "prbf_matvec", "prbf_matvec_dev", "prbf_evaluate", "prbf_evaluate_dev",
     "prbf_rasm_setup", "prbf_rasm_setup_pc", "prbf_set_tuning", "prbf_rasm_apply", "prbf_rasm_apply_dev", "prbf_rasm_get_core_rows",
-    "prbf_solve", "prbf_solve_dev",
+    "prbf_solve", "prbf_solve_dev", "prbf_rbf_interpolation_f32",
 ]
 
 
@@ -90,6 +90,8 @@ def load():
     L.prbf_rasm_get_core_rows.argtypes = [ctx, i64, dp, C.POINTER(i32), C.POINTER(i32)]
     for f in ("prbf_solve", "prbf_solve_dev"):
         getattr(L, f).argtypes = [ctx, C.POINTER(PrbfSolveOpts), dp, dp, C.POINTER(PrbfSolveInfo), dp, i32]
+    L.prbf_rbf_interpolation_f32.argtypes = [C.POINTER(PrbfParams), i64, dp, dp, dp, dp, i64, dp, dp, dp, dp,
+                                             C.POINTER(PrbfSolveOpts), C.POINTER(i32)]
     for f in EXPORTS:
         if f not in ("prbf_solve_opts_default", "prbf_last_error"):
             getattr(L, f).restype = C.c_int

@@ -729,7 +729,9 @@ prbf_status prbf_destroy(prbf_ctx* c) {
   return PRBF_OK;
 }
 
-const char* prbf_last_error(const prbf_ctx* c) { return c ? c->err.c_str() : "null context"; }
+static thread_local std::string g_free_err;
+
+const char* prbf_last_error(const prbf_ctx* c) { return c ? c->err.c_str() : g_free_err.c_str(); }
 
 prbf_status prbf_set_stream(prbf_ctx* c, void* stream) {
   return guarded(c, [&] {
@@ -967,6 +969,52 @@ prbf_status prbf_solve(prbf_ctx* c, const prbf_solve_opts* opts, const double* w
     PRBF_HIP(hipMemcpyAsync(gamma, dg, sizeof(double) * c->ni, hipMemcpyDeviceToHost, c->st));
     sync(c);
   });
+}
+
+// RBFInterpolation (3d/wrapper.cxx:20-105): FP32 grid values in, grid
+// strengths out.  When the grid is not the particle set (first coordinates
+// differ by more than 1e-6, 3d/wrapper.cxx:93) the field of the particles
+// (strengths gp) is first evaluated at the grid; then the grid strengths
+// solving A~ g = w are written back over wg.
+prbf_status prbf_rbf_interpolation_f32(const prbf_params* p, int64_t ng, const float* xg, const float* yg,
+                                       const float* zg, float* wg, int64_t np_, const float* xp, const float* yp,
+                                       const float* zp, const float* gp, const prbf_solve_opts* opts, int* its) {
+  if (!p || ng <= 0 || !xg || !yg || !wg || (p->dim == 3 && !zg)) {
+    g_free_err = "invalid arguments";
+    return PRBF_ERR_ARG;
+  }
+  const int dim = p->dim;
+  auto up = [](const float* a, int64_t n) {
+    std::vector<double> v(a ? n : 0);
+    for (int64_t i = 0; a && i < n; ++i) v[i] = a[i];
+    return v;
+  };
+  std::vector<double> gx = up(xg, ng), gy = up(yg, ng), gz = up(dim == 3 ? zg : nullptr, ng), w = up(wg, ng);
+  prbf_ctx* c = nullptr;
+  prbf_status st = prbf_create(&c, p);
+  if (st) {
+    g_free_err = "prbf_create failed";
+    return st;
+  }
+  const bool remesh = np_ > 0 && xp && std::fabs((double)xg[0] - (double)xp[0]) > kEpsf;
+  if (remesh) {
+    std::vector<double> px = up(xp, np_), py = up(yp, np_), pz = up(dim == 3 ? zp : nullptr, np_), pg = up(gp, np_);
+    st = prbf_set_points(c, ng, gx.data(), gy.data(), dim == 3 ? gz.data() : nullptr, np_, px.data(), py.data(),
+                         dim == 3 ? pz.data() : nullptr);
+    if (!st) st = prbf_evaluate(c, 1, pg.data(), w.data());
+  }
+  std::vector<double> g(ng);
+  prbf_solve_info info{};
+  if (!st) st = prbf_set_points(c, ng, gx.data(), gy.data(), dim == 3 ? gz.data() : nullptr, 0, nullptr, nullptr, nullptr);
+  if (!st) st = prbf_solve(c, opts, w.data(), g.data(), &info, nullptr, 0);
+  if (st) {
+    g_free_err = c->err;
+  } else {
+    for (int64_t i = 0; i < ng; ++i) wg[i] = (float)g[i];
+    if (its) *its = info.its;
+  }
+  prbf_destroy(c);
+  return st;
 }
 
 }  // extern "C"

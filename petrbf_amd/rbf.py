@@ -287,3 +287,27 @@ def vorticity_evaluation(xi, yi, xj, yj, gj, sigma, nsigma_box, sigma_buffer, si
         return ctx.evaluate(gj)
     finally:
         ctx.close()
+
+
+def RBFInterpolation(xg, yg, zg, wg, xp, yp, zp, gp, sigma, nsigma_box, sigma_buffer, sigma_trunc,
+                     ksp="gmres", pc="rasm", rtol=1e-10, restart=30, max_it=1000):
+    """The reference's library entry (3d/wrapper.cxx:20-23): FP32 grid values
+    wg in, grid strengths out (returned as a new float32 array, with the
+    iteration count).  zg/zp None selects 2D."""
+    Lb = L.load()
+    dim = 2 if zg is None else 3
+    f32 = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+    xg, yg, zg, xp, yp, zp, gp = (f32(a) for a in (xg, yg, zg, xp, yp, zp, gp))
+    out = np.ascontiguousarray(wg, dtype=np.float32).copy()
+    prm = L.PrbfParams(dim, float(sigma), float(nsigma_box), float(sigma_buffer), float(sigma_trunc))
+    o = L.PrbfSolveOpts()
+    Lb.prbf_solve_opts_default(C.byref(o))
+    o.ksp, o.pc, o.rtol, o.restart, o.max_it = _KSP[ksp], _PC[pc], float(rtol), int(restart), int(max_it)
+    its = C.c_int(0)
+    npart = 0 if xp is None else xp.shape[0]
+    st = Lb.prbf_rbf_interpolation_f32(C.byref(prm), xg.shape[0], _p(xg), _p(yg), _p(zg), _p(out), npart,
+                                       _p(xp), _p(yp), _p(zp), _p(gp), C.byref(o), C.byref(its))
+    if st != 0:
+        msg = Lb.prbf_last_error(None)
+        raise L.PrbfError(st, msg.decode() if msg else "")
+    return out, its.value
