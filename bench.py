@@ -206,10 +206,10 @@ def main():
                                    if len(results) == 2 else None),
         "pair_evals_per_s": pairs / t_mv,
         "pairs_per_matvec": pairs,
-        "roofline": {"bound": "hbm", "kernel": "k_rasm_apply<16,double>", "achieved": apply_bytes / t_ap / 1e9,
+        "roofline": {"bound": "hbm", "kernel": "k_rasm_apply<double>", "achieved": apply_bytes / t_ap / 1e9,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": apply_bytes / t_ap / 1e9 / PEAK_HBM_GBS,
                      "traffic": None, "avg_ms": t_ap * 1e3, "bytes_per_launch": apply_bytes},
-        "roofline_apply_fp32": {"bound": "hbm", "kernel": "k_rasm_apply<16,float>",
+        "roofline_apply_fp32": {"bound": "hbm", "kernel": "k_rasm_apply<float>",
                                 "achieved": apply32_bytes / t_ap32 / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                                 "frac": apply32_bytes / t_ap32 / 1e9 / PEAK_HBM_GBS, "avg_ms": t_ap32 * 1e3,
                                 "bytes_per_launch": apply32_bytes},

@@ -394,7 +394,7 @@ void rasm_setup_impl(prbf_ctx* c, int pc) {
   for (long long k = 0; k < nc; ++k) {
     long long n = c->h_bptr[k + 1] - c->h_bptr[k];
     long long T = n ? (c->h_iend[k] - c->h_ista[k] + 1) : 0;
-    c->h_mptr[k + 1] = c->h_mptr[k] + n * T;
+    c->h_mptr[k + 1] = c->h_mptr[k] + apply_elems((int)n, (int)T);
     wsz[k] = n ? block_workspace((int)n, (int)T) : 0;
   }
   need(block_lds_bytes(fp32, c->max_buffer, c->max_core, 4, 1) <= kMaxLdsBytes, PRBF_ERR_LIMIT,
@@ -422,7 +422,7 @@ void rasm_setup_impl(prbf_ctx* c, int pc) {
       long long T = c->h_iend[k] - c->h_ista[k] + 1;
       h_fb[k] = tot;
       h_m64[k] = tot;
-      tot += n * T;
+      tot += apply_elems((int)n, (int)T);
       h_w.push_back(wtot);
       wtot += wsz[k];
       nmax = std::max<int>(nmax, (int)n);
@@ -946,8 +946,11 @@ prbf_status prbf_rasm_get_core_rows(prbf_ctx* c, int64_t k, double* M, int32_t* 
     if (n) *n = nn;
     if (T) *T = tt;
     if (M && nn) {
-      d2h(c, M, c->M.p + c->h_mptr[k], sizeof(double) * (size_t)nn * tt);
+      std::vector<double> st((size_t)apply_elems(nn, tt));
+      d2h(c, st.data(), c->M.p + c->h_mptr[k], sizeof(double) * st.size());
       sync(c);
+      for (int t = 0; t < tt; ++t)
+        for (int b = 0; b < nn; ++b) M[(size_t)t * nn + b] = st[apply_offset<2>(nn, t, b)];
     }
   });
 }

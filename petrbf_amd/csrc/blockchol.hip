@@ -437,12 +437,18 @@ __global__ __launch_bounds__(NW * 64) void k_block_chol(
       }
       __syncthreads();
     }
-    // ---- phase 3: M_c[t][b] = X[pos(b)][t - 16 tc] for the core members of this chunk
-    const int t0 = 16 * tc, tcn = min(16, T - t0);
-    for (int e = tid; e < n * tcn; e += NW * 64) {
-      int t = e / n, b = e % n;
-      int p = b < T ? ncore0 + b : b - T;
-      Mc[(long long)(t0 + t) * n + b] = X[p * kLS + t];
+    // ---- phase 3: M_c[16 tc + tt][b] = X[pos(b)][tt], rows past T zero, in
+    // the apply's storage order (apply_offset), written contiguously
+    const int tcn = min(16, T - 16 * tc);
+    constexpr int V = 16 / (int)sizeof(Real);
+    Real* Mk = Mc + (long long)tc * apply_chunk_elems(n);
+    for (int e = tid; e < 16 * n; e += NW * 64) {
+      const int g = e / 1024, rr = e % 1024;
+      const int ng = min(64, n - 64 * g);
+      const int q = rr / (ng * V), rem = rr % (ng * V);
+      const int b = 64 * g + rem / V, tt = q * V + rem % V;
+      const int p = b < T ? ncore0 + b : b - T;
+      Mk[e] = tt < tcn ? X[p * kLS + tt] : Real(0);
     }
     __syncthreads();
   }

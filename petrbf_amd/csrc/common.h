@@ -51,6 +51,18 @@ __host__ __device__ inline double cell_centre(double xmin, int k, double box) {
   return xmin + off;
 }
 
+// The dispatcher deals workgroups round-robin over the 8 XCDs (workgroup b
+// runs on XCD b % 8), each with its own L2.  This maps workgroup b to a
+// logical index so that every XCD receives one contiguous range of logical
+// indices (neighbouring clusters share the buffer and truncation particles
+// they gather); a bijection on [0, n) for any n.
+constexpr unsigned kXcds = 8;
+__device__ inline unsigned xcd_block(unsigned b, unsigned n) {
+  const unsigned q = n / kXcds, r = n % kXcds;
+  const unsigned x = b % kXcds, k = b / kXcds;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
+
 // Gaussian normalisation 1/(2 pi sigma^2) as the reference writes it,
 // exp(-r2/(2 sigma sigma)) / (2 pi sigma sigma) (2d/matmult.cxx:39-40); the
 // 3D tree uses the same 2D constant (3d/matmult.cxx:40-41).

@@ -26,6 +26,25 @@ void matvec(hipStream_t st, int dim, int nrhs, long long ncell, const int* tsta,
             long long ldw, double sigma);
 
 // blockchol.hip / rasm.hip ----------------------------------------------------
+// Storage of the core rows M_c (T x n) for the apply.  The rows come in
+// chunks of 16 (rows T..16*ceil(T/16)-1 are zero); inside a chunk the buffer
+// slots form groups of 64 (the last group holds ng = n - 64 g slots), and a
+// group stores its 16 x ng values as 16/V vector slabs [q][slot][V], V values
+// (16 bytes) per slot and slab.  Lane l of a wavefront thus reads 16 bytes per
+// slab at l*16 bytes and every slab load is one contiguous 1 KB access.
+constexpr int kRowChunk = 16;
+__host__ __device__ inline long long apply_chunk_elems(int n) { return (long long)kRowChunk * n; }
+__host__ __device__ inline long long apply_elems(int n, int T) {
+  return n ? (long long)((T + kRowChunk - 1) / kRowChunk) * apply_chunk_elems(n) : 0;
+}
+// offset of the value of row t (all chunks) and slot b; V = 16 / sizeof(element)
+template <int V>
+__host__ __device__ inline long long apply_offset(int n, int t, int b) {
+  const int kc = t / kRowChunk, tt = t % kRowChunk;
+  const int g = b / 64, l = b % 64, ng = n - 64 * g < 64 ? n - 64 * g : 64;
+  return kc * apply_chunk_elems(n) + 1024LL * g + (long long)((tt / V) * ng + l) * V + tt % V;
+}
+
 // Workspace (elements) and LDS bytes of one block factorisation.
 long long block_workspace(int n, int T);
 size_t block_lds_bytes(bool fp32, int nmax, int tmax, int nw, int npc);

@@ -8,7 +8,8 @@
 // The restrict form only ever reads the core rows of A_c^{-1}.  The setup
 // therefore computes, once, M_c = (A_c^{-1})[core, buffer] (T_c x n_c) and the
 // apply is a batched gather + small GEMV that streams M_c from HBM: T_c n_c
-// doubles per cluster instead of the n_c^2 of a stored factor.
+// values per cluster (T_c rounded up to 16) instead of the n_c^2 of a stored
+// factor.
 //
 // The setup kernel lives in blockchol.hip; this file holds the apply.
 #include "common.h"
@@ -16,81 +17,139 @@
 
 namespace prbf {
 
-// Apply: one wavefront per cluster.  Lanes stride over buffer slots b,
-// gather r[b] once, and accumulate T partial sums against the rows of M_c
-// (row-major T x n, so each load is a contiguous wave access); partials are
-// combined through LDS in a fixed order.  MT = double (plain) or float (mixed;
-// products and sums still in FP64).  In mixed mode a cluster whose FP32 block
-// broke down carries FP64 rows in M64 at offset fb[c] (fb[c] < 0 otherwise).
+// Apply: one wavefront per cluster.  Lane l owns the buffer slots l, l+64,
+// ... (one per slot group), gathers r at those slots once, and accumulates 16
+// row partials against the core rows, which it reads as 16-byte vectors from
+// contiguous 1 KB slabs (the storage order of apply_offset).  The 16 x 64
+// partials are combined by a butterfly over the wavefront (reduce-scatter by
+// halves, then two full exchanges): a fixed order, so the result is identical
+// run to run.  MT = double (plain) or float (mixed; products and sums still
+// in FP64).  In mixed mode a cluster whose FP32 block broke down carries FP64
+// rows in M64 at offset fb[c] (fb[c] < 0 otherwise).  Lanes past the end of
+// the last group read a valid slot of the same group with r taken as 0, so
+// the loads need no branches.
 constexpr int kApplyWaves = 4;  // clusters per workgroup, one per wavefront
-constexpr int kKB = 8;          // buffer slots per lane per pass (n <= 512 in one pass)
 
-// acc[t] += sum over this lane's slots of M[t0+t][b] * r[b] for one pass
-template <int TMAX, typename MT>
-__device__ inline void rows_times(const MT* __restrict__ Mc, int n, int t0, int tc, int bb, int lane,
-                                  const double (&rb)[kKB], double (&acc)[TMAX]) {
+template <typename ET>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  using type = float4;
+};
+template <>
+struct Vec16<double> {
+  using type = double2;
+};
+
+// acc[t] += sum over this lane's slots of M[t][b] r[b], 16 rows of one chunk;
+// KG slot groups per pass (all loads of a pass are issued before the FMAs)
+template <typename ET, int KG>
+__device__ inline void chunk_times(const ET* __restrict__ Mk, int n, int lane, const int* __restrict__ bi,
+                                   const double* __restrict__ r, double (&acc)[kRowChunk]) {
+  constexpr int V = 16 / (int)sizeof(ET), QN = kRowChunk / V;
+  using VT = typename Vec16<ET>::type;
+  const int ngrp = (n + 63) / 64;
+  for (int g0 = 0; g0 < ngrp; g0 += KG) {
+    int idx[KG];
+    bool ok[KG];
+    VT m[KG][QN];
 #pragma unroll
-  for (int t = 0; t < TMAX; ++t) {
-    if (t < tc) {
-      const long long row = (long long)(t0 + t) * n;
+    for (int k = 0; k < KG; ++k) {
+      const int g = g0 + k < ngrp ? g0 + k : ngrp - 1;
+      const int ng = min(64, n - 64 * g);
+      const int l = min(lane, ng - 1);
+      ok[k] = (g0 + k < ngrp) && lane < ng;
+      idx[k] = bi[64 * g + l];
+      const VT* p = reinterpret_cast<const VT*>(Mk + 1024LL * g) + l;
 #pragma unroll
-      for (int k = 0; k < kKB; ++k) {
-        const int b = bb + lane + 64 * k;
-        if (b < n) acc[t] = fma((double)Mc[row + b], rb[k], acc[t]);
+      for (int q = 0; q < QN; ++q) m[k][q] = p[q * ng];
+    }
+    double rb[KG];
+#pragma unroll
+    for (int k = 0; k < KG; ++k) {
+      const double v = r[idx[k]];
+      rb[k] = ok[k] ? v : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < KG; ++k) {
+#pragma unroll
+      for (int q = 0; q < QN; ++q) {
+        const ET* e = reinterpret_cast<const ET*>(&m[k][q]);
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[q * V + v] = fma((double)e[v], rb[k], acc[q * V + v]);
       }
     }
   }
 }
 
-template <int TMAX, typename MT>
+// Sum of acc[t] over the wavefront; on return lanes with (lane & 3) == 0 hold
+// the total of row t = 8 b5 + 4 b4 + 2 b3 + b2 (b_i = bit i of the lane).
+__device__ inline double wave_rows_sum(const double (&acc)[kRowChunk], int lane) {
+  double v8[8], v4[4], v2[2];
+  {
+    const bool hi = lane & 32;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const double keep = hi ? acc[8 + i] : acc[i], send = hi ? acc[i] : acc[8 + i];
+      v8[i] = keep + __shfl_xor(send, 32);
+    }
+  }
+  {
+    const bool hi = lane & 16;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const double keep = hi ? v8[4 + i] : v8[i], send = hi ? v8[i] : v8[4 + i];
+      v4[i] = keep + __shfl_xor(send, 16);
+    }
+  }
+  {
+    const bool hi = lane & 8;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const double keep = hi ? v4[2 + i] : v4[i], send = hi ? v4[i] : v4[2 + i];
+      v2[i] = keep + __shfl_xor(send, 8);
+    }
+  }
+  const bool hi = lane & 4;
+  double v = (hi ? v2[1] : v2[0]) + __shfl_xor(hi ? v2[0] : v2[1], 4);
+  v += __shfl_xor(v, 2);
+  v += __shfl_xor(v, 1);
+  return v;
+}
+
+template <typename ET, int KG>
+__device__ inline void apply_cluster(const ET* __restrict__ Mc, int n, int T, int lane, const int* __restrict__ bi,
+                                     const double* __restrict__ r, double* __restrict__ zc) {
+  const int trow = 8 * ((lane >> 5) & 1) + 4 * ((lane >> 4) & 1) + 2 * ((lane >> 3) & 1) + ((lane >> 2) & 1);
+  for (int t0 = 0; t0 < T; t0 += kRowChunk) {
+    double acc[kRowChunk];
+#pragma unroll
+    for (int t = 0; t < kRowChunk; ++t) acc[t] = 0.0;
+    chunk_times<ET, KG>(Mc + (long long)(t0 / kRowChunk) * apply_chunk_elems(n), n, lane, bi, r, acc);
+    const double s = wave_rows_sum(acc, lane);
+    if ((lane & 3) == 0 && t0 + trow < T) zc[t0 + trow] = s;
+  }
+}
+
+template <typename MT>
 __global__ __launch_bounds__(64 * kApplyWaves) void k_rasm_apply(
     long long ncell, const long long* __restrict__ bptr, const int* __restrict__ bidx,
     const int* __restrict__ ista, const int* __restrict__ iend, const long long* __restrict__ mptr,
     const MT* __restrict__ M, const long long* __restrict__ fb, const double* __restrict__ M64,
     const double* __restrict__ r, double* __restrict__ z) {
-  __shared__ double red[kApplyWaves][TMAX][65];
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  const long long c = (long long)blockIdx.x * kApplyWaves + wave;
+  const long long c = (long long)xcd_block(blockIdx.x, gridDim.x) * kApplyWaves + wave;
   if (c >= ncell) return;
   const long long b0 = bptr[c];
   const int n = (int)(bptr[c + 1] - b0);
   if (n == 0) return;
-  const int T = iend[c] - ista[c] + 1;
+  const int s = ista[c];
+  const int T = iend[c] - s + 1;
   const long long f = fb ? fb[c] : -1;
-  const MT* Mc = M + mptr[c];
-  const double* Md = f >= 0 ? M64 + f : nullptr;
-  double (*rw)[65] = red[wave];
-  for (int t0 = 0; t0 < T; t0 += TMAX) {
-    const int tc = min(TMAX, T - t0);
-    double acc[TMAX];
-#pragma unroll
-    for (int t = 0; t < TMAX; ++t) acc[t] = 0.0;
-    for (int bb = 0; bb < n; bb += 64 * kKB) {
-      // all index loads, then all gathers, then the streamed rows
-      int idx[kKB];
-      double rb[kKB];
-#pragma unroll
-      for (int k = 0; k < kKB; ++k) {
-        const int b = bb + lane + 64 * k;
-        idx[k] = b < n ? bidx[b0 + b] : -1;
-      }
-#pragma unroll
-      for (int k = 0; k < kKB; ++k) rb[k] = idx[k] >= 0 ? r[idx[k]] : 0.0;
-      if (Md)
-        rows_times<TMAX, double>(Md, n, t0, tc, bb, lane, rb, acc);
-      else
-        rows_times<TMAX, MT>(Mc, n, t0, tc, bb, lane, rb, acc);
-    }
-#pragma unroll
-    for (int t = 0; t < TMAX; ++t) rw[t][lane] = acc[t];
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    if (lane < tc) {
-      double s = 0.0;
-      for (int l = 0; l < 64; ++l) s += rw[lane][l];
-      z[ista[c] + t0 + lane] = s;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  }
+  if (f >= 0)
+    apply_cluster<double, 2>(M64 + f, n, T, lane, bidx + b0, r, z + s);
+  else
+    apply_cluster<MT, sizeof(MT) == 4 ? 4 : 2>(M + mptr[c], n, T, lane, bidx + b0, r, z + s);
 }
 
 static inline dim3 apply_grid(long long ncell) { return dim3((unsigned)((ncell + kApplyWaves - 1) / kApplyWaves)); }
@@ -99,15 +158,15 @@ void rasm_apply_fp64(hipStream_t st, long long ncell, const long long* bptr, con
                      const int* ista, const int* iend, const long long* mptr, const double* M,
                      const double* r, double* z, int tmax) {
   (void)tmax;
-  hipLaunchKernelGGL((k_rasm_apply<16, double>), apply_grid(ncell), dim3(64 * kApplyWaves), 0, st, ncell, bptr,
-                     bidx, ista, iend, mptr, M, nullptr, nullptr, r, z);
+  hipLaunchKernelGGL((k_rasm_apply<double>), apply_grid(ncell), dim3(64 * kApplyWaves), 0, st, ncell, bptr, bidx,
+                     ista, iend, mptr, M, nullptr, nullptr, r, z);
 }
 
 void rasm_apply_fp32(hipStream_t st, long long ncell, const long long* bptr, const int* bidx,
                      const int* ista, const int* iend, const long long* mptr, const float* M,
                      const long long* fb, const double* M64, const double* r, double* z) {
-  hipLaunchKernelGGL((k_rasm_apply<16, float>), apply_grid(ncell), dim3(64 * kApplyWaves), 0, st, ncell, bptr,
-                     bidx, ista, iend, mptr, M, fb, M64, r, z);
+  hipLaunchKernelGGL((k_rasm_apply<float>), apply_grid(ncell), dim3(64 * kApplyWaves), 0, st, ncell, bptr, bidx,
+                     ista, iend, mptr, M, fb, M64, r, z);
 }
 
 }  // namespace prbf
