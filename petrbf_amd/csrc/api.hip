@@ -290,96 +290,80 @@ std::vector<std::pair<long long, CholCfg>>& tuned_cache() {
 }
 
 template <typename Real>
-void launch_chunk(prbf_ctx* c, bool fp32, const CholCfg& cfg, long long k0, long long ncl, int nmax, int tmax,
-                  const long long* wptr, Real* W, Real* Mout, int* status) {
+void launch_blocks(prbf_ctx* c, bool fp32, const CholCfg& cfg, long long nwork, const int* order, int* next, int nwg,
+                   long long ws_slot, int nmax, int tmax, Real* W, const long long* mptr, Real* Mout, int* status) {
   const Grid& g = c->g;
   if (fp32)
-    block_chol_fp32(c->st, g.dim, cfg.nw, cfg.npc, k0, ncl, nmax, tmax, c->bptr, c->bidx, c->ista, c->iend, c->xi,
-                    c->yi, c->zi, wptr, (float*)W, c->mptr, (float*)Mout, c->prm.sigma, status);
+    block_chol_fp32(c->st, g.dim, cfg.nw, cfg.npc, nwork, order, next, nwg, ws_slot, nmax, tmax, c->bptr, c->bidx,
+                    c->ista, c->iend, c->xi, c->yi, c->zi, (float*)W, mptr, (float*)Mout, c->prm.sigma, status);
   else
-    block_chol_fp64(c->st, g.dim, cfg.nw, cfg.npc, k0, nullptr, ncl, nmax, tmax, c->bptr, c->bidx, c->ista, c->iend,
-                    c->xi, c->yi, c->zi, wptr, (double*)W, c->mptr, (double*)Mout, c->prm.sigma, c->d_err);
+    block_chol_fp64(c->st, g.dim, cfg.nw, cfg.npc, nwork, order, next, nwg, ws_slot, nmax, tmax, c->bptr, c->bidx,
+                    c->ista, c->iend, c->xi, c->yi, c->zi, (double*)W, mptr, (double*)Mout, c->prm.sigma,
+                    c->d_err);
   PRBF_HIP(hipGetLastError());
 }
 
-// Runs the block factorisation over all clusters in chunks bounded by a
-// workspace budget.  fp32: blocks in FP32 on the FP32 MFMA, core rows stored
-// in FP32, failed blocks listed in `failed` for an FP64 redo.  Unless a
-// configuration was forced, the first call for a (precision, block size)
-// times every launch configuration that fits in LDS on the first
-// min(4096, chunk) blocks and keeps the fastest (the project's autotuner).
+// Factorises the blocks of the clusters in `order` (largest first) in one
+// persistent launch, each resident workgroup reusing one workspace slot.
+// fp32: blocks in FP32 on the FP32 MFMA, per-block status in `status`.  The
+// launch configuration is forced, cached, or, on the first call for a
+// (precision, block size), timed for every configuration that fits in LDS on
+// the first min(4096, nwork) blocks of the list, keeping the fastest (the
+// project's autotuner).
 template <typename Real>
-void factor_chunks(prbf_ctx* c, bool fp32, const std::vector<long long>& wsz, Real* Mout, std::vector<int>* failed) {
+void factor_blocks(prbf_ctx* c, bool fp32, const std::vector<int>& order, int nmax, int tmax, const long long* mptr,
+                   Real* Mout, int* status, bool tune_allowed) {
   const Grid& g = c->g;
-  const long long nc = g.ncell;
+  const long long nwork = (long long)order.size();
+  if (nwork == 0) return;
+  DBuf<int> d_order, next;
+  d_order.ensure(order.size());
+  next.ensure(1);
+  PRBF_HIP(hipMemcpyAsync(d_order, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice, c->st));
+  const long long ws_slot = block_workspace(nmax, tmax);
   size_t free_b = 0, total_b = 0;
   PRBF_HIP(hipMemGetInfo(&free_b, &total_b));
-  long long budget = (long long)std::min<size_t>(free_b / 3, (size_t)8 << 30) / (long long)sizeof(Real);
-  long long maxw = *std::max_element(wsz.begin(), wsz.end());
-  need(budget >= maxw, PRBF_ERR_NOMEM, "not enough device memory for one RASM block");
-  DBuf<Real> W;
-  DBuf<long long> wptr;
-  DBuf<int> status;
-  if (fp32) {
-    status.ensure(nc);
-    PRBF_HIP(hipMemsetAsync(status, 0, sizeof(int) * nc, c->st));
-  }
-  std::vector<long long> h_wptr;
-  const long long kMaxChunk = 1LL << 20;
-  wptr.ensure(std::min<long long>(nc, kMaxChunk) + 1);
-  // configuration: forced, cached, or tuned on the first chunk
-  const int ktmax = (c->max_buffer + 15) / 16;
+  const long long cap = (long long)(free_b / 3 / sizeof(Real)) / ws_slot;
+  need(cap >= 1, PRBF_ERR_NOMEM, "not enough device memory for one RASM block");
+  auto workgroups = [&](const CholCfg& cf) {
+    long long w = block_chol_workgroups(fp32, g.dim, cf.nw, cf.npc, nmax, tmax);
+    return (int)std::max<long long>(1, std::min(std::min(w, nwork), cap));
+  };
+  const int ktmax = (nmax + 15) / 16;
   const long long key = (fp32 ? 1000000LL : 0LL) + ktmax * 1000LL + g.dim;
   CholCfg cfg{c->chol_waves, c->chol_panel};
-  bool tune = !c->tuning_forced;
+  bool tune = tune_allowed && !c->tuning_forced;
   if (tune)
     for (auto& kv : tuned_cache())
       if (kv.first == key) { cfg = kv.second; tune = false; }
-  long long k0 = 0;
-  while (k0 < nc) {
-    long long k1 = k0, used = 0;
-    h_wptr.clear();
-    int nmax = 0, tmax = 0;
-    while (k1 < nc && k1 - k0 < kMaxChunk) {
-      if (used + wsz[k1] > budget && k1 > k0) break;
-      h_wptr.push_back(used);
-      used += wsz[k1];
-      nmax = std::max<int>(nmax, (int)(c->h_bptr[k1 + 1] - c->h_bptr[k1]));
-      tmax = std::max(tmax, c->h_iend[k1] - c->h_ista[k1] + 1);
-      ++k1;
+  DBuf<Real> W;
+  if (tune) {
+    const long long ns = std::min<long long>(4096, nwork);
+    float best = 1e30f;
+    for (const CholCfg& cand : kCholCfgs) {
+      if (block_lds_bytes(fp32, nmax, tmax, cand.nw, cand.npc) > kMaxLdsBytes) continue;
+      const int nwg = workgroups(cand);
+      W.ensure((size_t)nwg * ws_slot);
+      PRBF_HIP(hipEventRecord(c->ev0, c->st));
+      launch_blocks<Real>(c, fp32, cand, ns, d_order, next, nwg, ws_slot, nmax, tmax, W.p, mptr, Mout, status);
+      PRBF_HIP(hipEventRecord(c->ev1, c->st));
+      PRBF_HIP(hipEventSynchronize(c->ev1));
+      float ms = 0;
+      PRBF_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+      if (ms < best) { best = ms; cfg = cand; }
     }
-    W.ensure((size_t)std::max<long long>(used, 1));
-    sync(c);  // the previous chunk has consumed wptr before it is overwritten
-    PRBF_HIP(hipMemcpy(wptr, h_wptr.data(), sizeof(long long) * h_wptr.size(), hipMemcpyHostToDevice));
-    if (tune) {
-      const long long ns = std::min<long long>(4096, k1 - k0);
-      float best = 1e30f;
-      for (const CholCfg& cand : kCholCfgs) {
-        if (block_lds_bytes(fp32, c->max_buffer, c->max_core, cand.nw, cand.npc) > kMaxLdsBytes) continue;
-        PRBF_HIP(hipEventRecord(c->ev0, c->st));
-        launch_chunk<Real>(c, fp32, cand, k0, ns, nmax, tmax, wptr, W.p, Mout, status);
-        PRBF_HIP(hipEventRecord(c->ev1, c->st));
-        PRBF_HIP(hipEventSynchronize(c->ev1));
-        float ms = 0;
-        PRBF_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-        if (ms < best) { best = ms; cfg = cand; }
-      }
-      tuned_cache().push_back({key, cfg});
-      tune = false;
-    }
-    launch_chunk<Real>(c, fp32, cfg, k0, k1 - k0, nmax, tmax, wptr, W.p, Mout, status);
-    k0 = k1;
+    tuned_cache().push_back({key, cfg});
   }
+  if (block_lds_bytes(fp32, nmax, tmax, cfg.nw, cfg.npc) > kMaxLdsBytes) cfg = CholCfg{4, 1};
+  need(block_lds_bytes(fp32, nmax, tmax, cfg.nw, cfg.npc) <= kMaxLdsBytes, PRBF_ERR_LIMIT,
+       fp32 ? "RASM block too large for the LDS panel of the block factorisation"
+            : "RASM block too large for the LDS panel of the FP64 block factorisation");
+  const int nwg = workgroups(cfg);
+  W.ensure((size_t)nwg * ws_slot);
+  launch_blocks<Real>(c, fp32, cfg, nwork, d_order, next, nwg, ws_slot, nmax, tmax, W.p, mptr, Mout, status);
   c->used_nw = cfg.nw;
   c->used_npc = cfg.npc;
   sync(c);
-  if (fp32 && failed) {
-    std::vector<int> h(nc);
-    PRBF_HIP(hipMemcpy(h.data(), status, sizeof(int) * nc, hipMemcpyDeviceToHost));
-    failed->clear();
-    for (long long k = 0; k < nc; ++k)
-      if (h[k]) failed->push_back((int)k);
-  }
 }
 
 void rasm_setup_impl(prbf_ctx* c, int pc) {
@@ -390,13 +374,18 @@ void rasm_setup_impl(prbf_ctx* c, int pc) {
   const Grid& g = c->g;
   const long long nc = g.ncell;
   c->h_mptr.assign(nc + 1, 0);
-  std::vector<long long> wsz(nc, 0);
+  std::vector<int> order;
+  order.reserve(nc);
   for (long long k = 0; k < nc; ++k) {
     long long n = c->h_bptr[k + 1] - c->h_bptr[k];
     long long T = n ? (c->h_iend[k] - c->h_ista[k] + 1) : 0;
     c->h_mptr[k + 1] = c->h_mptr[k] + apply_elems((int)n, (int)T);
-    wsz[k] = n ? block_workspace((int)n, (int)T) : 0;
+    if (n) order.push_back((int)k);
   }
+  // largest blocks first: the persistent workgroups then finish together
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return c->h_bptr[a + 1] - c->h_bptr[a] > c->h_bptr[b + 1] - c->h_bptr[b];
+  });
   need(block_lds_bytes(fp32, c->max_buffer, c->max_core, 4, 1) <= kMaxLdsBytes, PRBF_ERR_LIMIT,
        "RASM block too large for the LDS panel of the block factorisation");
   c->rasm_ready = false;
@@ -406,25 +395,29 @@ void rasm_setup_impl(prbf_ctx* c, int pc) {
   c->n_fallback = 0;
   if (!fp32) {
     c->M.ensure(c->h_mptr[nc]);
-    factor_chunks<double>(c, false, wsz, c->M.p, nullptr);
+    factor_blocks<double>(c, false, order, c->max_buffer, c->max_core, c->mptr, c->M.p, nullptr, true);
   } else {
     c->M32.ensure(c->h_mptr[nc]);
+    DBuf<int> status;
+    status.ensure(nc);
+    PRBF_HIP(hipMemsetAsync(status, 0, sizeof(int) * nc, c->st));
+    factor_blocks<float>(c, true, order, c->max_buffer, c->max_core, c->mptr, c->M32.p, status, true);
+    std::vector<int> h(nc);
+    PRBF_HIP(hipMemcpy(h.data(), status, sizeof(int) * nc, hipMemcpyDeviceToHost));
     std::vector<int> failed;
-    factor_chunks<float>(c, true, wsz, c->M32.p, &failed);
+    for (int k : order)
+      if (h[k]) failed.push_back(k);
     // FP64 redo of the blocks whose FP32 factorisation broke down
     c->fb.ensure(nc);
     std::vector<long long> h_fb(nc, -1), h_m64(nc + 1, 0);
-    long long tot = 0, wtot = 0;
+    long long tot = 0;
     int nmax = 0, tmax = 0;
-    std::vector<long long> h_w;
     for (int k : failed) {
       long long n = c->h_bptr[k + 1] - c->h_bptr[k];
       long long T = c->h_iend[k] - c->h_ista[k] + 1;
       h_fb[k] = tot;
       h_m64[k] = tot;
       tot += apply_elems((int)n, (int)T);
-      h_w.push_back(wtot);
-      wtot += wsz[k];
       nmax = std::max<int>(nmax, (int)n);
       tmax = std::max<int>(tmax, (int)T);
     }
@@ -432,20 +425,13 @@ void rasm_setup_impl(prbf_ctx* c, int pc) {
     PRBF_HIP(hipMemcpyAsync(c->fb, h_fb.data(), sizeof(long long) * nc, hipMemcpyHostToDevice, c->st));
     if (!failed.empty()) {
       c->M64fb.ensure(tot);
-      DBuf<double> W;
-      DBuf<long long> wp, mp;
-      DBuf<int> cl;
-      W.ensure(wtot);
-      wp.ensure(h_w.size());
+      DBuf<long long> mp;
       mp.ensure(nc + 1);
-      cl.ensure(failed.size());
-      PRBF_HIP(hipMemcpyAsync(wp, h_w.data(), sizeof(long long) * h_w.size(), hipMemcpyHostToDevice, c->st));
       PRBF_HIP(hipMemcpyAsync(mp, h_m64.data(), sizeof(long long) * (nc + 1), hipMemcpyHostToDevice, c->st));
-      PRBF_HIP(hipMemcpyAsync(cl, failed.data(), sizeof(int) * failed.size(), hipMemcpyHostToDevice, c->st));
-      block_chol_fp64(c->st, g.dim, 4, 1, 0, cl, (long long)failed.size(), nmax, tmax, c->bptr, c->bidx,
-                      c->ista, c->iend, c->xi, c->yi, c->zi, wp, W, mp, c->M64fb, c->prm.sigma, c->d_err);
-      PRBF_HIP(hipGetLastError());
-      sync(c);
+      const int saved_nw = c->used_nw, saved_npc = c->used_npc;
+      factor_blocks<double>(c, false, failed, nmax, tmax, mp, c->M64fb.p, nullptr, false);
+      c->used_nw = saved_nw;
+      c->used_npc = saved_npc;
     }
   }
   int herr = 0;

@@ -1,11 +1,13 @@
-// Batched RASM block setup on the matrix cores: one workgroup per cluster
-// block, 16x16 tiles, MFMA for every O(n^3) and O(n^2 T) term.
+// Batched RASM block setup on the matrix cores: one persistent launch whose
+// workgroups each factorise one cluster block at a time, 16x16 tiles, MFMA
+// for every O(n^3) and O(n^2 T) term.
 //
 // For cluster c with n buffer particles and T core particles the block is the
 // dense untruncated Gaussian over the buffer (2d/matmult.cxx:83-93).  PCASM's
 // restrict form (2d/rbf_interpolation.cxx:213-229) only ever reads the core
 // rows of A_c^{-1}, so the setup produces M_c = (A_c^{-1})[core, buffer]
-// (T x n, row-major) and the apply is a batched GEMV over M_c.
+// (T x n, stored as apply_offset lays it out) and the apply is a batched GEMV
+// over M_c.
 //
 //   phase 0  order the buffer with the core LAST and padding (identity) up to
 //            NP = 16*ceil(n/16); write the lower tiles of A' to a workspace in
@@ -21,7 +23,7 @@
 //            full backward tile substitution, both MFMA with the
 //            L_II^{-1} tiles kept from phase 1, 16 right-hand sides at a
 //            time held in LDS;
-//   phase 3  scatter each 16-column slice of X to M_c in buffer order.
+//   phase 3  write each 16-column slice of X to M_c in buffer order.
 // T = double is the plain path (exact FP64 block solves); T = float runs the
 // same kernel in FP32 on the FP32 MFMA (mixed precision), with a per-block
 // status so a block whose FP32 factorisation breaks down is redone in FP64.
@@ -117,16 +119,15 @@ __host__ __device__ inline size_t blk_lds_bytes(int nmax, int tmax, int nw, int 
   return region + (size_t)(nw + 2) * kLT * sizeof(T) + 16;
 }
 
+// One block: cluster c, workspace Wb (blk_ws elements), LDS as laid out below.
 template <typename Real, int DIM, int NW, int NPC>
-__global__ __launch_bounds__(NW * 64) void k_block_chol(
-    long long c0, const int* __restrict__ clist, const long long* __restrict__ bptr, const int* __restrict__ bidx,
-    const int* __restrict__ ista, const int* __restrict__ iend, const double* __restrict__ xs,
-    const double* __restrict__ ys, const double* __restrict__ zs, const long long* __restrict__ wptr,
-    Real* __restrict__ Wbase, const long long* __restrict__ mptr, Real* __restrict__ M, double inv_two_s2,
-    double norm, int* __restrict__ err, int* __restrict__ blk_status) {
+__device__ void factor_block(
+    long long c, Real* __restrict__ Wb, unsigned char* __restrict__ lds_raw, const long long* __restrict__ bptr,
+    const int* __restrict__ bidx, const int* __restrict__ ista, const int* __restrict__ iend,
+    const double* __restrict__ xs, const double* __restrict__ ys, const double* __restrict__ zs,
+    const long long* __restrict__ mptr, Real* __restrict__ M, double inv_two_s2, double norm, int* __restrict__ err,
+    int* __restrict__ blk_status) {
   typedef typename MF<Real>::v4 v4;
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  const long long c = clist ? (long long)clist[blockIdx.x] : c0 + blockIdx.x;
   const long long b0 = bptr[c];
   const int n = (int)(bptr[c + 1] - b0);
   if (n == 0) return;
@@ -137,7 +138,6 @@ __global__ __launch_bounds__(NW * 64) void k_block_chol(
   const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
   const int lr = lane & 15, lq = lane >> 4;
 
-  Real* Wb = Wbase + wptr[blockIdx.x];
   Real* Wd = Wb + (long long)nt * kTE;       // L_II^{-1}, row-major
 
   // LDS carve-up
@@ -455,14 +455,33 @@ __global__ __launch_bounds__(NW * 64) void k_block_chol(
   if (blk_status && tid == 0) blk_status[c] = 0;
 }
 
+// Persistent batched launch: each workgroup owns one workspace slot and takes
+// the next block from a global counter until the work list is exhausted (the
+// list is ordered largest block first, so uneven blocks balance over the
+// workgroups).  Every wavefront leaves through the same counter test.
+template <typename Real, int DIM, int NW, int NPC>
+__global__ __launch_bounds__(NW * 64) void k_block_chol(
+    long long nwork, const int* __restrict__ order, int* __restrict__ next, long long ws_slot,
+    const long long* __restrict__ bptr, const int* __restrict__ bidx, const int* __restrict__ ista,
+    const int* __restrict__ iend, const double* __restrict__ xs, const double* __restrict__ ys,
+    const double* __restrict__ zs, Real* __restrict__ Wbase, const long long* __restrict__ mptr, Real* __restrict__ M,
+    double inv_two_s2, double norm, size_t item_slot, int* __restrict__ err, int* __restrict__ blk_status) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  int* s_item = (int*)(lds_raw + item_slot);
+  Real* Wb = Wbase + (long long)blockIdx.x * ws_slot;
+  for (;;) {
+    if (threadIdx.x == 0) *s_item = atomicAdd(next, 1);
+    __syncthreads();
+    const long long item = *s_item;
+    __syncthreads();
+    if (item >= nwork) break;
+    factor_block<Real, DIM, NW, NPC>(order[item], Wb, lds_raw, bptr, bidx, ista, iend, xs, ys, zs, mptr, M,
+                                     inv_two_s2, norm, err, blk_status);
+  }
+}
+
 template <typename T>
-static void launch_block_chol(hipStream_t st, int dim, int nw, int npc, long long c0, const int* clist, long long ncl,
-                              int nmax, int tmax, const long long* bptr, const int* bidx, const int* ista,
-                              const int* iend, const double* xs, const double* ys, const double* zs,
-                              const long long* wptr, T* W, const long long* mptr, T* M, double sigma, int* err,
-                              int* blk_status) {
-  Kern k = make_kern(sigma);
-  size_t lds = blk_lds_bytes<T>(nmax, tmax, nw, npc);
+static void set_lds_attr() {
   static bool attr = false;
   if (!attr) {
     const void* fns[] = {(const void*)k_block_chol<T, 2, 4, 1>, (const void*)k_block_chol<T, 3, 4, 1>,
@@ -473,10 +492,46 @@ static void launch_block_chol(hipStream_t st, int dim, int nw, int npc, long lon
       PRBF_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     attr = true;
   }
-  dim3 grid((unsigned)ncl);
-#define PRBF_BC(D, NWV, NPCV)                                                                                  \
-  hipLaunchKernelGGL((k_block_chol<T, D, NWV, NPCV>), grid, dim3(NWV * 64), lds, st, c0, clist, bptr, bidx, ista, \
-                     iend, xs, ys, zs, wptr, W, mptr, M, k.inv_two_s2, k.norm, err, blk_status)
+}
+
+template <typename T>
+static const void* chol_fn(int dim, int nw, int npc) {
+#define PRBF_FN(D, NWV, NPCV) (const void*)k_block_chol<T, D, NWV, NPCV>
+  if (dim == 2) {
+    if (nw == 8) return npc == 2 ? PRBF_FN(2, 8, 2) : PRBF_FN(2, 8, 1);
+    return npc == 2 ? PRBF_FN(2, 4, 2) : PRBF_FN(2, 4, 1);
+  }
+  if (nw == 8) return npc == 2 ? PRBF_FN(3, 8, 2) : PRBF_FN(3, 8, 1);
+  return npc == 2 ? PRBF_FN(3, 4, 2) : PRBF_FN(3, 4, 1);
+#undef PRBF_FN
+}
+
+template <typename T>
+static int resident_workgroups(int dim, int nw, int npc, int nmax, int tmax) {
+  set_lds_attr<T>();
+  size_t lds = blk_lds_bytes<T>(nmax, tmax, nw, npc);
+  int per_cu = 0, dev = 0, ncu = 0;
+  PRBF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, chol_fn<T>(dim, nw, npc), nw * 64, lds));
+  PRBF_HIP(hipGetDevice(&dev));
+  PRBF_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  return (per_cu < 1 ? 1 : per_cu) * ncu;
+}
+
+template <typename T>
+static void launch_block_chol(hipStream_t st, int dim, int nw, int npc, long long nwork, const int* order, int* next,
+                              int nwg, long long ws_slot, int nmax, int tmax, const long long* bptr, const int* bidx,
+                              const int* ista, const int* iend, const double* xs, const double* ys, const double* zs,
+                              T* W, const long long* mptr, T* M, double sigma, int* err, int* blk_status) {
+  Kern k = make_kern(sigma);
+  set_lds_attr<T>();
+  size_t lds = blk_lds_bytes<T>(nmax, tmax, nw, npc);
+  const size_t item_slot = lds - 8;  // past the flag word of the largest block
+  PRBF_HIP(hipMemsetAsync(next, 0, sizeof(int), st));
+  dim3 grid((unsigned)nwg);
+#define PRBF_BC(D, NWV, NPCV)                                                                                     \
+  hipLaunchKernelGGL((k_block_chol<T, D, NWV, NPCV>), grid, dim3(NWV * 64), lds, st, nwork, order, next, ws_slot, \
+                     bptr, bidx, ista, iend, xs, ys, zs, W, mptr, M, k.inv_two_s2, k.norm, item_slot, err,         \
+                     blk_status)
 #define PRBF_BC_NPC(D, NWV) \
   if (npc == 2) PRBF_BC(D, NWV, 2); else PRBF_BC(D, NWV, 1)
   if (dim == 2) {
@@ -494,20 +549,25 @@ size_t block_lds_bytes(bool fp32, int nmax, int tmax, int nw, int npc) {
   return fp32 ? blk_lds_bytes<float>(nmax, tmax, nw, npc) : blk_lds_bytes<double>(nmax, tmax, nw, npc);
 }
 
-void block_chol_fp64(hipStream_t st, int dim, int nw, int npc, long long c0, const int* clist, long long ncl,
-                     int nmax, int tmax, const long long* bptr, const int* bidx, const int* ista, const int* iend,
-                     const double* xs, const double* ys, const double* zs, const long long* wptr, double* W,
-                     const long long* mptr, double* M, double sigma, int* err) {
-  launch_block_chol<double>(st, dim, nw, npc, c0, clist, ncl, nmax, tmax, bptr, bidx, ista, iend, xs, ys, zs, wptr,
-                            W, mptr, M, sigma, err, nullptr);
+int block_chol_workgroups(bool fp32, int dim, int nw, int npc, int nmax, int tmax) {
+  return fp32 ? resident_workgroups<float>(dim, nw, npc, nmax, tmax)
+              : resident_workgroups<double>(dim, nw, npc, nmax, tmax);
 }
 
-void block_chol_fp32(hipStream_t st, int dim, int nw, int npc, long long c0, long long ncl, int nmax, int tmax,
-                     const long long* bptr, const int* bidx, const int* ista, const int* iend, const double* xs,
-                     const double* ys, const double* zs, const long long* wptr, float* W, const long long* mptr,
-                     float* M, double sigma, int* blk_status) {
-  launch_block_chol<float>(st, dim, nw, npc, c0, nullptr, ncl, nmax, tmax, bptr, bidx, ista, iend, xs, ys, zs, wptr,
-                           W, mptr, M, sigma, nullptr, blk_status);
+void block_chol_fp64(hipStream_t st, int dim, int nw, int npc, long long nwork, const int* order, int* next, int nwg,
+                     long long ws_slot, int nmax, int tmax, const long long* bptr, const int* bidx, const int* ista,
+                     const int* iend, const double* xs, const double* ys, const double* zs, double* W,
+                     const long long* mptr, double* M, double sigma, int* err) {
+  launch_block_chol<double>(st, dim, nw, npc, nwork, order, next, nwg, ws_slot, nmax, tmax, bptr, bidx, ista, iend,
+                            xs, ys, zs, W, mptr, M, sigma, err, nullptr);
+}
+
+void block_chol_fp32(hipStream_t st, int dim, int nw, int npc, long long nwork, const int* order, int* next, int nwg,
+                     long long ws_slot, int nmax, int tmax, const long long* bptr, const int* bidx, const int* ista,
+                     const int* iend, const double* xs, const double* ys, const double* zs, float* W,
+                     const long long* mptr, float* M, double sigma, int* blk_status) {
+  launch_block_chol<float>(st, dim, nw, npc, nwork, order, next, nwg, ws_slot, nmax, tmax, bptr, bidx, ista, iend,
+                           xs, ys, zs, W, mptr, M, sigma, nullptr, blk_status);
 }
 
 }  // namespace prbf

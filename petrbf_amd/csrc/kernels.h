@@ -48,16 +48,19 @@ __host__ __device__ inline long long apply_offset(int n, int t, int b) {
 // Workspace (elements) and LDS bytes of one block factorisation.
 long long block_workspace(int n, int T);
 size_t block_lds_bytes(bool fp32, int nmax, int tmax, int nw, int npc);
-// Factorises the blocks of clusters [c0, c0+ncl) and writes their core rows.
-// clist != nullptr: factorise clusters clist[0..ncl) instead of [c0, c0+ncl).
-void block_chol_fp64(hipStream_t st, int dim, int nw, int npc, long long c0, const int* clist, long long ncl,
-                     int nmax, int tmax, const long long* bptr, const int* bidx, const int* ista, const int* iend,
-                     const double* xs, const double* ys, const double* zs, const long long* wptr, double* W,
+// Workgroups of the persistent block factorisation that fit on the device at once.
+int block_chol_workgroups(bool fp32, int dim, int nw, int npc, int nmax, int tmax);
+// Factorises the blocks of clusters order[0..nwork) and writes their core rows:
+// nwg persistent workgroups, each with a workspace slot of ws_slot elements
+// of W, take blocks from the counter *next (reset by the launcher).
+void block_chol_fp64(hipStream_t st, int dim, int nw, int npc, long long nwork, const int* order, int* next, int nwg,
+                     long long ws_slot, int nmax, int tmax, const long long* bptr, const int* bidx, const int* ista,
+                     const int* iend, const double* xs, const double* ys, const double* zs, double* W,
                      const long long* mptr, double* M, double sigma, int* err);
-void block_chol_fp32(hipStream_t st, int dim, int nw, int npc, long long c0, long long ncl, int nmax, int tmax,
-                     const long long* bptr, const int* bidx, const int* ista, const int* iend, const double* xs,
-                     const double* ys, const double* zs, const long long* wptr, float* W, const long long* mptr,
-                     float* M, double sigma, int* blk_status);
+void block_chol_fp32(hipStream_t st, int dim, int nw, int npc, long long nwork, const int* order, int* next, int nwg,
+                     long long ws_slot, int nmax, int tmax, const long long* bptr, const int* bidx, const int* ista,
+                     const int* iend, const double* xs, const double* ys, const double* zs, float* W,
+                     const long long* mptr, float* M, double sigma, int* blk_status);
 void rasm_apply_fp64(hipStream_t st, long long ncell, const long long* bptr, const int* bidx,
                      const int* ista, const int* iend, const long long* mptr, const double* M,
                      const double* r, double* z, int tmax);
