@@ -72,6 +72,10 @@ typedef struct {
   double rtol;      /* relative tolerance on the solver's residual norm */
   double abstol;    /* 1e-50 */
   double dtol;      /* 1e5 */
+  int graph;        /* 1 (default): each Arnoldi step (preconditioner, matvec,
+                       Gram-Schmidt, Givens update) runs as one captured HIP
+                       graph, one host wait per iteration; 0: kernel by kernel.
+                       Both give the same bits. */
 } prbf_solve_opts;
 
 typedef struct {
@@ -81,8 +85,8 @@ typedef struct {
   double rnorm;      /* final residual norm of the convergence test */
   double t_setup_s;  /* RASM setup wall time (0 if it was already set up) */
   double t_solve_s;  /* Krylov iterations wall time */
-  double t_matvec_s; /* summed device time of the matvec kernels */
-  double t_pc_s;     /* summed device time of the preconditioner applies */
+  double t_matvec_s; /* summed device time of the matvec kernels (0 with graph = 1) */
+  double t_pc_s;     /* summed device time of the preconditioner applies (0 with graph = 1) */
   int n_matvec;
   int n_pc;
   int n_fallback;    /* RASM_FP32: blocks redone in FP64 after an FP32 breakdown */

@@ -36,7 +36,7 @@ class PrbfParams(C.Structure):
 
 class PrbfSolveOpts(C.Structure):
     _fields_ = [("ksp", C.c_int), ("pc", C.c_int), ("restart", C.c_int), ("max_it", C.c_int),
-                ("rtol", C.c_double), ("abstol", C.c_double), ("dtol", C.c_double)]
+                ("rtol", C.c_double), ("abstol", C.c_double), ("dtol", C.c_double), ("graph", C.c_int)]
 
 
 class PrbfSolveInfo(C.Structure):

@@ -92,6 +92,16 @@ struct prbf_ctx {
   DBuf<long long> fb;
   DBuf<double> M64fb;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // Arnoldi steps captured as graphs: device state of the cycle, one
+  // executable graph per step index, and the launch arguments they captured
+  DBuf<double> kst;
+  std::vector<hipGraphExec_t> kgraphs;
+  std::vector<unsigned long long> kgraph_key;
+  void drop_graphs() {
+    for (hipGraphExec_t e : kgraphs) (void)hipGraphExecDestroy(e);
+    kgraphs.clear();
+    kgraph_key.clear();
+  }
 
   const double* xj_() const { return same ? xi.p : xj.p; }
   const double* yj_() const { return same ? yi.p : yj.p; }
@@ -480,6 +490,79 @@ struct Timed {
   }
 };
 
+// Preconditioner application without timing (usable under graph capture).
+void pc_apply_raw(prbf_ctx* c, int pc, long long n, const double* v, double* y) {
+  if (pc == PRBF_PC_NONE)
+    PRBF_HIP(hipMemcpyAsync(y, v, sizeof(double) * n, hipMemcpyDeviceToDevice, c->st));
+  else
+    rasm_apply_impl(c, v, y);
+}
+
+// Offsets in the Arnoldi state (layout in krylov.hip).
+inline long long ks_rs(int m) { return (long long)(m + 1) * (m + 1) + 2 * (m + 1); }
+inline long long ks_inv(int m) { return ks_rs(m) + m + 2; }
+inline long long ks_out(int m) { return ks_inv(m) + 1; }
+constexpr int kPinStep = 256;  // pinned slots receiving a step's residual estimate and flags
+
+// Enqueues Arnoldi step j of a cycle: the new basis vector, its Gram-Schmidt
+// coefficients and norm, the Givens update of H and the residual estimate,
+// the normalisation, and the copy of the estimate to pinned memory.
+void enqueue_arnoldi_step(prbf_ctx* c, int pc, bool flex, long long n, int m, int j) {
+  double* V = c->V;
+  double* w = V + (long long)(j + 1) * n;
+  double* dsc = c->scal;
+  if (flex) {
+    double* zj = c->Z.p + (long long)j * n;
+    pc_apply_raw(c, pc, n, V + (long long)j * n, zj);
+    matvec_impl(c, 1, zj, w);
+  } else {
+    matvec_impl(c, 1, V + (long long)j * n, c->v1);
+    pc_apply_raw(c, pc, n, c->v1, w);
+  }
+  mdot(c->st, n, w, V, n, j + 1, c->part, dsc);
+  maxpy_norm2(c->st, n, w, V, n, j + 1, dsc, -1.0, c->part, dsc + 64);
+  arnoldi_step(c->st, j, m, dsc, c->kst);
+  scale_dev(c->st, n, w, c->kst.p + ks_inv(m));
+  PRBF_HIP(hipMemcpyAsync(c->pin + kPinStep, c->kst.p + ks_out(m), 3 * sizeof(double), hipMemcpyDeviceToHost,
+                          c->st));
+}
+
+// (Re)captures the m Arnoldi-step graphs when anything they captured changed:
+// sizes, the preconditioner, or the address of any buffer their kernels read.
+void ensure_graphs(prbf_ctx* c, int pc, bool flex, long long n, int m) {
+  auto u = [](const void* p) { return (unsigned long long)(uintptr_t)p; };
+  unsigned long long sig;
+  std::memcpy(&sig, &c->prm.sigma, sizeof(sig));
+  std::vector<unsigned long long> key = {
+      (unsigned long long)n, (unsigned long long)m, flex ? 1ULL : 0ULL, (unsigned long long)pc,
+      (unsigned long long)c->rasm_pc, (unsigned long long)c->g.ncell, (unsigned long long)c->ni,
+      (unsigned long long)c->nj, (unsigned long long)c->prm.dim, sig, u(c->V.p), u(c->Z.p), u(c->v1.p),
+      u(c->part.p), u(c->scal.p), u(c->kst.p), u(c->pin), u(c->ista.p), u(c->iend.p), u(c->tptr.p), u(c->tidx.p),
+      u(c->xi.p), u(c->yi.p), u(c->zi.p), u(c->xj_()), u(c->yj_()), u(c->zj_()), u(c->bptr.p), u(c->bidx.p),
+      u(c->mptr.p), u(c->M.p), u(c->M32.p), u(c->fb.p), u(c->M64fb.p)};
+  if (key == c->kgraph_key && (int)c->kgraphs.size() == m) return;
+  c->drop_graphs();
+  for (int j = 0; j < m; ++j) {
+    PRBF_HIP(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    hipGraph_t gr = nullptr;
+    try {
+      enqueue_arnoldi_step(c, pc, flex, n, m, j);
+    } catch (...) {
+      (void)hipStreamEndCapture(c->st, &gr);
+      if (gr) (void)hipGraphDestroy(gr);
+      (void)hipGetLastError();
+      throw;
+    }
+    PRBF_HIP(hipStreamEndCapture(c->st, &gr));
+    hipGraphExec_t ex = nullptr;
+    hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    PRBF_HIP(e);
+    c->kgraphs.push_back(ex);
+  }
+  c->kgraph_key = key;
+}
+
 // Restarted GMRES (left PC) or FGMRES (right PC), PETSc's KSPGMRESCycle
 // structure: classical Gram-Schmidt without refinement, Givens rotations on
 // the host, solution update at the end of each cycle.
@@ -490,12 +573,17 @@ void krylov_impl(prbf_ctx* c, const prbf_solve_opts& o, const double* b, double*
   need(m >= 1 && m < 63, PRBF_ERR_ARG, "restart must be in 1..62");
   need(o.pc == PRBF_PC_NONE || c->rasm_ready, PRBF_ERR_STATE, "rasm_setup first");
   const bool flex = (o.ksp == PRBF_KSP_FGMRES);
+  const bool graph = o.graph != 0 && c->st != nullptr;  // the legacy default stream cannot be captured
   c->V.ensure((size_t)(m + 1) * n);
   if (flex) c->Z.ensure((size_t)m * n);
   c->v1.ensure(n);
   c->v2.ensure(n);
   c->part.ensure(kRedBlocks * 64);
   c->scal.ensure(256);
+  if (graph) {
+    c->kst.ensure(arnoldi_state_len(m));
+    ensure_graphs(c, o.pc, flex, n, m);
+  }
   double* V = c->V;
   auto Vj = [&](int j) { return V + (long long)j * n; };
   double* tmp = c->v1;
@@ -551,6 +639,7 @@ void krylov_impl(prbf_ctx* c, const prbf_solve_opts& o, const double* b, double*
     double res = read_norm();
     if (res != 0.0) scale(c->st, n, Vj(0), 1.0 / res);
     rs[0] = res;
+    if (graph) set_scalar(c->st, c->kst.p + ks_rs(m), res);
     log_res(res);
     if (res == 0.0) {
       reason = 3;
@@ -560,6 +649,25 @@ void krylov_impl(prbf_ctx* c, const prbf_solve_opts& o, const double* b, double*
     int loc = 0;
     bool hapend = false;
     while (!reason && loc < m && its < o.max_it) {
+      if (graph) {
+        PRBF_HIP(hipGraphLaunch(c->kgraphs[loc], c->st));
+        sync(c);
+        const double* out = pin + kPinStep;
+        ++inf.n_matvec;
+        if (o.pc != PRBF_PC_NONE) ++inf.n_pc;
+        if (out[2] != 0.0) {
+          reason = -2;
+          break;
+        }
+        hapend = out[1] != 0.0;
+        res = out[0];
+        ++loc;
+        ++its;
+        log_res(res);
+        reason = conv(its, res);
+        if (hapend && !reason) reason = 8;
+        continue;
+      }
       if (flex) {
         B(Vj(loc), c->Z.p + (long long)loc * n);
         A(c->Z.p + (long long)loc * n, Vj(loc + 1));
@@ -606,7 +714,10 @@ void krylov_impl(prbf_ctx* c, const prbf_solve_opts& o, const double* b, double*
       if (hapend && !reason) reason = 8;
     }
     int it = loc - 1;
-    if (it >= 0) {
+    if (it >= 0 && graph) {
+      arnoldi_backsub(c->st, it, m, c->kst, dsc + 128);
+      update_x(c->st, n, x, flex ? c->Z.p : V, n, it + 1, dsc + 128);
+    } else if (it >= 0) {
       nrs[it] = rs[it] / HH(it, it);
       for (int ii = 1; ii <= it; ++ii) {
         int k = it - ii;
@@ -682,6 +793,7 @@ void prbf_solve_opts_default(prbf_solve_opts* o) {
   o->rtol = 1e-10;
   o->abstol = 1e-50;
   o->dtol = 1e5;
+  o->graph = 1;
 }
 
 prbf_status prbf_create(prbf_ctx** out, const prbf_params* p) {
@@ -707,6 +819,7 @@ prbf_status prbf_create(prbf_ctx** out, const prbf_params* p) {
 prbf_status prbf_destroy(prbf_ctx* c) {
   if (!c) return PRBF_ERR_ARG;
   if (c->st) (void)hipStreamSynchronize(c->st);
+  c->drop_graphs();
   if (c->pin) (void)hipHostFree(c->pin);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);

@@ -82,5 +82,12 @@ void update_x(hipStream_t st, long long n, double* x, const double* V, long long
 void gather(hipStream_t st, long long n, const int* perm, const double* src, double* dst);   // dst[s] = src[perm[s]]
 void scatter(hipStream_t st, long long n, const int* perm, const double* src, double* dst);  // dst[perm[s]] = src[s]
 void fill(hipStream_t st, long long n, double* v, double a);
+// Device-side Arnoldi bookkeeping of one GMRES cycle (state layout in krylov.hip).
+long long arnoldi_state_len(int m);
+// H(:, j) from the dots dsc[0..j] and |w|^2 = dsc[64], Givens rotations, residual estimate
+void arnoldi_step(hipStream_t st, int j, int m, const double* dsc, double* ks);
+void arnoldi_backsub(hipStream_t st, int it, int m, const double* ks, double* y);
+void scale_dev(hipStream_t st, long long n, double* v, const double* s);  // v *= *s
+void set_scalar(hipStream_t st, double* p, double a);
 
 }  // namespace prbf

@@ -176,4 +176,95 @@ void fill(hipStream_t st, long long n, double* v, double a) {
   hipLaunchKernelGGL(k_fill, g1(n), dim3(256), 0, st, n, v, a);
 }
 
+
+// ---- device-side Arnoldi bookkeeping (for iterations captured as graphs) --
+// State of one restart cycle, column-major H with leading dimension m + 1:
+//   ks[0 .. (m+1)^2)            H
+//   ks[kc .. kc+m+1)            cosines, kc = (m+1)^2
+//   ks[kc+m+1 .. kc+2m+2)       sines
+//   ks[kc+2m+2 .. kc+3m+4)      rs (rotated right-hand side)
+//   ks[kc+3m+4]                 1 / h_{j+1,j} (1 if zero) for the next basis vector
+//   ks[kc+3m+5 .. kc+3m+8)      out: residual estimate, happy flag, breakdown flag
+// The arithmetic is the host loop's, operation for operation and without
+// contraction into fma, so both paths give the same bits.
+long long arnoldi_state_len(int m) { return (long long)(m + 1) * (m + 1) + 3 * m + 8; }
+
+__global__ void k_arnoldi_step(int j, int m, const double* __restrict__ dsc, double* __restrict__ ks) {
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int ld = m + 1, kc = ld * ld;
+  double* H = ks;
+  double* cs = ks + kc;
+  double* sn = cs + m + 1;
+  double* rs = sn + m + 1;
+  double* inv = rs + m + 2;
+  double* out = inv + 1;
+  for (int i = 0; i <= j; ++i) H[j * ld + i] = dsc[i];
+  const double tt = sqrt(dsc[64]);
+  *inv = tt != 0.0 ? 1.0 / tt : 1.0;
+  H[j * ld + j + 1] = tt;
+  const double q = fabs(tt / rs[j]);
+  const double hapbnd = 1e-30 < q ? 1e-30 : q;  // std::min(q, 1e-30) of the host loop
+  const bool hapend = tt < hapbnd;
+  for (int i = 1; i <= j; ++i) {
+    const double a0 = H[j * ld + i - 1];
+    H[j * ld + i - 1] = cs[i - 1] * a0 + sn[i - 1] * H[j * ld + i];
+    H[j * ld + i] = cs[i - 1] * H[j * ld + i] - sn[i - 1] * a0;
+  }
+  double res = 0.0, breakdown = 0.0;
+  if (!hapend) {
+    const double a0 = H[j * ld + j], a1 = H[j * ld + j + 1];
+    const double t = sqrt(a0 * a0 + a1 * a1);
+    if (t == 0.0) {
+      breakdown = 1.0;
+    } else {
+      cs[j] = a0 / t;
+      sn[j] = a1 / t;
+      rs[j + 1] = -(sn[j] * rs[j]);
+      rs[j] = cs[j] * rs[j];
+      H[j * ld + j] = cs[j] * a0 + sn[j] * a1;
+      res = fabs(rs[j + 1]);
+    }
+  }
+  out[0] = res;
+  out[1] = hapend ? 1.0 : 0.0;
+  out[2] = breakdown;
+}
+
+// y = H(0:it, 0:it)^{-1} rs(0:it) by back substitution (upper triangular)
+__global__ void k_arnoldi_backsub(int it, int m, const double* __restrict__ ks, double* __restrict__ y) {
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int ld = m + 1, kc = ld * ld;
+  const double* H = ks;
+  const double* rs = ks + kc + 2 * (m + 1);
+  y[it] = rs[it] / H[it * ld + it];
+  for (int ii = 1; ii <= it; ++ii) {
+    const int k = it - ii;
+    double t = rs[k];
+    for (int j = k + 1; j <= it; ++j) t = t - H[j * ld + k] * y[j];
+    y[k] = t / H[k * ld + k];
+  }
+}
+
+__global__ void k_scale_dev(long long n, double* __restrict__ v, const double* __restrict__ s) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] *= *s;
+}
+
+__global__ void k_set1(double* __restrict__ p, double a) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = a;
+}
+
+void arnoldi_step(hipStream_t st, int j, int m, const double* dsc, double* ks) {
+  hipLaunchKernelGGL(k_arnoldi_step, dim3(1), dim3(64), 0, st, j, m, dsc, ks);
+}
+void arnoldi_backsub(hipStream_t st, int it, int m, const double* ks, double* y) {
+  hipLaunchKernelGGL(k_arnoldi_backsub, dim3(1), dim3(64), 0, st, it, m, ks, y);
+}
+void scale_dev(hipStream_t st, long long n, double* v, const double* s) {
+  hipLaunchKernelGGL(k_scale_dev, g1(n), dim3(256), 0, st, n, v, s);
+}
+void set_scalar(hipStream_t st, double* p, double a) { hipLaunchKernelGGL(k_set1, dim3(1), dim3(64), 0, st, p, a); }
+
 }  // namespace prbf

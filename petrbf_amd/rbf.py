@@ -217,7 +217,7 @@ class Context:
         return M  # (T, n): rows of A_c^{-1} for the core members
 
     # ---- solve
-    def _opts(self, ksp, pc, rtol, restart, max_it):
+    def _opts(self, ksp, pc, rtol, restart, max_it, graph=True):
         o = L.PrbfSolveOpts()
         self._L.prbf_solve_opts_default(C.byref(o))
         o.ksp = _KSP[ksp]
@@ -225,6 +225,7 @@ class Context:
         o.rtol = float(rtol)
         o.restart = int(restart)
         o.max_it = int(max_it)
+        o.graph = 1 if graph else 0
         return o
 
     def _info(self, inf, hist):
@@ -232,9 +233,9 @@ class Context:
         return SolveInfo(inf.its, inf.reason, inf.rnorm0, inf.rnorm, inf.t_setup_s, inf.t_solve_s,
                          inf.t_matvec_s, inf.t_pc_s, inf.n_matvec, inf.n_pc, inf.n_fallback, hist[:n])
 
-    def solve(self, w, ksp="gmres", pc="rasm", rtol=1e-10, restart=30, max_it=1000):
+    def solve(self, w, ksp="gmres", pc="rasm", rtol=1e-10, restart=30, max_it=1000, graph=True):
         """Strengths gamma (caller's order) with A~ gamma = w."""
-        o = self._opts(ksp, pc, rtol, restart, max_it)
+        o = self._opts(ksp, pc, rtol, restart, max_it, graph)
         w = _f64(w)
         g = np.zeros(self.ni)
         hist = np.zeros(max_it + 64)
@@ -244,8 +245,8 @@ class Context:
         info.hist = info.hist[: self._hist_len(info)]
         return g, info
 
-    def solve_dev(self, w, gamma, ksp="gmres", pc="rasm", rtol=1e-10, restart=30, max_it=1000):
-        o = self._opts(ksp, pc, rtol, restart, max_it)
+    def solve_dev(self, w, gamma, ksp="gmres", pc="rasm", rtol=1e-10, restart=30, max_it=1000, graph=True):
+        o = self._opts(ksp, pc, rtol, restart, max_it, graph)
         hist = np.zeros(max_it + 64)
         inf = L.PrbfSolveInfo()
         self._check(self._L.prbf_solve_dev(self._h, C.byref(o), _p(w), _p(gamma), C.byref(inf), _p(hist),

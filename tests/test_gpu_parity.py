@@ -257,3 +257,28 @@ def test_block_factorisation_launch_configs_agree(prec):
     tol = 1e-9 if prec == "rasm" else 1e-2
     for o in outs[1:]:
         assert np.linalg.norm(o - outs[0]) <= tol * np.linalg.norm(outs[0])
+
+
+@pytest.mark.parametrize("ksp,pc", [("gmres", "rasm"), ("fgmres", "rasm32"), ("fgmres", "none")])
+def test_graph_captured_iterations_match_kernel_by_kernel(ksp, pc):
+    """Arnoldi steps replayed as captured HIP graphs give the same bits as the
+    same kernels launched one by one, over several restart cycles and after
+    the context's buffers are rebuilt for a different problem size."""
+    _announce(f"graph vs eager {ksp}/{pc}")
+    p = lattice(60, seed=0x5EED003C)
+    part = p.x < 0.4  # a smaller point set with the same parameters
+    ctx = Context.from_problem(p)
+    for sel, restart, max_it in ((part, 8, 60), (None, 30, 1000), (part, 30, 1000)):
+        x, y, w = (p.x, p.y, p.w[0]) if sel is None else (p.x[sel], p.y[sel], p.w[0][sel])
+        ctx.set_points(x, y)
+        if pc == "none":
+            max_it = 40
+        g0, i0 = ctx.solve(w, ksp=ksp, pc=pc, rtol=1e-10, restart=restart, max_it=max_it, graph=False)
+        for rep in range(2):
+            g1, i1 = ctx.solve(w, ksp=ksp, pc=pc, rtol=1e-10, restart=restart, max_it=max_it, graph=True)
+            assert (i1.its, i1.reason) == (i0.its, i0.reason)
+            assert np.array_equal(i1.hist, i0.hist)
+            assert np.array_equal(g1, g0)
+        if pc != "none" and restart == 30:
+            assert i0.reason == 2
+    ctx.close()
