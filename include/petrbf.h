@@ -58,7 +58,9 @@ typedef enum { PRBF_KSP_GMRES = 0, PRBF_KSP_FGMRES = 1 } prbf_ksp;
 /* RASM_FP64: exact FP64 block solves (the reference's -sub_pc_type lu).
  * RASM_FP32: the same blocks factorised and solved in FP32 on the matrix
  * cores, applied with FP64 accumulation; meant for FGMRES. */
-typedef enum { PRBF_PC_NONE = 0, PRBF_PC_RASM_FP64 = 1, PRBF_PC_RASM_FP32 = 2 } prbf_pc;
+/* LEARNED: the network of prbf_learned_attach() on the clusters it accepted,
+ * exact FP32 blocks on the others; FGMRES only. */
+typedef enum { PRBF_PC_NONE = 0, PRBF_PC_RASM_FP64 = 1, PRBF_PC_RASM_FP32 = 2, PRBF_PC_LEARNED = 3 } prbf_pc;
 typedef enum { PRBF_TUNE_CHOL_WAVES = 1, PRBF_TUNE_CHOL_PANEL = 2 } prbf_tune_key;
 
 /* Solver options; prbf_solve_opts_default() gives the reference's run line
@@ -89,7 +91,9 @@ typedef struct {
   double t_pc_s;     /* summed device time of the preconditioner applies (0 with graph = 1) */
   int n_matvec;
   int n_pc;
-  int n_fallback;    /* RASM_FP32: blocks redone in FP64 after an FP32 breakdown */
+  int n_fallback;    /* RASM_FP32 / LEARNED: blocks redone in FP64 after an FP32 breakdown */
+  int n_network;     /* LEARNED: clusters served by the network at the end of the solve */
+  int n_net_fallback; /* LEARNED: clusters the solve moved from the network to exact blocks */
 } prbf_solve_info;
 
 /* ---- lifetime -------------------------------------------------------- */
@@ -172,6 +176,51 @@ prbf_status prbf_solve(prbf_ctx* ctx, const prbf_solve_opts* opts, const double*
                        prbf_solve_info* info, double* hist, int hist_cap);
 prbf_status prbf_solve_dev(prbf_ctx* ctx, const prbf_solve_opts* opts, const double* w, double* gamma,
                            prbf_solve_info* info, double* hist, int hist_cap);
+
+/* ---- learned preconditioner (2D; no counterpart in the reference) --------
+ * A small MLP (192 -> 256 -> 256 -> 256 -> 16, GELU, BF16 weights and
+ * activations on the BF16 matrix cores, FP32 accumulation) maps the residual
+ * on the 8x8 lattice-slot patch around a cluster of 4x4 lattice spacings
+ * (scaled by its RMS) and the particles' offsets from their slots to a
+ * correction on the cluster's 16 core particles.  The network is trained on
+ * the device (Adam, FP32 master weights) on pairs (patch, exact RASM core
+ * solve) harvested from FGMRES runs with the exact preconditioner.  Device
+ * arrays; these calls synchronise. */
+typedef struct prbf_mlp prbf_mlp;
+int64_t prbf_mlp_num_params(void);
+prbf_status prbf_mlp_create(prbf_mlp** net, uint64_t seed);
+prbf_status prbf_mlp_destroy(prbf_mlp* net);
+const char* prbf_mlp_last_error(const prbf_mlp* net);
+/* params: host, prbf_mlp_num_params() floats (W0, b0, ..., W3, b3; W_l row-major in x out) */
+prbf_status prbf_mlp_set_params(prbf_mlp* net, const float* params, double out_scale);
+prbf_status prbf_mlp_get_params(prbf_mlp* net, float* params, double* out_scale);
+/* Y (m x 16) = net(X (m x 192)), row-major device arrays */
+prbf_status prbf_mlp_forward_dev(prbf_mlp* net, int64_t m, const float* X, float* Y);
+/* mean-square loss against T (m x 16) and its gradient (device, num_params floats); m % 16 == 0 */
+prbf_status prbf_mlp_loss_grad_dev(prbf_mlp* net, int64_t m, const float* X, const float* T, float* loss,
+                                   float* grad);
+/* `steps` Adam steps on minibatches of `batch` pairs drawn (seeded) from the
+ * npairs rows of X, T; loss_hist (host, steps floats, nullable) */
+prbf_status prbf_mlp_train_dev(prbf_mlp* net, int64_t npairs, const float* X, const float* T, int batch, int steps,
+                               float lr, uint64_t seed, float* loss_hist);
+/* number of clusters with a complete patch (the network's candidates) */
+prbf_status prbf_learned_eligible(prbf_ctx* ctx, int64_t* n);
+/* For the sorted vector r: features X (n x 192) of the eligible clusters, the
+ * patch RMS s (n), and with z != NULL the targets T = z_core / (s out_scale)
+ * (n x 16); at most cap rows. */
+prbf_status prbf_learned_pairs_dev(prbf_ctx* ctx, const double* r, const double* z, double out_scale, float* X,
+                                   float* T, double* s, int64_t cap, int64_t* n);
+/* rho = |r - A z| / |r| on each eligible cluster's patch for the core values z */
+prbf_status prbf_learned_rho_dev(prbf_ctx* ctx, const double* r, const double* z, double* rho, int64_t cap,
+                                 int64_t* n);
+/* Attach a trained network: an eligible cluster uses it when its patch
+ * residual reduction rho stays <= tau on two seeded probe residuals; every
+ * other cluster falls back to its exact block, factorised in FP32 here.
+ * n_network receives the number of clusters on the network. */
+prbf_status prbf_learned_attach(prbf_ctx* ctx, prbf_mlp* net, double tau, int64_t* n_network);
+/* Copies basis vector j of the last FGMRES cycle and its preconditioned
+ * image into the device arrays v and z (sorted order; for harvesting pairs). */
+prbf_status prbf_krylov_vectors_dev(prbf_ctx* ctx, int j, double* v, double* z);
 
 /* ---- library entry of the 3D tree ---------------------------------------
  * RBFInterpolation (3d/wrapper.cxx:20-105), for dim 2 or 3: FP32 grid

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpetrbf.so")
 
 PRBF_KSP_GMRES, PRBF_KSP_FGMRES = 0, 1
-PRBF_PC_NONE, PRBF_PC_RASM_FP64, PRBF_PC_RASM_FP32 = 0, 1, 2
+PRBF_PC_NONE, PRBF_PC_RASM_FP64, PRBF_PC_RASM_FP32, PRBF_PC_LEARNED = 0, 1, 2, 3
 PRBF_TUNE_CHOL_WAVES, PRBF_TUNE_CHOL_PANEL = 1, 2
 
 STATUS = {0: "PRBF_OK", 1: "PRBF_ERR_ARG", 2: "PRBF_ERR_HIP", 3: "PRBF_ERR_NOMEM",
@@ -26,6 +26,10 @@ EXPORTS = [
     "prbf_get_sorted_points", "prbf_matvec", "prbf_matvec_dev", "prbf_evaluate", "prbf_evaluate_dev",
     "prbf_rasm_setup", "prbf_rasm_setup_pc", "prbf_set_tuning", "prbf_rasm_apply", "prbf_rasm_apply_dev", "prbf_rasm_get_core_rows",
     "prbf_solve", "prbf_solve_dev", "prbf_rbf_interpolation_f32",
+    "prbf_mlp_num_params", "prbf_mlp_create", "prbf_mlp_destroy", "prbf_mlp_last_error", "prbf_mlp_set_params",
+    "prbf_mlp_get_params", "prbf_mlp_forward_dev", "prbf_mlp_loss_grad_dev", "prbf_mlp_train_dev",
+    "prbf_learned_eligible", "prbf_learned_pairs_dev", "prbf_learned_rho_dev", "prbf_learned_attach",
+    "prbf_krylov_vectors_dev",
 ]
 
 
@@ -42,7 +46,8 @@ class PrbfSolveOpts(C.Structure):
 class PrbfSolveInfo(C.Structure):
     _fields_ = [("its", C.c_int), ("reason", C.c_int), ("rnorm0", C.c_double), ("rnorm", C.c_double),
                 ("t_setup_s", C.c_double), ("t_solve_s", C.c_double), ("t_matvec_s", C.c_double),
-                ("t_pc_s", C.c_double), ("n_matvec", C.c_int), ("n_pc", C.c_int), ("n_fallback", C.c_int)]
+                ("t_pc_s", C.c_double), ("n_matvec", C.c_int), ("n_pc", C.c_int), ("n_fallback", C.c_int),
+                ("n_network", C.c_int), ("n_net_fallback", C.c_int)]
 
 
 class PrbfError(RuntimeError):
@@ -61,6 +66,14 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with `python petrbf_amd/build.py`")
+    # torch-ROCm ships its own HIP runtime under the same soname
+    # (libamdhip64.so.7).  Loaded first, it is the one this library binds to,
+    # so device pointers and streams are shared with torch; loaded after this
+    # library, torch would start a second runtime that finds no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     vp, i32, i64, dp = C.c_void_p, C.c_int, C.c_int64, C.c_void_p
     ctx = C.c_void_p
@@ -92,8 +105,25 @@ def load():
         getattr(L, f).argtypes = [ctx, C.POINTER(PrbfSolveOpts), dp, dp, C.POINTER(PrbfSolveInfo), dp, i32]
     L.prbf_rbf_interpolation_f32.argtypes = [C.POINTER(PrbfParams), i64, dp, dp, dp, dp, i64, dp, dp, dp, dp,
                                              C.POINTER(PrbfSolveOpts), C.POINTER(i32)]
+    net = C.c_void_p
+    L.prbf_mlp_num_params.argtypes = []
+    L.prbf_mlp_create.argtypes = [C.POINTER(net), C.c_uint64]
+    L.prbf_mlp_destroy.argtypes = [net]
+    L.prbf_mlp_last_error.restype = C.c_char_p
+    L.prbf_mlp_last_error.argtypes = [net]
+    L.prbf_mlp_set_params.argtypes = [net, dp, C.c_double]
+    L.prbf_mlp_get_params.argtypes = [net, dp, C.POINTER(C.c_double)]
+    L.prbf_mlp_forward_dev.argtypes = [net, i64, dp, dp]
+    L.prbf_mlp_loss_grad_dev.argtypes = [net, i64, dp, dp, C.POINTER(C.c_float), dp]
+    L.prbf_mlp_train_dev.argtypes = [net, i64, dp, dp, i32, i32, C.c_float, C.c_uint64, dp]
+    L.prbf_learned_eligible.argtypes = [ctx, C.POINTER(i64)]
+    L.prbf_learned_pairs_dev.argtypes = [ctx, dp, dp, C.c_double, dp, dp, dp, i64, C.POINTER(i64)]
+    L.prbf_learned_rho_dev.argtypes = [ctx, dp, dp, dp, i64, C.POINTER(i64)]
+    L.prbf_learned_attach.argtypes = [ctx, net, C.c_double, C.POINTER(i64)]
+    L.prbf_krylov_vectors_dev.argtypes = [ctx, i32, dp, dp]
     for f in EXPORTS:
-        if f not in ("prbf_solve_opts_default", "prbf_last_error"):
+        if f not in ("prbf_solve_opts_default", "prbf_last_error", "prbf_mlp_last_error"):
             getattr(L, f).restype = C.c_int
+    L.prbf_mlp_num_params.restype = C.c_int64
     _lib = L
     return L

@@ -53,6 +53,44 @@ double now_s() {
 
 }  // namespace
 
+// The learned preconditioner's network: FP32 master parameters with their
+// gradient and Adam moments, BF16 weight copies in both orientations, and the
+// activations of the largest batch run so far.
+struct prbf_mlp {
+  hipStream_t st = nullptr;
+  std::string err;
+  DBuf<float> P, G, m1, m2;
+  DBuf<unsigned short> Wb, Wt;
+  double out_scale = 1.0;
+  int t = 0;  // Adam steps taken
+  long long cap = 0;
+  DBuf<unsigned short> Xb, act, dYb, dHb;
+  DBuf<float> pre, Y, Tb, dYf, dHf, part, loss, lhist;
+  DBuf<long long> idx;
+  void ensure(long long m) {
+    if (m <= cap) return;
+    Xb.ensure((size_t)m * kMlpIn);
+    act.ensure((size_t)3 * m * kMlpHidden);
+    pre.ensure((size_t)3 * m * kMlpHidden);
+    Y.ensure((size_t)m * kMlpOut);
+    Tb.ensure((size_t)m * kMlpOut);
+    dYf.ensure((size_t)m * kMlpOut);
+    dYb.ensure((size_t)m * kMlpOut);
+    dHf.ensure((size_t)2 * m * kMlpHidden);
+    dHb.ensure((size_t)2 * m * kMlpHidden);
+    part.ensure(256);
+    loss.ensure(1);
+    idx.ensure((size_t)m);
+    cap = m;
+  }
+  void layers(float* pr[3], unsigned short* ac[3]) {
+    for (int l = 0; l < 3; ++l) {
+      pr[l] = pre.p + (long long)l * cap * kMlpHidden;
+      ac[l] = act.p + (long long)l * cap * kMlpHidden;
+    }
+  }
+};
+
 struct prbf_ctx {
   prbf_params prm{};
   Grid g{};
@@ -94,6 +132,21 @@ struct prbf_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // Arnoldi steps captured as graphs: device state of the cycle, one
   // executable graph per step index, and the launch arguments they captured
+  int last_restart = 0;  // basis size of the last Krylov solve
+  // learned preconditioner: patches of the eligible clusters, the attached
+  // network, the clusters it serves (skip[c] = 1) and inference buffers
+  bool patches_ready = false, learned_ready = false;
+  DBuf<int> pidx, cidx, elig, mlp_list;
+  DBuf<double> pctr;
+  DBuf<unsigned char> pok, skip;
+  long long n_elig = 0, n_mlp = 0;
+  int n_net_fallback = 0;
+  std::vector<unsigned char> h_skip;
+  std::vector<int> h_mlp_list;
+  DBuf<double> lnorm;
+  prbf_mlp* net = nullptr;
+  DBuf<float> lX;
+  DBuf<double> ls;
   DBuf<double> kst;
   std::vector<hipGraphExec_t> kgraphs;
   std::vector<unsigned long long> kgraph_key;
@@ -217,6 +270,8 @@ void set_points_impl(prbf_ctx* c, long long ni, const double* xi, const double* 
   }
   c->has_points = false;
   c->rasm_ready = false;
+  c->patches_ready = false;
+  c->learned_ready = false;
   double lo[3], hi[3], lo2[3], hi2[3];
   bbox_of(c, ni, xi, yi, zi, lo, hi);
   if (!same) {
@@ -379,8 +434,12 @@ void factor_blocks(prbf_ctx* c, bool fp32, const std::vector<int>& order, int nm
 void rasm_setup_impl(prbf_ctx* c, int pc) {
   need(c->has_points, PRBF_ERR_STATE, "set_points first");
   need(c->same, PRBF_ERR_STATE, "RASM needs the source set equal to the target set");
-  need(pc == PRBF_PC_RASM_FP64 || pc == PRBF_PC_RASM_FP32, PRBF_ERR_ARG, "unknown preconditioner");
-  const bool fp32 = (pc == PRBF_PC_RASM_FP32);
+  need(pc == PRBF_PC_RASM_FP64 || pc == PRBF_PC_RASM_FP32 || pc == PRBF_PC_LEARNED, PRBF_ERR_ARG,
+       "unknown preconditioner");
+  const bool fp32 = (pc != PRBF_PC_RASM_FP64);
+  // with the learned preconditioner only the clusters off the network get blocks
+  const bool learned = (pc == PRBF_PC_LEARNED);
+  need(!learned || c->h_skip.size() == (size_t)c->g.ncell, PRBF_ERR_STATE, "prbf_learned_attach first");
   const Grid& g = c->g;
   const long long nc = g.ncell;
   c->h_mptr.assign(nc + 1, 0);
@@ -389,8 +448,9 @@ void rasm_setup_impl(prbf_ctx* c, int pc) {
   for (long long k = 0; k < nc; ++k) {
     long long n = c->h_bptr[k + 1] - c->h_bptr[k];
     long long T = n ? (c->h_iend[k] - c->h_ista[k] + 1) : 0;
+    // (space is kept for network clusters too, so a later fallback can add their blocks)
     c->h_mptr[k + 1] = c->h_mptr[k] + apply_elems((int)n, (int)T);
-    if (n) order.push_back((int)k);
+    if (n && !(learned && c->h_skip[k])) order.push_back((int)k);
   }
   // largest blocks first: the persistent workgroups then finish together
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
@@ -452,10 +512,31 @@ void rasm_setup_impl(prbf_ctx* c, int pc) {
   c->rasm_pc = pc;
 }
 
+// The network's corrections on the clusters in `list` for the sorted vector r.
+void network_apply(prbf_ctx* c, const int* list, long long n, const double* r, double* z) {
+  if (n <= 0) return;
+  prbf_mlp* net = c->net;
+  const double hl = c->g.box / 4.0;
+  patch_features(c->st, n, list, hl, c->prm.sigma, c->pidx, c->cidx, c->pctr, c->xi, c->yi, r, nullptr,
+                 net->out_scale, c->lX, nullptr, c->ls);
+  mlp_to_bf16(c->st, n * kMlpIn, c->lX, net->Xb);
+  float* pr[3];
+  unsigned short* ac[3];
+  net->layers(pr, ac);
+  mlp_forward(c->st, (int)n, net->P, net->Wt, net->Xb, pr, ac, net->Y);
+  core_scatter(c->st, n, list, c->cidx, net->Y, c->ls, net->out_scale, z);
+}
+
 void rasm_apply_impl(prbf_ctx* c, const double* r, double* z) {
   need(c->rasm_ready, PRBF_ERR_STATE, "rasm_setup first");
-  if (c->rasm_pc == PRBF_PC_RASM_FP32)
-    rasm_apply_fp32(c->st, c->g.ncell, c->bptr, c->bidx, c->ista, c->iend, c->mptr, c->M32, c->fb, c->M64fb.p, r, z);
+  if (c->rasm_pc == PRBF_PC_LEARNED) {
+    need(c->learned_ready && c->net, PRBF_ERR_STATE, "prbf_learned_attach first");
+    rasm_apply_fp32(c->st, c->g.ncell, c->bptr, c->bidx, c->ista, c->iend, c->mptr, c->M32, c->fb, c->M64fb.p,
+                    c->skip, r, z);
+    network_apply(c, c->mlp_list, c->n_mlp, r, z);
+  } else if (c->rasm_pc == PRBF_PC_RASM_FP32)
+    rasm_apply_fp32(c->st, c->g.ncell, c->bptr, c->bidx, c->ista, c->iend, c->mptr, c->M32, c->fb, c->M64fb.p,
+                    nullptr, r, z);
   else
     rasm_apply_fp64(c->st, c->g.ncell, c->bptr, c->bidx, c->ista, c->iend, c->mptr, c->M, r, z, c->max_core);
   PRBF_HIP(hipGetLastError());
@@ -490,6 +571,36 @@ struct Timed {
   }
 };
 
+// Moves clusters from the network to exact FP32 blocks (the learned
+// preconditioner's per-cluster fallback) and factorises their blocks.
+void learned_fallback(prbf_ctx* c, const std::vector<int>& newc) {
+  if (newc.empty()) return;
+  for (int k : newc) c->h_skip[k] = 0;
+  std::vector<int> list;
+  for (long long k = 0; k < c->g.ncell; ++k)
+    if (c->h_skip[k]) list.push_back((int)k);
+  c->n_mlp = (long long)list.size();
+  if (!list.empty())
+    PRBF_HIP(hipMemcpyAsync(c->mlp_list, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice, c->st));
+  PRBF_HIP(hipMemcpyAsync(c->skip, c->h_skip.data(), c->g.ncell, hipMemcpyHostToDevice, c->st));
+  std::vector<int> order(newc);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return c->h_bptr[a + 1] - c->h_bptr[a] > c->h_bptr[b + 1] - c->h_bptr[b];
+  });
+  DBuf<int> status;
+  status.ensure(c->g.ncell);
+  PRBF_HIP(hipMemsetAsync(status, 0, sizeof(int) * c->g.ncell, c->st));
+  factor_blocks<float>(c, true, order, c->max_buffer, c->max_core, c->mptr, c->M32.p, status, true);
+  std::vector<int> h(c->g.ncell);
+  PRBF_HIP(hipMemcpy(h.data(), status, sizeof(int) * c->g.ncell, hipMemcpyDeviceToHost));
+  bool failed = false;
+  for (int k : newc) failed |= h[k] != 0;
+  c->n_net_fallback += (int)newc.size();
+  c->h_mlp_list = list;
+  if (failed) rasm_setup_impl(c, PRBF_PC_LEARNED);  // an FP32 breakdown: the full setup adds FP64 blocks
+  sync(c);
+}
+
 // Preconditioner application without timing (usable under graph capture).
 void pc_apply_raw(prbf_ctx* c, int pc, long long n, const double* v, double* y) {
   if (pc == PRBF_PC_NONE)
@@ -503,6 +614,7 @@ inline long long ks_rs(int m) { return (long long)(m + 1) * (m + 1) + 2 * (m + 1
 inline long long ks_inv(int m) { return ks_rs(m) + m + 2; }
 inline long long ks_out(int m) { return ks_inv(m) + 1; }
 constexpr int kPinStep = 256;  // pinned slots receiving a step's residual estimate and flags
+constexpr double kNetCycleDrop = 0.1;  // least core-residual reduction per cycle on the network
 
 // Enqueues Arnoldi step j of a cycle: the new basis vector, its Gram-Schmidt
 // coefficients and norm, the Givens update of H and the residual estimate,
@@ -539,7 +651,15 @@ void ensure_graphs(prbf_ctx* c, int pc, bool flex, long long n, int m) {
       (unsigned long long)c->nj, (unsigned long long)c->prm.dim, sig, u(c->V.p), u(c->Z.p), u(c->v1.p),
       u(c->part.p), u(c->scal.p), u(c->kst.p), u(c->pin), u(c->ista.p), u(c->iend.p), u(c->tptr.p), u(c->tidx.p),
       u(c->xi.p), u(c->yi.p), u(c->zi.p), u(c->xj_()), u(c->yj_()), u(c->zj_()), u(c->bptr.p), u(c->bidx.p),
-      u(c->mptr.p), u(c->M.p), u(c->M32.p), u(c->fb.p), u(c->M64fb.p)};
+      u(c->mptr.p), u(c->M.p), u(c->M32.p), u(c->fb.p), u(c->M64fb.p), u(c->skip.p), u(c->mlp_list.p),
+      (unsigned long long)c->n_mlp, u(c->pidx.p), u(c->cidx.p), u(c->pctr.p), u(c->lX.p), u(c->ls.p), u(c->net)};
+  if (c->net) {
+    unsigned long long os;
+    std::memcpy(&os, &c->net->out_scale, sizeof(os));
+    for (unsigned long long v : {os, u(c->net->P.p), u(c->net->Wt.p), u(c->net->Xb.p), u(c->net->act.p),
+                                 u(c->net->pre.p), u(c->net->Y.p), (unsigned long long)c->net->cap})
+      key.push_back(v);
+  }
   if (key == c->kgraph_key && (int)c->kgraphs.size() == m) return;
   c->drop_graphs();
   for (int j = 0; j < m; ++j) {
@@ -573,7 +693,9 @@ void krylov_impl(prbf_ctx* c, const prbf_solve_opts& o, const double* b, double*
   need(m >= 1 && m < 63, PRBF_ERR_ARG, "restart must be in 1..62");
   need(o.pc == PRBF_PC_NONE || c->rasm_ready, PRBF_ERR_STATE, "rasm_setup first");
   const bool flex = (o.ksp == PRBF_KSP_FGMRES);
+  need(flex || o.pc != PRBF_PC_LEARNED, PRBF_ERR_ARG, "the learned preconditioner needs FGMRES");
   const bool graph = o.graph != 0 && c->st != nullptr;  // the legacy default stream cannot be captured
+  c->last_restart = m;
   c->V.ensure((size_t)(m + 1) * n);
   if (flex) c->Z.ensure((size_t)m * n);
   c->v1.ensure(n);
@@ -620,6 +742,7 @@ void krylov_impl(prbf_ctx* c, const prbf_solve_opts& o, const double* b, double*
   };
   const double t0 = now_s();
   fill(c->st, n, x, 0.0);
+  std::vector<double> prev_core;  // network clusters' core residuals at the last cycle start
   bool guess_zero = true;
   int its = 0, itcount = 0, reason = 0;
   while (!reason) {
@@ -629,6 +752,28 @@ void krylov_impl(prbf_ctx* c, const prbf_solve_opts& o, const double* b, double*
     } else {
       A(x, tmp2);
       xpay_sub(c->st, n, b, tmp2, tmp);
+    }
+    if (o.pc == PRBF_PC_LEARNED && c->n_mlp > 0) {
+      // FGMRES's residual judges the network per cluster: a cluster whose core
+      // residual fell by less than kNetCycleDrop over the last cycle moves to
+      // its exact block
+      const long long nm = c->n_mlp;
+      std::vector<double> cur(nm);
+      core_norms(c->st, nm, c->mlp_list, c->cidx, tmp, c->lnorm);
+      PRBF_HIP(hipMemcpyAsync(cur.data(), c->lnorm, sizeof(double) * nm, hipMemcpyDeviceToHost, c->st));
+      sync(c);
+      std::vector<int> newc;
+      if (!prev_core.empty())
+        for (long long k = 0; k < nm; ++k) {
+          const int cl = c->h_mlp_list[k];
+          if (!(cur[k] <= kNetCycleDrop * kNetCycleDrop * prev_core[cl])) newc.push_back(cl);
+        }
+      prev_core.assign(c->g.ncell, 0.0);
+      for (long long k = 0; k < nm; ++k) prev_core[c->h_mlp_list[k]] = cur[k];
+      if (!newc.empty()) {
+        learned_fallback(c, newc);
+        if (graph) ensure_graphs(c, o.pc, flex, n, m);
+      }
     }
     if (flex) {
       PRBF_HIP(hipMemcpyAsync(Vj(0), tmp, sizeof(double) * n, hipMemcpyDeviceToDevice, c->st));
@@ -753,6 +898,7 @@ void solve_dev_impl(prbf_ctx* c, const prbf_solve_opts* opts, const double* w, d
   prbf_solve_opts_default(&o);
   if (opts) o = *opts;
   double t_setup = 0.0;
+  need(o.pc != PRBF_PC_LEARNED || c->learned_ready, PRBF_ERR_STATE, "prbf_learned_attach first");
   if (o.pc != PRBF_PC_NONE && (!c->rasm_ready || c->rasm_pc != o.pc)) {
     double t0 = now_s();
     rasm_setup_impl(c, o.pc);
@@ -776,6 +922,10 @@ void solve_dev_impl(prbf_ctx* c, const prbf_solve_opts* opts, const double* w, d
   scatter(c->st, n, c->perm_i, xs, gamma);
   local.t_setup_s = t_setup;
   local.n_fallback = c->n_fallback;
+  if (o.pc == PRBF_PC_LEARNED) {
+    local.n_network = (int)c->n_mlp;
+    local.n_net_fallback = c->n_net_fallback;
+  }
   if (info) *info = local;
 }
 
@@ -1117,6 +1267,319 @@ prbf_status prbf_rbf_interpolation_f32(const prbf_params* p, int64_t ng, const f
   }
   prbf_destroy(c);
   return st;
+}
+
+// ------------------------------------------------------------ learned preconditioner
+
+}  // extern "C"
+
+namespace {
+
+template <class F>
+prbf_status guarded_mlp(prbf_mlp* net, F&& f) {
+  if (!net) return PRBF_ERR_ARG;
+  try {
+    f();
+    net->err.clear();
+    return PRBF_OK;
+  } catch (const ApiError& e) {
+    net->err = e.what;
+    return (prbf_status)e.code;
+  } catch (const HipError& e) {
+    net->err = e.what;
+    return PRBF_ERR_HIP;
+  }
+}
+
+void mlp_sync(prbf_mlp* net) { PRBF_HIP(hipStreamSynchronize(net->st)); }
+
+void mlp_refresh_bf16(prbf_mlp* net) { mlp_weights_bf16(net->st, net->P, net->Wb, net->Wt); }
+
+// Patch slots of every cluster and the list of eligible clusters.
+void ensure_patches(prbf_ctx* c) {
+  need(c->has_points && c->same, PRBF_ERR_STATE, "set_points (one set) first");
+  need(c->prm.dim == 2, PRBF_ERR_ARG, "the learned preconditioner is two-dimensional");
+  if (c->patches_ready) return;
+  const long long nc = c->g.ncell;
+  c->pidx.ensure((size_t)nc * kPatch);
+  c->cidx.ensure((size_t)nc * kMlpOut);
+  c->pok.ensure((size_t)nc);
+  c->pctr.ensure((size_t)2 * nc);
+  patch_build(c->st, c->g, c->g.box / 4.0, c->ista, c->iend, c->xi, c->yi, c->pidx, c->cidx, c->pctr, c->pok);
+  PRBF_HIP(hipGetLastError());
+  std::vector<unsigned char> h(nc);
+  PRBF_HIP(hipMemcpyAsync(h.data(), c->pok, nc, hipMemcpyDeviceToHost, c->st));
+  sync(c);
+  std::vector<int> e;
+  for (long long k = 0; k < nc; ++k)
+    if (h[k]) e.push_back((int)k);
+  c->n_elig = (long long)e.size();
+  c->elig.ensure(std::max<size_t>(1, e.size()));
+  if (!e.empty())
+    PRBF_HIP(hipMemcpyAsync(c->elig, e.data(), sizeof(int) * e.size(), hipMemcpyHostToDevice, c->st));
+  sync(c);
+  c->patches_ready = true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t prbf_mlp_num_params(void) { return mlp_nparams(); }
+
+prbf_status prbf_mlp_create(prbf_mlp** out, uint64_t seed) {
+  if (!out) return PRBF_ERR_ARG;
+  *out = nullptr;
+  prbf_mlp* net = new prbf_mlp;
+  if (hipStreamCreateWithFlags(&net->st, hipStreamNonBlocking) != hipSuccess) {
+    (void)hipGetLastError();
+    delete net;
+    return PRBF_ERR_HIP;
+  }
+  prbf_status st = guarded_mlp(net, [&] {
+    const long long np = mlp_nparams();
+    // Glorot-uniform weights, zero biases, from a splitmix64 stream
+    std::vector<float> h(np, 0.0f);
+    unsigned long long z = seed;
+    auto next = [&]() {
+      z += 0x9E3779B97F4A7C15ULL;
+      unsigned long long v = z;
+      v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ULL;
+      v = (v ^ (v >> 27)) * 0x94D049BB133111EBULL;
+      v ^= v >> 31;
+      return (double)(v >> 11) * (1.0 / 9007199254740992.0);
+    };
+    for (int l = 0; l < kMlpLayers; ++l) {
+      int K, N;
+      mlp_dims(l, &K, &N);
+      const double a = std::sqrt(6.0 / (K + N));
+      for (long long i = 0; i < (long long)K * N; ++i) h[mlp_w_off(l) + i] = (float)((2.0 * next() - 1.0) * a);
+    }
+    net->P.ensure(np);
+    net->G.ensure(np);
+    net->m1.ensure(np);
+    net->m2.ensure(np);
+    net->Wb.ensure(np);
+    net->Wt.ensure(np);
+    PRBF_HIP(hipMemcpyAsync(net->P, h.data(), sizeof(float) * np, hipMemcpyHostToDevice, net->st));
+    PRBF_HIP(hipMemsetAsync(net->m1, 0, sizeof(float) * np, net->st));
+    PRBF_HIP(hipMemsetAsync(net->m2, 0, sizeof(float) * np, net->st));
+    mlp_refresh_bf16(net);
+    mlp_sync(net);
+  });
+  if (st) {
+    prbf_mlp_destroy(net);
+    return st;
+  }
+  *out = net;
+  return PRBF_OK;
+}
+
+prbf_status prbf_mlp_destroy(prbf_mlp* net) {
+  if (!net) return PRBF_ERR_ARG;
+  if (net->st) {
+    (void)hipStreamSynchronize(net->st);
+    (void)hipStreamDestroy(net->st);
+  }
+  delete net;
+  return PRBF_OK;
+}
+
+const char* prbf_mlp_last_error(const prbf_mlp* net) { return net ? net->err.c_str() : ""; }
+
+prbf_status prbf_mlp_set_params(prbf_mlp* net, const float* params, double out_scale) {
+  return guarded_mlp(net, [&] {
+    need(params != nullptr && out_scale > 0, PRBF_ERR_ARG, "params and a positive out_scale required");
+    const long long np = mlp_nparams();
+    PRBF_HIP(hipMemcpyAsync(net->P, params, sizeof(float) * np, hipMemcpyHostToDevice, net->st));
+    PRBF_HIP(hipMemsetAsync(net->m1, 0, sizeof(float) * np, net->st));
+    PRBF_HIP(hipMemsetAsync(net->m2, 0, sizeof(float) * np, net->st));
+    net->t = 0;
+    net->out_scale = out_scale;
+    mlp_refresh_bf16(net);
+    mlp_sync(net);
+  });
+}
+
+prbf_status prbf_mlp_get_params(prbf_mlp* net, float* params, double* out_scale) {
+  return guarded_mlp(net, [&] {
+    if (params) PRBF_HIP(hipMemcpyAsync(params, net->P, sizeof(float) * mlp_nparams(), hipMemcpyDeviceToHost, net->st));
+    if (out_scale) *out_scale = net->out_scale;
+    mlp_sync(net);
+  });
+}
+
+prbf_status prbf_mlp_forward_dev(prbf_mlp* net, int64_t m, const float* X, float* Y) {
+  return guarded_mlp(net, [&] {
+    need(m > 0 && X && Y, PRBF_ERR_ARG, "invalid arguments");
+    net->ensure(m);
+    float* pr[3];
+    unsigned short* ac[3];
+    net->layers(pr, ac);
+    mlp_to_bf16(net->st, m * kMlpIn, X, net->Xb);
+    mlp_forward(net->st, (int)m, net->P, net->Wt, net->Xb, pr, ac, net->Y);
+    PRBF_HIP(hipMemcpyAsync(Y, net->Y, sizeof(float) * m * kMlpOut, hipMemcpyDeviceToDevice, net->st));
+    PRBF_HIP(hipGetLastError());
+    mlp_sync(net);
+  });
+}
+
+prbf_status prbf_mlp_loss_grad_dev(prbf_mlp* net, int64_t m, const float* X, const float* T, float* loss,
+                                   float* grad) {
+  return guarded_mlp(net, [&] {
+    need(m > 0 && m % 16 == 0 && X && T, PRBF_ERR_ARG, "m must be a positive multiple of 16");
+    net->ensure(m);
+    float* pr[3];
+    unsigned short* ac[3];
+    net->layers(pr, ac);
+    mlp_to_bf16(net->st, m * kMlpIn, X, net->Xb);
+    mlp_forward(net->st, (int)m, net->P, net->Wt, net->Xb, pr, ac, net->Y);
+    mlp_backward(net->st, (int)m, net->P, net->Wb, net->Xb, pr, ac, net->Y, T, net->G, net->dYf, net->dYb,
+                 net->dHf, net->dHb, net->part, net->loss);
+    if (grad)
+      PRBF_HIP(hipMemcpyAsync(grad, net->G, sizeof(float) * mlp_nparams(), hipMemcpyDeviceToDevice, net->st));
+    float h = 0.0f;
+    PRBF_HIP(hipMemcpyAsync(&h, net->loss, sizeof(float), hipMemcpyDeviceToHost, net->st));
+    PRBF_HIP(hipGetLastError());
+    mlp_sync(net);
+    if (loss) *loss = h;
+  });
+}
+
+prbf_status prbf_mlp_train_dev(prbf_mlp* net, int64_t npairs, const float* X, const float* T, int batch, int steps,
+                               float lr, uint64_t seed, float* loss_hist) {
+  return guarded_mlp(net, [&] {
+    need(npairs > 0 && X && T && batch > 0 && batch % 16 == 0 && steps >= 0, PRBF_ERR_ARG,
+         "batch must be a positive multiple of 16");
+    net->ensure(batch);
+    net->lhist.ensure(std::max(1, steps));
+    float* pr[3];
+    unsigned short* ac[3];
+    net->layers(pr, ac);
+    const long long np = mlp_nparams();
+    for (int s = 0; s < steps; ++s) {
+      mlp_random_rows(net->st, batch, npairs, seed, s, net->idx);
+      mlp_gather_rows(net->st, batch, net->idx, X, T, net->Xb, net->Tb);
+      mlp_forward(net->st, batch, net->P, net->Wt, net->Xb, pr, ac, net->Y);
+      mlp_backward(net->st, batch, net->P, net->Wb, net->Xb, pr, ac, net->Y, net->Tb, net->G, net->dYf, net->dYb,
+                   net->dHf, net->dHb, net->part, net->loss);
+      copy_f32(net->st, net->loss, net->lhist.p + s);
+      mlp_adam(net->st, np, net->P, net->G, net->m1, net->m2, lr, ++net->t);
+      mlp_refresh_bf16(net);
+    }
+    PRBF_HIP(hipGetLastError());
+    if (loss_hist && steps > 0)
+      PRBF_HIP(hipMemcpyAsync(loss_hist, net->lhist, sizeof(float) * steps, hipMemcpyDeviceToHost, net->st));
+    mlp_sync(net);
+  });
+}
+
+prbf_status prbf_learned_eligible(prbf_ctx* c, int64_t* n) {
+  return guarded(c, [&] {
+    ensure_patches(c);
+    if (n) *n = c->n_elig;
+  });
+}
+
+prbf_status prbf_learned_pairs_dev(prbf_ctx* c, const double* r, const double* z, double out_scale, float* X,
+                                   float* T, double* s, int64_t cap, int64_t* n) {
+  return guarded(c, [&] {
+    need(r && X && s && cap >= 0 && out_scale > 0 && (!z || T), PRBF_ERR_ARG, "invalid arguments");
+    ensure_patches(c);
+    const long long m = std::min<long long>(cap, c->n_elig);
+    patch_features(c->st, m, c->elig, c->g.box / 4.0, c->prm.sigma, c->pidx, c->cidx, c->pctr, c->xi, c->yi, r, z,
+                   out_scale, X, T, s);
+    PRBF_HIP(hipGetLastError());
+    if (n) *n = m;
+  });
+}
+
+prbf_status prbf_learned_rho_dev(prbf_ctx* c, const double* r, const double* z, double* rho, int64_t cap,
+                                 int64_t* n) {
+  return guarded(c, [&] {
+    need(r && z && rho && cap >= 0, PRBF_ERR_ARG, "invalid arguments");
+    ensure_patches(c);
+    const long long m = std::min<long long>(cap, c->n_elig);
+    patch_rho(c->st, m, c->elig, c->pidx, c->cidx, c->xi, c->yi, r, z, c->prm.sigma, rho);
+    PRBF_HIP(hipGetLastError());
+    if (n) *n = m;
+  });
+}
+
+prbf_status prbf_learned_attach(prbf_ctx* c, prbf_mlp* net, double tau, int64_t* n_network) {
+  return guarded(c, [&] {
+    need(net != nullptr && tau >= 0, PRBF_ERR_ARG, "a network and tau >= 0 required");
+    ensure_patches(c);
+    c->learned_ready = false;
+    c->net = net;
+    const long long ne = c->n_elig, n = c->ni, nc = c->g.ncell;
+    std::vector<unsigned char> accept(ne, 1);
+    if (ne > 0) {
+      mlp_sync(net);  // the network's own work is done before the context's stream uses it
+      net->ensure(ne);
+      c->lX.ensure((size_t)ne * kMlpIn);
+      c->ls.ensure((size_t)ne);
+      DBuf<double> g, r, z, rho;
+      g.ensure(n);
+      r.ensure(n);
+      z.ensure(n);
+      rho.ensure(ne);
+      std::vector<double> hg(n), hr(ne);
+      for (int probe = 0; probe < 2; ++probe) {
+        // probe residual r = A g for a seeded g uniform in (-1, 1)
+        unsigned long long st = 0x5EED5EEDULL + probe;
+        for (long long i = 0; i < n; ++i) {
+          st += 0x9E3779B97F4A7C15ULL;
+          unsigned long long v = st;
+          v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ULL;
+          v = (v ^ (v >> 27)) * 0x94D049BB133111EBULL;
+          v ^= v >> 31;
+          hg[i] = 2.0 * ((double)(v >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
+        }
+        PRBF_HIP(hipMemcpyAsync(g, hg.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->st));
+        matvec_impl(c, 1, g, r);
+        fill(c->st, n, z, 0.0);
+        network_apply(c, c->elig, ne, r, z);
+        patch_rho(c->st, ne, c->elig, c->pidx, c->cidx, c->xi, c->yi, r, z, c->prm.sigma, rho);
+        PRBF_HIP(hipMemcpyAsync(hr.data(), rho, sizeof(double) * ne, hipMemcpyDeviceToHost, c->st));
+        sync(c);
+        for (long long k = 0; k < ne; ++k)
+          if (!(hr[k] <= tau)) accept[k] = 0;
+      }
+    }
+    std::vector<int> he(std::max<long long>(ne, 1));
+    if (ne > 0) PRBF_HIP(hipMemcpy(he.data(), c->elig, sizeof(int) * ne, hipMemcpyDeviceToHost));
+    c->h_skip.assign(nc, 0);
+    std::vector<int> list;
+    for (long long k = 0; k < ne; ++k)
+      if (accept[k]) {
+        list.push_back(he[k]);
+        c->h_skip[he[k]] = 1;
+      }
+    c->n_mlp = (long long)list.size();
+    c->h_mlp_list = list;
+    c->n_net_fallback = 0;
+    c->lnorm.ensure(std::max<size_t>(1, list.size()));
+    c->mlp_list.ensure(std::max<size_t>(1, list.size()));
+    c->skip.ensure(nc);
+    if (!list.empty())
+      PRBF_HIP(hipMemcpyAsync(c->mlp_list, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice, c->st));
+    PRBF_HIP(hipMemcpyAsync(c->skip, c->h_skip.data(), nc, hipMemcpyHostToDevice, c->st));
+    sync(c);
+    rasm_setup_impl(c, PRBF_PC_LEARNED);
+    c->learned_ready = true;
+    if (n_network) *n_network = c->n_mlp;
+  });
+}
+
+prbf_status prbf_krylov_vectors_dev(prbf_ctx* c, int j, double* v, double* z) {
+  return guarded(c, [&] {
+    need(c->last_restart > 0 && j >= 0 && j < c->last_restart, PRBF_ERR_STATE, "no such basis vector");
+    need(c->Z.p != nullptr && c->Z.n >= (size_t)c->last_restart * c->ni, PRBF_ERR_STATE, "run an FGMRES solve first");
+    const size_t b = sizeof(double) * c->ni;
+    if (v) PRBF_HIP(hipMemcpyAsync(v, c->V.p + (long long)j * c->ni, b, hipMemcpyDeviceToDevice, c->st));
+    if (z) PRBF_HIP(hipMemcpyAsync(z, c->Z.p + (long long)j * c->ni, b, hipMemcpyDeviceToDevice, c->st));
+  });
 }
 
 }  // extern "C"

@@ -66,7 +66,8 @@ void rasm_apply_fp64(hipStream_t st, long long ncell, const long long* bptr, con
                      const double* r, double* z, int tmax);
 void rasm_apply_fp32(hipStream_t st, long long ncell, const long long* bptr, const int* bidx,
                      const int* ista, const int* iend, const long long* mptr, const float* M,
-                     const long long* fb, const double* M64, const double* r, double* z);
+                     const long long* fb, const double* M64, const unsigned char* skip, const double* r,
+                     double* z);
 
 // krylov.hip ------------------------------------------------------------------
 constexpr int kRedBlocks = 1024;  // fixed partition => run-to-run identical sums
@@ -89,5 +90,39 @@ void arnoldi_step(hipStream_t st, int j, int m, const double* dsc, double* ks);
 void arnoldi_backsub(hipStream_t st, int it, int m, const double* ks, double* y);
 void scale_dev(hipStream_t st, long long n, double* v, const double* s);  // v *= *s
 void set_scalar(hipStream_t st, double* p, double a);
+
+// learned.hip -----------------------------------------------------------------
+constexpr int kPatch = 64;      // 8 x 8 lattice slots around a 4 x 4 cluster
+constexpr int kMlpIn = 3 * kPatch;
+constexpr int kMlpHidden = 256;
+constexpr int kMlpOut = 16;     // the cluster's 4 x 4 core
+constexpr int kMlpLayers = 4;
+void mlp_dims(int l, int* K, int* N);
+long long mlp_w_off(int l);
+long long mlp_b_off(int l);
+long long mlp_nparams();
+void mlp_weights_bf16(hipStream_t st, const float* P, unsigned short* Wb, unsigned short* Wt);
+void mlp_forward(hipStream_t st, int M, const float* P, const unsigned short* Wt, const unsigned short* Xb,
+                 float* const pre[3], unsigned short* const act[3], float* Y);
+void mlp_backward(hipStream_t st, int M, const float* P, const unsigned short* Wb, const unsigned short* Xb,
+                  float* const pre[3], unsigned short* const act[3], const float* Y, const float* T, float* G,
+                  float* dYf, unsigned short* dYb, float* dHf, unsigned short* dHb, float* part, float* loss);
+void mlp_gather_rows(hipStream_t st, int M, const long long* idx, const float* X, const float* T,
+                     unsigned short* Xb, float* Tb);
+void mlp_to_bf16(hipStream_t st, long long n, const float* a, unsigned short* b);
+void mlp_adam(hipStream_t st, long long n, float* p, const float* g, float* m1, float* m2, float lr, int t);
+void mlp_random_rows(hipStream_t st, int m, long long npairs, unsigned long long seed, long long step,
+                     long long* idx);
+void copy_f32(hipStream_t st, const float* a, float* b);
+void patch_build(hipStream_t st, const Grid& g, double hl, const int* sta, const int* end, const double* xs,
+                 const double* ys, int* pidx, int* cidx, double* pctr, unsigned char* pok);
+void patch_features(hipStream_t st, long long nk, const int* list, double hl, double sigma, const int* pidx,
+                    const int* cidx, const double* pctr, const double* xs, const double* ys, const double* r,
+                    const double* z, double out_scale, float* X, float* T, double* s_out);
+void core_scatter(hipStream_t st, long long nk, const int* list, const int* cidx, const float* Y, const double* s,
+                  double out_scale, double* z);
+void core_norms(hipStream_t st, long long nk, const int* list, const int* cidx, const double* r, double* out);
+void patch_rho(hipStream_t st, long long nk, const int* list, const int* pidx, const int* cidx, const double* xs,
+               const double* ys, const double* r, const double* z, double sigma, double* rho);
 
 }  // namespace prbf

@@ -25,7 +25,8 @@ namespace prbf {
 // halves, then two full exchanges): a fixed order, so the result is identical
 // run to run.  MT = double (plain) or float (mixed; products and sums still
 // in FP64).  In mixed mode a cluster whose FP32 block broke down carries FP64
-// rows in M64 at offset fb[c] (fb[c] < 0 otherwise).  Lanes past the end of
+// rows in M64 at offset fb[c] (fb[c] < 0 otherwise); clusters with skip[c]
+// set are left to another part of the preconditioner.  Lanes past the end of
 // the last group read a valid slot of the same group with r taken as 0, so
 // the loads need no branches.
 constexpr int kApplyWaves = 4;  // clusters per workgroup, one per wavefront
@@ -136,10 +137,11 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_rasm_apply(
     long long ncell, const long long* __restrict__ bptr, const int* __restrict__ bidx,
     const int* __restrict__ ista, const int* __restrict__ iend, const long long* __restrict__ mptr,
     const MT* __restrict__ M, const long long* __restrict__ fb, const double* __restrict__ M64,
-    const double* __restrict__ r, double* __restrict__ z) {
+    const unsigned char* __restrict__ skip, const double* __restrict__ r, double* __restrict__ z) {
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const long long c = (long long)xcd_block(blockIdx.x, gridDim.x) * kApplyWaves + wave;
   if (c >= ncell) return;
+  if (skip && skip[c]) return;  // core written by another preconditioner part
   const long long b0 = bptr[c];
   const int n = (int)(bptr[c + 1] - b0);
   if (n == 0) return;
@@ -159,14 +161,15 @@ void rasm_apply_fp64(hipStream_t st, long long ncell, const long long* bptr, con
                      const double* r, double* z, int tmax) {
   (void)tmax;
   hipLaunchKernelGGL((k_rasm_apply<double>), apply_grid(ncell), dim3(64 * kApplyWaves), 0, st, ncell, bptr, bidx,
-                     ista, iend, mptr, M, nullptr, nullptr, r, z);
+                     ista, iend, mptr, M, nullptr, nullptr, nullptr, r, z);
 }
 
 void rasm_apply_fp32(hipStream_t st, long long ncell, const long long* bptr, const int* bidx,
                      const int* ista, const int* iend, const long long* mptr, const float* M,
-                     const long long* fb, const double* M64, const double* r, double* z) {
+                     const long long* fb, const double* M64, const unsigned char* skip, const double* r,
+                     double* z) {
   hipLaunchKernelGGL((k_rasm_apply<float>), apply_grid(ncell), dim3(64 * kApplyWaves), 0, st, ncell, bptr, bidx,
-                     ista, iend, mptr, M, fb, M64, r, z);
+                     ista, iend, mptr, M, fb, M64, skip, r, z);
 }
 
 }  // namespace prbf

@@ -21,7 +21,7 @@ from . import _lib as L
 
 _KSP = {"gmres": L.PRBF_KSP_GMRES, "fgmres": L.PRBF_KSP_FGMRES}
 _PC = {"none": L.PRBF_PC_NONE, "rasm": L.PRBF_PC_RASM_FP64, "asm": L.PRBF_PC_RASM_FP64,
-       "rasm32": L.PRBF_PC_RASM_FP32}
+       "rasm32": L.PRBF_PC_RASM_FP32, "learned": L.PRBF_PC_LEARNED}
 
 
 def _f64(a):
@@ -51,6 +51,8 @@ class SolveInfo:
     n_pc: int
     n_fallback: int
     hist: np.ndarray
+    n_network: int = 0
+    n_net_fallback: int = 0
 
     @property
     def converged(self) -> bool:
@@ -207,6 +209,33 @@ class Context:
     def rasm_apply_dev(self, r, z):
         self._check(self._L.prbf_rasm_apply_dev(self._h, _p(r), _p(z)))
 
+    # ---- learned preconditioner (device arrays)
+    def learned_eligible(self):
+        n = C.c_int64()
+        self._check(self._L.prbf_learned_eligible(self._h, C.byref(n)))
+        return n.value
+
+    def learned_pairs_dev(self, r, X, s, z=None, T=None, out_scale=1.0):
+        """Features (and with z, targets) of the eligible clusters; returns the row count."""
+        n = C.c_int64()
+        self._check(self._L.prbf_learned_pairs_dev(self._h, _p(r), _p(z), float(out_scale), _p(X), _p(T), _p(s),
+                                                   int(X.shape[0]), C.byref(n)))
+        return n.value
+
+    def learned_rho_dev(self, r, z, rho):
+        n = C.c_int64()
+        self._check(self._L.prbf_learned_rho_dev(self._h, _p(r), _p(z), _p(rho), int(rho.shape[0]), C.byref(n)))
+        return n.value
+
+    def learned_attach(self, net, tau):
+        """Clusters whose patch residual reduction stays <= tau use the network; returns their count."""
+        n = C.c_int64()
+        self._check(self._L.prbf_learned_attach(self._h, net._h, float(tau), C.byref(n)))
+        return n.value
+
+    def krylov_vectors_dev(self, j, v, z):
+        self._check(self._L.prbf_krylov_vectors_dev(self._h, int(j), _p(v), _p(z)))
+
     def core_rows(self, c):
         n = C.c_int()
         t = C.c_int()
@@ -231,7 +260,8 @@ class Context:
     def _info(self, inf, hist):
         n = min(hist.shape[0], inf.its + 64)
         return SolveInfo(inf.its, inf.reason, inf.rnorm0, inf.rnorm, inf.t_setup_s, inf.t_solve_s,
-                         inf.t_matvec_s, inf.t_pc_s, inf.n_matvec, inf.n_pc, inf.n_fallback, hist[:n])
+                         inf.t_matvec_s, inf.t_pc_s, inf.n_matvec, inf.n_pc, inf.n_fallback, hist[:n],
+                         inf.n_network, inf.n_net_fallback)
 
     def solve(self, w, ksp="gmres", pc="rasm", rtol=1e-10, restart=30, max_it=1000, graph=True):
         """Strengths gamma (caller's order) with A~ gamma = w."""

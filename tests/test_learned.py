@@ -1,0 +1,192 @@
+"""The learned preconditioner: its network's forward and backward kernels
+against the CPU oracle (oracle/mlp_oracle.py), which is itself pinned here
+against torch autograd (float64, CPU), central differences and bfloat16
+rounding cases; then training on the device and the learned FGMRES solve."""
+
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as MO
+
+
+def _announce(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _params(seed):
+    rng = np.random.default_rng(seed)
+    P = []
+    for k, n in MO.DIMS:
+        a = np.sqrt(6.0 / (k + n))
+        P.append(rng.uniform(-a, a, k * n))
+        P.append(rng.uniform(-0.1, 0.1, n))
+    return np.concatenate(P).astype(np.float32).astype(np.float64)
+
+
+def _batch(seed, m):
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((m, 192)).astype(np.float32).astype(np.float64)
+    T = rng.standard_normal((m, 16)).astype(np.float32).astype(np.float64)
+    return X, T
+
+
+# ---------------------------------------------------------------- CPU: the oracle
+def test_bf16_rounding_cases():
+    e = 2.0 ** -7  # bf16 spacing at 1 (8 significant bits)
+    vals = np.array([1.0, 1.0 + e / 2, 1.0 + 1.5 * e, 1.0 + e / 2 + 2 ** -20, -3.0 - e, 2.0 ** -130, 65504.0])
+    want = np.array([1.0, 1.0, 1.0 + 2 * e, 1.0 + e, -3.0, 2.0 ** -130, 65536.0])
+    # ties go to the even significand (1 + e/2 -> 1, 1 + 1.5e -> 1 + 2e, -3 - e -> -3);
+    # just above a tie rounds up; 2^-130 is exact in bf16's subnormal range
+    assert np.array_equal(MO.bf16(vals), want)
+
+
+def test_oracle_gradient_matches_torch_autograd():
+    torch = pytest.importorskip("torch")
+    P = _params(1)
+    X, T = _batch(2, 32)
+    loss, G = MO.loss_grad(P, X, T)
+    Pt = torch.tensor(P, dtype=torch.float64, requires_grad=True)
+    h = torch.tensor(X)
+    o = 0
+    for l, (k, n) in enumerate(MO.DIMS):
+        W = Pt[o:o + k * n].view(k, n)
+        o += k * n
+        b = Pt[o:o + n]
+        o += n
+        h = h @ W + b
+        if l < 3:
+            h = torch.nn.functional.gelu(h, approximate="tanh")
+    lt = torch.mean((h - torch.tensor(T)) ** 2)
+    lt.backward()
+    assert abs(loss - lt.item()) <= 1e-13 * abs(loss)
+    g = Pt.grad.numpy()
+    assert np.linalg.norm(G - g) <= 1e-11 * np.linalg.norm(g)
+
+
+def test_oracle_gradient_central_differences():
+    P = _params(3)
+    X, T = _batch(4, 16)
+    _, G = MO.loss_grad(P, X, T)
+    rng = np.random.default_rng(5)
+    idx = np.concatenate([rng.integers(0, MO.num_params(), 24), [MO.num_params() - 1]])
+    for i in idx:
+        h = 1e-5
+        Pp, Pm = P.copy(), P.copy()
+        Pp[i] += h
+        Pm[i] -= h
+        fd = (MO.loss_grad(Pp, X, T)[0] - MO.loss_grad(Pm, X, T)[0]) / (2 * h)
+        assert abs(fd - G[i]) <= 1e-6 * max(1e-3, abs(G[i])), (i, fd, G[i])
+
+
+def test_oracle_bf16_mode_stays_close_to_wide_mode():
+    P = _params(6)
+    X, T = _batch(7, 64)
+    l0, g0 = MO.loss_grad(P, X, T)
+    l1, g1 = MO.loss_grad(P, X, T, bf=True)
+    assert abs(l1 - l0) <= 0.05 * l0
+    assert np.linalg.norm(g1 - g0) <= 0.05 * np.linalg.norm(g0)
+
+
+# ---------------------------------------------------------------- GPU: kernels vs oracle
+@pytest.mark.gpu
+def test_mlp_forward_and_gradient_match_oracle():
+    torch = pytest.importorskip("torch")
+    from petrbf_amd.learned import Mlp
+    _announce("mlp forward/backward vs oracle")
+    dev = torch.device("cuda:0")
+    net = Mlp(seed=11)
+    P = _params(8)
+    net.set_params(P.astype(np.float32), 1.0)
+    Pg, s = net.params()
+    assert np.array_equal(Pg, P.astype(np.float32)) and s == 1.0
+    for m in (64, 208):
+        X, T = _batch(9 + m, m)
+        Xd = torch.tensor(X, dtype=torch.float32, device=dev)
+        Td = torch.tensor(T, dtype=torch.float32, device=dev)
+        Yd = torch.empty((m, 16), dtype=torch.float32, device=dev)
+        torch.cuda.synchronize()
+        net.forward_dev(Xd, Yd)
+        Y = Yd.cpu().numpy().astype(np.float64)
+        Yo = MO.forward(P, X, bf=True)[0]
+        assert np.linalg.norm(Y - Yo) <= 1e-4 * np.linalg.norm(Yo)
+        Gd = torch.empty(MO.num_params(), dtype=torch.float32, device=dev)
+        loss = net.loss_grad_dev(Xd, Td, Gd)
+        lo, Go = MO.loss_grad(P, X, T, bf=True)
+        assert abs(loss - lo) <= 1e-4 * lo
+        G = Gd.cpu().numpy().astype(np.float64)
+        o = 0
+        for k, n in MO.DIMS:
+            for size in (k * n, n):
+                a, b = G[o:o + size], Go[o:o + size]
+                assert np.linalg.norm(a - b) <= 2e-3 * np.linalg.norm(b) + 1e-12, (m, o, size)
+                o += size
+    net.close()
+
+
+@pytest.mark.gpu
+def test_learned_preconditioner_trains_and_solve_reaches_fp64_residual():
+    """Harvest (patch, exact block solve) pairs from an exact-RASM FGMRES run
+    on one seeded lattice, train the network on the device, and solve another
+    seeded lattice with the learned preconditioner: FGMRES reaches rtol 1e-10
+    with the FP64 true residual of the exact-block solve, its per-cluster
+    fallback moving clusters whose core residual stalls to exact blocks.  The
+    gain of the trained network is measured against the same method with an
+    untrained network (held-out loss, iterations) and against exact FP32
+    blocks everywhere."""
+    torch = pytest.importorskip("torch")
+    from petrbf_amd import problems
+    from petrbf_amd.learned import Mlp, harvest
+    from petrbf_amd.rbf import Context
+    _announce("learned preconditioner: harvest, train, solve")
+    dev = torch.device("cuda:0")
+    pt = problems.lattice_2d_problem(200, seed=0x5EED0002)
+    ct = Context.from_problem(pt)
+    ct.set_points(pt.x, pt.y)
+    X, T, osc = harvest(ct, torch.from_numpy(pt.w[0]).to(dev), 24, torch)
+    assert X.shape[0] > 10000
+    p = problems.lattice_2d_problem(200, seed=0x5EED0001)
+    ctx = Context.from_problem(p)
+    ctx.set_points(p.x, p.y)
+    Xh, Th, osh = harvest(ctx, torch.from_numpy(p.w[0]).to(dev), 24, torch)
+    Th *= osh / osc  # held-out targets on the training scale
+    net0 = Mlp(seed=7)
+    net = Mlp(seed=7)
+    net.set_params(net.params()[0], osc)
+    net0.set_params(net0.params()[0], osc)
+    hist = net.train_dev(X, T, batch=1024, steps=1500, lr=1e-3, seed=3)
+    assert hist[-50:].mean() < 0.5 * hist[:20].mean()  # the kernels fit the training pairs
+    held = []
+    for nn in (net0, net):
+        Y = torch.empty_like(Th)
+        nn.forward_dev(Xh, Y)
+        held.append(float(torch.mean((Y - Th) ** 2)))
+    w = torch.from_numpy(p.w[0]).to(dev)
+    gam = torch.empty_like(w)
+    wv = torch.empty_like(w)
+    out = {}
+    for tag, nn in (("trained", net), ("untrained", net0), ("exact", None)):
+        if nn is not None:
+            n_net = ctx.learned_attach(nn, 1e30)
+            assert n_net == ctx.learned_eligible() > 0
+        torch.cuda.synchronize()
+        inf = ctx.solve_dev(w, gam, ksp="fgmres", pc="learned" if nn is not None else "rasm32", rtol=1e-10)
+        ctx.evaluate_dev(gam, wv)
+        ctx.synchronize()
+        true_rel = float(torch.linalg.norm(wv - w) / torch.linalg.norm(w))
+        assert inf.reason == 2 and true_rel <= 1.05e-10, (tag, inf.reason, true_rel)
+        out[tag] = (inf.its, inf.n_network, inf.n_net_fallback, true_rel)
+    _announce(f"held-out loss untrained {held[0]:.3f} trained {held[1]:.3f}; "
+              f"its/network/moved/true residual: {out}")
+    # the gain of the trained network over exact blocks, in iterations (negative:
+    # a cost); the per-cluster fallback bounds that cost to the cycles it takes
+    # FGMRES's residual to expose the network's clusters
+    gain = out["exact"][0] - out["trained"][0]
+    _announce(f"iterations gained by the trained network: {gain} (vs untrained {out['untrained'][0] - out['trained'][0]})")
+    assert out["trained"][0] <= out["exact"][0] + 2 * 30
+    net.close()
+    net0.close()
+    ctx.close()
+    ct.close()
