@@ -25,7 +25,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
+PEAK_FP32_MFMA_TFLOPS = 157.3  # MI355X dense FP32 matrix
 PEAK_HBM_GBS = 8000.0     # MI355X HBM3E
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "block_chol_fp32_n4m_hbm_traffic.txt")
+
+
+def measured_traffic(kernel_key):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel
+    calibration), or None when the summary is absent."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            f = line.split()
+            if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel_key:
+                return float(f[2])
+    except OSError:
+        pass
+    return None
 FLOP_PER_PAIR_2D = 15     # 2d/matmult.cxx:44-45 (exp counted as one flop)
 
 
@@ -33,11 +49,12 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(p, its, budget_s=30.0):
+def cpu_baseline(p, its, pairs, stride=25, n_mv=3):
     """Host-core oracle (the reference's CPU algorithm: LU blocks, GMRES(30))
-    timed on a bounded sample of the same N = 4M workload and scaled to the
-    full solve: lists at full size, one full matvec, the block LU of every
-    k-th cluster and its apply, then t = lists + factor/frac + its*(matvec + apply/frac)."""
+    timed on a bounded sample of the same N = 4M workload (about 10 s of host
+    work) and scaled to the full solve: lists at full size, n_mv full
+    matvecs, the block LU of every stride-th cluster and its apply, then
+    t = lists + factor/frac + its*(matvec + apply/frac)."""
     from oracle.oracle import Oracle, num_threads
     t0 = time.perf_counter()
     o = Oracle.from_problem(p)
@@ -45,9 +62,9 @@ def cpu_baseline(p, its, budget_s=30.0):
     pi, _ = o.perm()
     g = p.w[0][pi].copy()
     t0 = time.perf_counter()
-    o.matvec(g)
-    t_mv = time.perf_counter() - t0
-    stride = 200
+    for _ in range(n_mv):
+        o.matvec(g)
+    t_mv = (time.perf_counter() - t0) / n_mv
     t0 = time.perf_counter()
     o.rasm_setup_sample(stride)
     t_fac = time.perf_counter() - t0
@@ -59,11 +76,11 @@ def cpu_baseline(p, its, budget_s=30.0):
     est = t_lists + t_fac / frac + its * (t_mv + t_ap / frac)
     return {
         "value": est, "unit": "s", "cores": num_threads(), "kind": "port",
-        "sample": (f"N=4M lattice: lists+sort full ({t_lists:.2f}s), one full matvec ({t_mv:.2f}s), "
+        "sample": (f"N=4M lattice: lists+sort full ({t_lists:.2f}s), {n_mv} full matvecs ({t_mv:.2f}s each), "
                    f"block LU of every {stride}th cluster ({t_fac:.2f}s) and its apply ({t_ap:.3f}s), "
                    f"scaled to all {ncell} clusters and {its} GMRES iterations"),
         "t_lists_s": t_lists, "t_matvec_s": t_mv,
-        "pair_evals_per_s": None,
+        "pair_evals_per_s": pairs / t_mv,
     }
 
 
@@ -77,6 +94,9 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=10)
     ap.add_argument("--mode", choices=["plain", "mixed", "both"], default="both")
     ap.add_argument("--chol-waves", type=int, default=0, help="4 or 8 wavefronts per block factorisation")
+    ap.add_argument("--chol-panel", type=int, default=0, help="1, 2 or 4 tile columns per panel")
+    ap.add_argument("--setup-only", choices=["rasm", "rasm32"], default=None,
+                    help="only build the workload's lists and run one RASM setup (for counter runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of 1 GiB five times (HBM counter calibration)")
     args = ap.parse_args()
@@ -110,10 +130,23 @@ def main():
     ctx.set_stream(stream.cuda_stream)
     if args.chol_waves:
         ctx.set_tuning(chol_waves=args.chol_waves)
+    if args.chol_panel:
+        ctx.set_tuning(chol_panel=args.chol_panel)
+    if args.setup_only:
+        ctx.set_points_dev(x, y)
+        t0 = time.perf_counter()
+        ctx.rasm_setup(args.setup_only)
+        torch.cuda.synchronize()
+        ci = ctx.info()
+        print(json.dumps({"setup": args.setup_only, "wall_s": time.perf_counter() - t0,
+                          "kernel_s": ci["chol_kernel_s"], "flops": ci["chol_flops"],
+                          "chol_waves": ci["chol_waves"], "chol_panel": ci["chol_panel"]}), flush=True)
+        return
 
     modes = {"plain": ("gmres", "rasm"), "mixed": ("fgmres", "rasm32")}
     run = ["plain", "mixed"] if args.mode == "both" else [args.mode]
     results = {}
+    chol = {}
     for mode in run:
         ksp, pc = modes[mode]
 
@@ -139,7 +172,9 @@ def main():
         true_rel = float(torch.linalg.norm(wv - w) / torch.linalg.norm(w))
         log(f"{mode}: {args.steps} steps {elapsed:.3f}s; its {inf.its} reason {inf.reason} true rel resid {true_rel:.3e}")
         ci = ctx.info()
-        log(f"{mode}: block factorisation config waves={ci['chol_waves']} panel={ci['chol_panel']}")
+        log(f"{mode}: block factorisation config waves={ci['chol_waves']} panel={ci['chol_panel']}, "
+            f"kernel {ci['chol_kernel_s']*1e3:.1f} ms, {ci['chol_flops']/ci['chol_kernel_s']/1e12:.1f} TFLOP/s")
+        chol[mode] = (ci["chol_kernel_s"], ci["chol_flops"], ci["chol_waves"], ci["chol_panel"])
         results[mode] = dict(ms_per_step=elapsed / args.steps * 1e3, its=inf.its, reason=inf.reason,
                              chol_waves=ci["chol_waves"], chol_panel=ci["chol_panel"],
                              t_setup_s=inf.t_setup_s, t_solve_s=inf.t_solve_s, t_matvec_s=inf.t_matvec_s,
@@ -206,9 +241,9 @@ def main():
                                    if len(results) == 2 else None),
         "pair_evals_per_s": pairs / t_mv,
         "pairs_per_matvec": pairs,
-        "roofline": {"bound": "hbm", "kernel": "k_rasm_apply<double>", "achieved": apply_bytes / t_ap / 1e9,
-                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": apply_bytes / t_ap / 1e9 / PEAK_HBM_GBS,
-                     "traffic": None, "avg_ms": t_ap * 1e3, "bytes_per_launch": apply_bytes},
+        "roofline_apply_fp64": {"bound": "hbm", "kernel": "k_rasm_apply<double>", "achieved": apply_bytes / t_ap / 1e9,
+                                "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": apply_bytes / t_ap / 1e9 / PEAK_HBM_GBS,
+                                "avg_ms": t_ap * 1e3, "bytes_per_launch": apply_bytes},
         "roofline_apply_fp32": {"bound": "hbm", "kernel": "k_rasm_apply<float>",
                                 "achieved": apply32_bytes / t_ap32 / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                                 "frac": apply32_bytes / t_ap32 / 1e9 / PEAK_HBM_GBS, "avg_ms": t_ap32 * 1e3,
@@ -218,10 +253,23 @@ def main():
                             "frac": mv_flops / t_mv / 1e12 / PEAK_FP64_TFLOPS, "avg_ms": t_mv * 1e3,
                             "flops_per_launch": mv_flops},
     }
+    # the dominant kernel of the headline (mixed) step: the FP32 block
+    # factorisation, one persistent launch per setup, timed with HIP events on
+    # the library's stream; FLOPs = sum over blocks of n^3/3 + 2 n^2 T
+    for mode, key, peak, name in (("mixed", "roofline", PEAK_FP32_MFMA_TFLOPS, "k_block_chol<float,2,%d,%d>"),
+                                  ("plain", "roofline_block_chol_fp64", PEAK_FP64_TFLOPS,
+                                   "k_block_chol<double,2,%d,%d>")):
+        if mode not in chol:
+            continue
+        t_k, fl, nw, npc = chol[mode]
+        kname = name % (nw, npc)
+        res[key] = {"bound": "mfma", "kernel": kname, "achieved": fl / t_k / 1e12, "peak": peak,
+                    "unit": "TFLOP/s", "frac": fl / t_k / 1e12 / peak, "avg_ms": t_k * 1e3,
+                    "flops_per_launch": fl, "traffic": measured_traffic(kname)}
     inf_its = results.get("plain", results[head])["its"]
     if not args.no_cpu:
         log("cpu baseline (oracle, bounded sample)")
-        res["cpu_baseline"] = cpu_baseline(p, inf_its)
+        res["cpu_baseline"] = cpu_baseline(p, inf_its, pairs)
         log(f"cpu baseline est {res['cpu_baseline']['value']:.1f}s on {res['cpu_baseline']['cores']} threads")
     print(json.dumps(res), flush=True)
 

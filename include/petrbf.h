@@ -120,7 +120,10 @@ prbf_status prbf_set_points_dev(prbf_ctx* ctx, int64_t ni, const double* xi, con
 /* info[0..15]: ncell, nx, ny, nz, nb_buffer, nb_trunc, buffer_nnz, trunc_nnz,
  * max_buffer, max_core, max_trunc, ni, nj, and the wavefronts and panel width
  * the last block factorisation used (0 before any), 0.
- * geom[0..5]: xmin, ymin, zmin, box, buffer half-width, trunc half-width. */
+ * geom[0..7]: xmin, ymin, zmin, box, buffer half-width, trunc half-width, the
+ * device time (s) of the last block-factorisation launch and its algorithmic
+ * FLOPs, sum over blocks of n^3/3 + 2 n^2 T (Cholesky and the two triangular
+ * solves with the T core right-hand sides). */
 prbf_status prbf_get_info(prbf_ctx* ctx, int64_t* info, double* geom);
 /* Sorted position -> original index (the reference's particle->ii / jj). */
 prbf_status prbf_get_permutation(prbf_ctx* ctx, int32_t* perm_i, int32_t* perm_j);
@@ -157,7 +160,7 @@ prbf_status prbf_rasm_setup(prbf_ctx* ctx);
 prbf_status prbf_rasm_setup_pc(prbf_ctx* ctx, int pc);
 /* Force a launch configuration of the block factorisation instead of the
  * built-in autotuner: PRBF_TUNE_CHOL_WAVES (4 or 8 wavefronts per workgroup),
- * PRBF_TUNE_CHOL_PANEL (1 or 2 16-wide tile columns per panel).  No
+ * PRBF_TUNE_CHOL_PANEL (1, 2 or 4 16-wide tile columns per panel).  No
  * reference counterpart. */
 prbf_status prbf_set_tuning(prbf_ctx* ctx, int key, int value);
 /* PCApply_ASM (restrict): z = sum_c P_core A_c^{-1} R_buffer r, sorted order. */

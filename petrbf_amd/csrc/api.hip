@@ -124,6 +124,7 @@ struct prbf_ctx {
   int chol_panel = 2;  // tile columns per panel (when forced)
   bool tuning_forced = false;
   int used_nw = 0, used_npc = 0;  // configuration of the last factorisation
+  double t_chol_s = 0, chol_flops = 0;  // its kernel time and algorithmic FLOPs
   int rasm_pc = 0;
   int n_fallback = 0;
   DBuf<float> M32;
@@ -345,7 +346,7 @@ struct CholCfg {
 
 // Launch configurations the tuner chooses among (wavefronts per workgroup,
 // tile columns per panel).
-const CholCfg kCholCfgs[] = {{4, 1}, {8, 1}, {4, 2}, {8, 2}};
+const CholCfg kCholCfgs[] = {{4, 1}, {8, 1}, {4, 2}, {8, 2}, {4, 4}, {8, 4}};
 
 // Process-wide cache of tuned configurations, keyed by precision and the
 // largest block's tile count.
@@ -425,10 +426,21 @@ void factor_blocks(prbf_ctx* c, bool fp32, const std::vector<int>& order, int nm
             : "RASM block too large for the LDS panel of the FP64 block factorisation");
   const int nwg = workgroups(cfg);
   W.ensure((size_t)nwg * ws_slot);
+  PRBF_HIP(hipEventRecord(c->ev0, c->st));
   launch_blocks<Real>(c, fp32, cfg, nwork, d_order, next, nwg, ws_slot, nmax, tmax, W.p, mptr, Mout, status);
+  PRBF_HIP(hipEventRecord(c->ev1, c->st));
   c->used_nw = cfg.nw;
   c->used_npc = cfg.npc;
   sync(c);
+  float ms = 0;
+  PRBF_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->t_chol_s = ms * 1e-3;
+  double fl = 0;
+  for (int k : order) {
+    const double nb = (double)(c->h_bptr[k + 1] - c->h_bptr[k]), T = c->h_iend[k] - c->h_ista[k] + 1;
+    fl += nb * nb * nb / 3.0 + 2.0 * nb * nb * T;
+  }
+  c->chol_flops = fl;
 }
 
 void rasm_setup_impl(prbf_ctx* c, int pc) {
@@ -1039,7 +1051,7 @@ prbf_status prbf_get_info(prbf_ctx* c, int64_t* info, double* geom) {
       std::memcpy(info, v, sizeof(v));
     }
     if (geom) {
-      double v[6] = {g.xmin, g.ymin, g.zmin, g.box, g.buffer_len, g.trunc_len};
+      double v[8] = {g.xmin, g.ymin, g.zmin, g.box, g.buffer_len, g.trunc_len, c->t_chol_s, c->chol_flops};
       std::memcpy(geom, v, sizeof(v));
     }
   });
@@ -1160,7 +1172,7 @@ prbf_status prbf_set_tuning(prbf_ctx* c, int key, int value) {
       need(value == 4 || value == 8, PRBF_ERR_ARG, "waves per block factorisation must be 4 or 8");
       c->chol_waves = value;
     } else if (key == PRBF_TUNE_CHOL_PANEL) {
-      need(value == 1 || value == 2, PRBF_ERR_ARG, "panel width must be 1 or 2 tile columns");
+      need(value == 1 || value == 2 || value == 4, PRBF_ERR_ARG, "panel width must be 1, 2 or 4 tile columns");
       c->chol_panel = value;
     } else {
       need(false, PRBF_ERR_ARG, "unknown tuning key");

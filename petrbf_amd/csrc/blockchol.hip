@@ -17,7 +17,8 @@
 //            (in LDS) with L_kk^{-1} beside it and the column as
 //            A_ik L_kk^{-T} on MFMA; then one trailing update
 //            A_ij -= sum L_ic L_jc^T per panel on MFMA with the panel in LDS
-//            (NPC = 2 halves the passes over the trailing tiles);
+//            (NPC = 2 or 4 halves or quarters the passes over the trailing
+//            tiles at the cost of LDS);
 //   phase 2  with the core last, L^{-1} E_core is zero above the core, so
 //            A'^{-1} E_core = L^{-T} (L^{-1} E_core) is a short forward and a
 //            full backward tile substitution, both MFMA with the
@@ -487,7 +488,9 @@ static void set_lds_attr() {
     const void* fns[] = {(const void*)k_block_chol<T, 2, 4, 1>, (const void*)k_block_chol<T, 3, 4, 1>,
                          (const void*)k_block_chol<T, 2, 8, 1>, (const void*)k_block_chol<T, 3, 8, 1>,
                          (const void*)k_block_chol<T, 2, 4, 2>, (const void*)k_block_chol<T, 3, 4, 2>,
-                         (const void*)k_block_chol<T, 2, 8, 2>, (const void*)k_block_chol<T, 3, 8, 2>};
+                         (const void*)k_block_chol<T, 2, 8, 2>, (const void*)k_block_chol<T, 3, 8, 2>,
+                         (const void*)k_block_chol<T, 2, 4, 4>, (const void*)k_block_chol<T, 3, 4, 4>,
+                         (const void*)k_block_chol<T, 2, 8, 4>, (const void*)k_block_chol<T, 3, 8, 4>};
     for (const void* f : fns)
       PRBF_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     attr = true;
@@ -497,12 +500,11 @@ static void set_lds_attr() {
 template <typename T>
 static const void* chol_fn(int dim, int nw, int npc) {
 #define PRBF_FN(D, NWV, NPCV) (const void*)k_block_chol<T, D, NWV, NPCV>
-  if (dim == 2) {
-    if (nw == 8) return npc == 2 ? PRBF_FN(2, 8, 2) : PRBF_FN(2, 8, 1);
-    return npc == 2 ? PRBF_FN(2, 4, 2) : PRBF_FN(2, 4, 1);
-  }
-  if (nw == 8) return npc == 2 ? PRBF_FN(3, 8, 2) : PRBF_FN(3, 8, 1);
-  return npc == 2 ? PRBF_FN(3, 4, 2) : PRBF_FN(3, 4, 1);
+#define PRBF_FN_NPC(D, NWV) \
+  (npc == 4 ? PRBF_FN(D, NWV, 4) : npc == 2 ? PRBF_FN(D, NWV, 2) : PRBF_FN(D, NWV, 1))
+  if (dim == 2) return nw == 8 ? PRBF_FN_NPC(2, 8) : PRBF_FN_NPC(2, 4);
+  return nw == 8 ? PRBF_FN_NPC(3, 8) : PRBF_FN_NPC(3, 4);
+#undef PRBF_FN_NPC
 #undef PRBF_FN
 }
 
@@ -533,7 +535,7 @@ static void launch_block_chol(hipStream_t st, int dim, int nw, int npc, long lon
                      bptr, bidx, ista, iend, xs, ys, zs, W, mptr, M, k.inv_two_s2, k.norm, item_slot, err,         \
                      blk_status)
 #define PRBF_BC_NPC(D, NWV) \
-  if (npc == 2) PRBF_BC(D, NWV, 2); else PRBF_BC(D, NWV, 1)
+  if (npc == 4) PRBF_BC(D, NWV, 4); else if (npc == 2) PRBF_BC(D, NWV, 2); else PRBF_BC(D, NWV, 1)
   if (dim == 2) {
     if (nw == 8) { PRBF_BC_NPC(2, 8); } else { PRBF_BC_NPC(2, 4); }
   } else {

@@ -115,12 +115,13 @@ class Context:
 
     def info(self):
         a = np.zeros(16, np.int64)
-        g = np.zeros(6, np.float64)
+        g = np.zeros(8, np.float64)
         self._check(self._L.prbf_get_info(self._h, _p(a), _p(g)))
         keys = ["ncell", "nx", "ny", "nz", "nb_buffer", "nb_trunc", "buffer_nnz", "trunc_nnz",
                 "max_buffer", "max_core", "max_trunc", "ni", "nj", "chol_waves", "chol_panel"]
         d = {k: int(a[i]) for i, k in enumerate(keys)}
-        d.update(xmin=g[0], ymin=g[1], zmin=g[2], box=g[3], buffer_len=g[4], trunc_len=g[5])
+        d.update(xmin=g[0], ymin=g[1], zmin=g[2], box=g[3], buffer_len=g[4], trunc_len=g[5], chol_kernel_s=g[6],
+                 chol_flops=g[7])
         return d
 
     def permutation(self):

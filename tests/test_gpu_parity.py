@@ -239,19 +239,22 @@ def test_mixed_core_rows_close_to_fp64():
 @pytest.mark.parametrize("prec", ["rasm", "rasm32"])
 def test_block_factorisation_launch_configs_agree(prec):
     """Every launch configuration the autotuner may pick (4/8 wavefronts,
-    1/2 tile columns per panel) yields the same preconditioner."""
+    1/2/4 tile columns per panel) yields the same preconditioner."""
     _announce(f"chol configs {prec}")
     p = lattice(40)
     rng = np.random.default_rng(2)
     r = rng.standard_normal(p.n)
     outs = []
     for nw in (4, 8):
-        for npc in (1, 2):
+        for npc in (1, 2, 4):
             ctx = Context.from_problem(p)
             ctx.set_tuning(chol_waves=nw, chol_panel=npc)
             ctx.set_points(p.x, p.y)
             ctx.rasm_setup(prec)
-            assert (ctx.info()["chol_waves"], ctx.info()["chol_panel"]) == (nw, npc)
+            used = (ctx.info()["chol_waves"], ctx.info()["chol_panel"])
+            # a forced configuration whose panel does not fit in LDS (FP64 with
+            # 4-column panels at these block sizes) is replaced by one that does
+            assert used == (nw, npc) or (prec == "rasm" and npc == 4), used
             outs.append(ctx.rasm_apply(r))
             ctx.close()
     tol = 1e-9 if prec == "rasm" else 1e-2
