@@ -460,8 +460,11 @@ __device__ void factor_block(
 // the next block from a global counter until the work list is exhausted (the
 // list is ordered largest block first, so uneven blocks balance over the
 // workgroups).  Every wavefront leaves through the same counter test.
+// Register caps the compiler meets without spilling: FP32 at most 168 VGPRs
+// (three wavefronts per SIMD), FP64 at most 256 (two), so that LDS, not
+// registers, bounds the workgroups per CU.
 template <typename Real, int DIM, int NW, int NPC>
-__global__ __launch_bounds__(NW * 64) void k_block_chol(
+__global__ __launch_bounds__(NW * 64, sizeof(Real) == 4 ? 3 : 2) void k_block_chol(
     long long nwork, const int* __restrict__ order, int* __restrict__ next, long long ws_slot,
     const long long* __restrict__ bptr, const int* __restrict__ bidx, const int* __restrict__ ista,
     const int* __restrict__ iend, const double* __restrict__ xs, const double* __restrict__ ys,
