@@ -95,6 +95,10 @@ def main():
     ap.add_argument("--mode", choices=["plain", "mixed", "both"], default="both")
     ap.add_argument("--chol-waves", type=int, default=0, help="4 or 8 wavefronts per block factorisation")
     ap.add_argument("--chol-panel", type=int, default=0, help="1, 2 or 4 tile columns per panel")
+    ap.add_argument("--workload", choices=["c2", "c5"], default="c2",
+                    help="c2: the 2D N=4M solve (the headline); c5: the 3D 160^3 three-RHS workload "
+                         "(lists, batched matvec, FP32 setup, FGMRES progress), its own JSON line")
+    ap.add_argument("--c5-its", type=int, default=150, help="FGMRES iteration budget per component (c5)")
     ap.add_argument("--setup-only", choices=["rasm", "rasm32"], default=None,
                     help="only build the workload's lists and run one RASM setup (for counter runs)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -118,6 +122,8 @@ def main():
     from petrbf_amd.rbf import Context
 
     dev = torch.device("cuda:0")
+    if args.workload == "c5":
+        return bench_c5(args, torch, problems, Context, dev)
     log(f"generating lattice {args.n_side}^2")
     p = problems.lattice_2d_problem(args.n_side, seed=0x5EED0001)
     x = torch.from_numpy(p.x).to(dev)
@@ -272,6 +278,66 @@ def main():
         res["cpu_baseline"] = cpu_baseline(p, inf_its, pairs)
         log(f"cpu baseline est {res['cpu_baseline']['value']:.1f}s on {res['cpu_baseline']['cores']} threads")
     print(json.dumps(res), flush=True)
+
+
+def bench_c5(args, torch, problems, Context, dev):
+    """3D vector-field workload: 160^3 jittered lattice, 3x3x3-spacing cells,
+    whole-cell buffer, 4-sigma cutoff, vortex-ring vorticity as three
+    right-hand sides.  Measures the lists, the three-RHS matvec (one exp per
+    pair shared by the three components), the FP32 block setup and how far
+    FGMRES(30) gets per component within a fixed iteration budget."""
+    n_side = 160
+    log(f"generating 3D lattice {n_side}^3")
+    p = problems.lattice_3d_problem(n_side)
+    x, y, z = (torch.from_numpy(a).to(dev) for a in (p.x, p.y, p.z))
+    w = torch.from_numpy(np.ascontiguousarray(p.w)).to(dev)  # (3, n)
+    ctx = Context.from_problem(p)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.set_points_dev(x, y, z)
+    torch.cuda.synchronize()
+    t_lists = time.perf_counter() - t0
+    info = ctx.info()
+    ista, iend, _, _ = ctx.cells()
+    T = (iend - ista + 1).astype(np.int64)
+    pairs = int((np.maximum(T, 0) * np.diff(ctx.trunc_ptr())).sum())
+    out = torch.empty_like(w)
+    ctx.matvec_dev(w, out, nrhs=3)
+    ctx.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.matvec_dev(w, out, nrhs=3)
+    ctx.synchronize()
+    t_mv = (time.perf_counter() - t0) / reps
+    flops = (18 + 2 * 2) * pairs  # 3D pair (18) + two more right-hand sides sharing the exp
+    log(f"3D lists {t_lists:.3f}s; 3-RHS matvec {t_mv*1e3:.2f} ms ({pairs/t_mv:.3e} pairs/s, "
+        f"{flops/t_mv/1e12:.1f} TFLOP/s)")
+    t0 = time.perf_counter()
+    ctx.rasm_setup("rasm32")
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    ci = ctx.info()
+    log(f"FP32 setup {t_setup:.3f}s (kernel {ci['chol_kernel_s']:.3f}s, {ci['chol_flops']/ci['chol_kernel_s']/1e12:.1f} "
+        f"TFLOP/s, max block {ci['max_buffer']})")
+    comps = []
+    gam = torch.empty(p.n, dtype=torch.float64, device=dev)
+    for c in range(3):
+        wc = w[c].contiguous()
+        t0 = time.perf_counter()
+        inf = ctx.solve_dev(wc, gam, ksp="fgmres", pc="rasm32", rtol=1e-10, restart=30, max_it=args.c5_its)
+        ctx.synchronize()
+        rel = inf.rnorm / inf.rnorm0 if inf.rnorm0 > 0 else 0.0  # the ring's z component is zero
+        comps.append({"its": inf.its, "reason": inf.reason, "rel_residual": rel,
+                      "t_solve_s": time.perf_counter() - t0})
+        log(f"component {c}: its {inf.its} reason {inf.reason} residual {rel:.3e}")
+    print(json.dumps({
+        "metric": "3D 160^3 three-RHS workload: lists, batched matvec, FP32 setup, FGMRES progress",
+        "n": p.n, "ncell": info["ncell"], "max_buffer": ci["max_buffer"], "t_lists_s": t_lists,
+        "matvec_3rhs_ms": t_mv * 1e3, "pairs_per_matvec": pairs, "pair_evals_per_s": pairs / t_mv,
+        "matvec_tflops": flops / t_mv / 1e12, "setup_fp32_s": t_setup, "chol_kernel_s": ci["chol_kernel_s"],
+        "chol_tflops": ci["chol_flops"] / ci["chol_kernel_s"] / 1e12, "fgmres": comps,
+    }), flush=True)
 
 
 if __name__ == "__main__":

@@ -131,6 +131,9 @@ struct prbf_ctx {
   DBuf<long long> fb;
   DBuf<double> M64fb;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // block factorisation: work list, counter and workspace slots
+  DBuf<int> fb_order, fb_next, fb_status;
+  DBuf<unsigned char> fb_ws;
   // Arnoldi steps captured as graphs: device state of the cycle, one
   // executable graph per step index, and the launch arguments they captured
   int last_restart = 0;  // basis size of the last Krylov solve
@@ -382,7 +385,8 @@ void factor_blocks(prbf_ctx* c, bool fp32, const std::vector<int>& order, int nm
   const Grid& g = c->g;
   const long long nwork = (long long)order.size();
   if (nwork == 0) return;
-  DBuf<int> d_order, next;
+  DBuf<int>& d_order = c->fb_order;
+  DBuf<int>& next = c->fb_next;
   d_order.ensure(order.size());
   next.ensure(1);
   PRBF_HIP(hipMemcpyAsync(d_order, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice, c->st));
@@ -402,7 +406,15 @@ void factor_blocks(prbf_ctx* c, bool fp32, const std::vector<int>& order, int nm
   if (tune)
     for (auto& kv : tuned_cache())
       if (kv.first == key) { cfg = kv.second; tune = false; }
-  DBuf<Real> W;
+  // the workspace slots, kept by the context from one setup to the next
+  struct {
+    DBuf<unsigned char>& raw;
+    Real* p = nullptr;
+    void ensure(size_t count) {
+      raw.ensure(count * sizeof(Real));
+      p = (Real*)raw.p;
+    }
+  } W{c->fb_ws};
   if (tune) {
     const long long ns = std::min<long long>(4096, nwork);
     float best = 1e30f;
@@ -464,10 +476,16 @@ void rasm_setup_impl(prbf_ctx* c, int pc) {
     c->h_mptr[k + 1] = c->h_mptr[k] + apply_elems((int)n, (int)T);
     if (n && !(learned && c->h_skip[k])) order.push_back((int)k);
   }
-  // largest blocks first: the persistent workgroups then finish together
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-    return c->h_bptr[a + 1] - c->h_bptr[a] > c->h_bptr[b + 1] - c->h_bptr[b];
-  });
+  // largest blocks first (stable counting sort by size): the persistent
+  // workgroups then finish together
+  {
+    const int nb = c->max_buffer + 1;
+    std::vector<int> cnt(nb + 1, 0), sorted(order.size());
+    for (int k : order) ++cnt[nb - 1 - (int)(c->h_bptr[k + 1] - c->h_bptr[k]) + 1];
+    for (int i = 0; i < nb; ++i) cnt[i + 1] += cnt[i];
+    for (int k : order) sorted[cnt[nb - 1 - (int)(c->h_bptr[k + 1] - c->h_bptr[k])]++] = k;
+    order.swap(sorted);
+  }
   need(block_lds_bytes(fp32, c->max_buffer, c->max_core, 4, 1) <= kMaxLdsBytes, PRBF_ERR_LIMIT,
        "RASM block too large for the LDS panel of the block factorisation");
   c->rasm_ready = false;
@@ -480,7 +498,7 @@ void rasm_setup_impl(prbf_ctx* c, int pc) {
     factor_blocks<double>(c, false, order, c->max_buffer, c->max_core, c->mptr, c->M.p, nullptr, true);
   } else {
     c->M32.ensure(c->h_mptr[nc]);
-    DBuf<int> status;
+    DBuf<int>& status = c->fb_status;
     status.ensure(nc);
     PRBF_HIP(hipMemsetAsync(status, 0, sizeof(int) * nc, c->st));
     factor_blocks<float>(c, true, order, c->max_buffer, c->max_core, c->mptr, c->M32.p, status, true);
