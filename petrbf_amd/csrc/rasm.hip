@@ -35,17 +35,74 @@ template <typename ET>
 struct Vec16;
 template <>
 struct Vec16<float> {
-  using type = float4;
+  typedef float type __attribute__((ext_vector_type(4)));
 };
 template <>
 struct Vec16<double> {
-  using type = double2;
+  typedef double type __attribute__((ext_vector_type(2)));
 };
 
+// Residuals at this lane's buffer slots (one per slot group, up to kMaxGroups
+// groups, i.e. n <= 64 kMaxGroups) into the wavefront's LDS slice rs[g][lane]:
+// gathered once per cluster, before the core rows stream in, so the index ->
+// residual dependency is paid once.
+constexpr int kMaxGroups = 8;
+__device__ __forceinline__ void gather_slots(int n, int lane, const int* __restrict__ bi, const double* __restrict__ r,
+                                             double* __restrict__ rs) {
+  const int ngrp = (n + 63) / 64;
+  int idx[kMaxGroups];
+#pragma unroll
+  for (int g = 0; g < kMaxGroups; ++g) {
+    const int gg = g < ngrp ? g : ngrp - 1;
+    const int ng = min(64, n - 64 * gg);
+    idx[g] = bi[64 * gg + min(lane, ng - 1)];
+  }
+#pragma unroll
+  for (int g = 0; g < kMaxGroups; ++g) {
+    const int gg = g < ngrp ? g : ngrp - 1;
+    const int ng = min(64, n - 64 * gg);
+    const double v = r[idx[g]];
+    rs[g * 64 + lane] = (g < ngrp && lane < ng) ? v : 0.0;
+  }
+}
+
 // acc[t] += sum over this lane's slots of M[t][b] r[b], 16 rows of one chunk;
-// KG slot groups per pass (all loads of a pass are issued before the FMAs)
+// KG slot groups per pass (all loads of a pass are issued before the FMAs),
+// residuals from the LDS slice.  The core rows are read once per apply:
+// non-temporal loads keep them from displacing the residuals that
+// neighbouring clusters gather.
 template <typename ET, int KG>
-__device__ inline void chunk_times(const ET* __restrict__ Mk, int n, int lane, const int* __restrict__ bi,
+__device__ __forceinline__ void chunk_times(const ET* __restrict__ Mk, int n, int lane, const double* __restrict__ rs,
+                                            double (&acc)[kRowChunk]) {
+  constexpr int V = 16 / (int)sizeof(ET), QN = kRowChunk / V;
+  using VT = typename Vec16<ET>::type;
+  const int ngrp = (n + 63) / 64;
+  for (int g0 = 0; g0 < ngrp; g0 += KG) {
+    VT m[KG][QN];
+#pragma unroll
+    for (int k = 0; k < KG; ++k) {
+      const int g = g0 + k < ngrp ? g0 + k : ngrp - 1;
+      const int ng = min(64, n - 64 * g);
+      const VT* p = reinterpret_cast<const VT*>(Mk + 1024LL * g) + min(lane, ng - 1);
+#pragma unroll
+      for (int q = 0; q < QN; ++q) m[k][q] = __builtin_nontemporal_load(p + q * ng);
+    }
+#pragma unroll
+    for (int k = 0; k < KG; ++k) {
+      const double rb = g0 + k < ngrp ? rs[(g0 + k) * 64 + lane] : 0.0;
+#pragma unroll
+      for (int q = 0; q < QN; ++q) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[q * V + v] = fma((double)m[k][q][v], rb, acc[q * V + v]);
+      }
+    }
+  }
+}
+
+// The same for blocks of more than 64 kMaxGroups slots: indices and residuals
+// gathered pass by pass, KG slot groups per pass
+template <typename ET, int KG>
+__device__ __forceinline__ void chunk_times_gather(const ET* __restrict__ Mk, int n, int lane, const int* __restrict__ bi,
                                    const double* __restrict__ r, double (&acc)[kRowChunk]) {
   constexpr int V = 16 / (int)sizeof(ET), QN = kRowChunk / V;
   using VT = typename Vec16<ET>::type;
@@ -119,14 +176,21 @@ __device__ inline double wave_rows_sum(const double (&acc)[kRowChunk], int lane)
 }
 
 template <typename ET, int KG>
-__device__ inline void apply_cluster(const ET* __restrict__ Mc, int n, int T, int lane, const int* __restrict__ bi,
-                                     const double* __restrict__ r, double* __restrict__ zc) {
+__device__ __forceinline__ void apply_cluster(const ET* __restrict__ Mc, int n, int T, int lane, const int* __restrict__ bi,
+                                              const double* __restrict__ r, double* __restrict__ rs,
+                                              double* __restrict__ zc) {
   const int trow = 8 * ((lane >> 5) & 1) + 4 * ((lane >> 4) & 1) + 2 * ((lane >> 3) & 1) + ((lane >> 2) & 1);
+  const bool small = n <= 64 * kMaxGroups;
+  if (small) gather_slots(n, lane, bi, r, rs);
   for (int t0 = 0; t0 < T; t0 += kRowChunk) {
     double acc[kRowChunk];
 #pragma unroll
     for (int t = 0; t < kRowChunk; ++t) acc[t] = 0.0;
-    chunk_times<ET, KG>(Mc + (long long)(t0 / kRowChunk) * apply_chunk_elems(n), n, lane, bi, r, acc);
+    const ET* Mk = Mc + (long long)(t0 / kRowChunk) * apply_chunk_elems(n);
+    if (small)
+      chunk_times<ET, KG>(Mk, n, lane, rs, acc);
+    else
+      chunk_times_gather<ET, KG>(Mk, n, lane, bi, r, acc);
     const double s = wave_rows_sum(acc, lane);
     if ((lane & 3) == 0 && t0 + trow < T) zc[t0 + trow] = s;
   }
@@ -138,6 +202,7 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_rasm_apply(
     const int* __restrict__ ista, const int* __restrict__ iend, const long long* __restrict__ mptr,
     const MT* __restrict__ M, const long long* __restrict__ fb, const double* __restrict__ M64,
     const unsigned char* __restrict__ skip, const double* __restrict__ r, double* __restrict__ z) {
+  __shared__ double rs[kApplyWaves][kMaxGroups * 64];
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const long long c = (long long)xcd_block(blockIdx.x, gridDim.x) * kApplyWaves + wave;
   if (c >= ncell) return;
@@ -149,9 +214,9 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_rasm_apply(
   const int T = iend[c] - s + 1;
   const long long f = fb ? fb[c] : -1;
   if (f >= 0)
-    apply_cluster<double, 2>(M64 + f, n, T, lane, bidx + b0, r, z + s);
+    apply_cluster<double, 2>(M64 + f, n, T, lane, bidx + b0, r, rs[wave], z + s);
   else
-    apply_cluster<MT, sizeof(MT) == 4 ? 4 : 2>(M + mptr[c], n, T, lane, bidx + b0, r, z + s);
+    apply_cluster<MT, sizeof(MT) == 4 ? 4 : 2>(M + mptr[c], n, T, lane, bidx + b0, r, rs[wave], z + s);
 }
 
 static inline dim3 apply_grid(long long ncell) { return dim3((unsigned)((ncell + kApplyWaves - 1) / kApplyWaves)); }
