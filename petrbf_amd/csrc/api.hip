@@ -259,6 +259,27 @@ void build_lists(prbf_ctx* c, bool buffer) {
   (buffer ? c->buffer_nnz : c->trunc_nnz) = total;
 }
 
+// Buffer and truncation lists of one particle set in a single scan.
+void build_lists_same(prbf_ctx* c) {
+  const Grid& g = c->g;
+  c->i_scr1.ensure(g.ncell);
+  c->i_scr2.ensure(g.ncell);
+  c->bptr.ensure(g.ncell + 1);
+  c->tptr.ensure(g.ncell + 1);
+  lists2_count(c->st, g, c->ista, c->iend, c->xi, c->yi, c->zi, c->i_scr1, c->i_scr2);
+  scan_excl(c->st, g.ncell, c->i_scr1, c->bptr);
+  scan_excl(c->st, g.ncell, c->i_scr2, c->tptr);
+  long long tot[2] = {0, 0};
+  PRBF_HIP(hipMemcpyAsync(&tot[0], c->bptr.p + g.ncell, sizeof(long long), hipMemcpyDeviceToHost, c->st));
+  PRBF_HIP(hipMemcpyAsync(&tot[1], c->tptr.p + g.ncell, sizeof(long long), hipMemcpyDeviceToHost, c->st));
+  sync(c);
+  c->bidx.ensure(tot[0]);
+  c->tidx.ensure(tot[1]);
+  lists2_fill(c->st, g, c->ista, c->iend, c->xi, c->yi, c->zi, c->bptr, c->bidx, c->tptr, c->tidx);
+  c->buffer_nnz = tot[0];
+  c->trunc_nnz = tot[1];
+}
+
 void set_points_impl(prbf_ctx* c, long long ni, const double* xi, const double* yi, const double* zi,
                      long long nj, const double* xj, const double* yj, const double* zj) {
   const int dim = c->prm.dim;
@@ -312,8 +333,12 @@ void set_points_impl(prbf_ctx* c, long long ni, const double* xi, const double* 
   PRBF_HIP(hipMemcpyAsync(&herr, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->st));
   sync(c);
   need(herr == 0, PRBF_ERR_OUTSIDE, "a point fell outside the cell grid");
-  build_lists(c, true);
-  build_lists(c, false);
+  if (same) {
+    build_lists_same(c);
+  } else {
+    build_lists(c, true);
+    build_lists(c, false);
+  }
   // host copies for chunking and size limits
   c->h_bptr.resize(g.ncell + 1);
   c->h_ista.resize(g.ncell);

@@ -227,6 +227,101 @@ __global__ void k_list_fill(Grid g, int reach, double len, const int* __restrict
   visit_list<DIM>(g, c, reach, len, tsta, tend, sta, end, xs, ys, zs, [&](int j) { idx[k++] = j; });
 }
 
+// Both lists of cluster c in one scan, for a cluster structure whose targets
+// are its sources (the solve): the buffer list (reach rb, half-width lb) and
+// the truncation list (rt, lt), each in the order visit_list gives it, with
+// the same box tests.  The scan covers the larger reach; a cell outside one
+// list's reach contributes nothing to that list.
+template <int DIM, class EmitB, class EmitT>
+__device__ inline void visit_lists2(const Grid& g, long long c, int rb, double lb, int rt, double lt,
+                                    const int* __restrict__ sta, const int* __restrict__ end,
+                                    const double* __restrict__ xs, const double* __restrict__ ys,
+                                    const double* __restrict__ zs, EmitB&& eb, EmitT&& et) {
+  if (sta[c] > end[c]) return;
+  int ix, iy, iz;
+  cell_coords(g, c, &ix, &iy, &iz);
+  const double xc = cell_centre(g.xmin, ix, g.box);
+  const double yc = cell_centre(g.ymin, iy, g.box);
+  const double zc = DIM == 3 ? cell_centre(g.zmin, iz, g.box) : 0.0;
+  for (int j = sta[c]; j <= end[c]; ++j) {
+    eb(j);
+    et(j);
+  }
+  const int R = max(rb, rt);
+  const int x0 = max(0, ix - R), x1 = min(g.nx - 1, ix + R);
+  const int y0 = max(0, iy - R), y1 = min(g.ny - 1, iy + R);
+  const int z0 = DIM == 3 ? max(0, iz - R) : 0, z1 = DIM == 3 ? min(g.nz - 1, iz + R) : 0;
+  for (int jx = x0; jx <= x1; ++jx)
+    for (int jy = y0; jy <= y1; ++jy)
+      for (int jz = z0; jz <= z1; ++jz) {
+        if (jx == ix && jy == iy && jz == iz) continue;
+        const int ax = abs(jx - ix), ay = abs(jy - iy), az = abs(jz - iz);
+        const bool cb = ax <= rb && ay <= rb && az <= rb;
+        const bool ct = ax <= rt && ay <= rt && az <= rt;
+        if (!cb && !ct) continue;
+        const long long jc = cell_index(g, jx, jy, jz);
+        for (int j = sta[jc]; j <= end[jc]; ++j) {
+          const double dx = fabs(xs[j] - xc), dy = fabs(ys[j] - yc), dz = DIM == 3 ? fabs(zs[j] - zc) : 0.0;
+          bool ib = cb && dx < lb && dy < lb;
+          bool it = ct && dx < lt && dy < lt;
+          if (DIM == 3) {
+            ib = ib && dz < lb;
+            it = it && dz < lt;
+          }
+          if (ib) eb(j);
+          if (it) et(j);
+        }
+      }
+}
+
+template <int DIM>
+__global__ void k_lists2_count(Grid g, int rb, double lb, int rt, double lt, const int* __restrict__ sta,
+                               const int* __restrict__ end, const double* __restrict__ xs,
+                               const double* __restrict__ ys, const double* __restrict__ zs, int* __restrict__ cb,
+                               int* __restrict__ ct) {
+  long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.ncell) return;
+  int kb = 0, kt = 0;
+  visit_lists2<DIM>(g, c, rb, lb, rt, lt, sta, end, xs, ys, zs, [&](int) { ++kb; }, [&](int) { ++kt; });
+  cb[c] = kb;
+  ct[c] = kt;
+}
+
+template <int DIM>
+__global__ void k_lists2_fill(Grid g, int rb, double lb, int rt, double lt, const int* __restrict__ sta,
+                              const int* __restrict__ end, const double* __restrict__ xs,
+                              const double* __restrict__ ys, const double* __restrict__ zs,
+                              const long long* __restrict__ pb, int* __restrict__ ib, const long long* __restrict__ pt,
+                              int* __restrict__ it) {
+  long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.ncell) return;
+  long long kb = pb[c], kt = pt[c];
+  visit_lists2<DIM>(g, c, rb, lb, rt, lt, sta, end, xs, ys, zs, [&](int j) { ib[kb++] = j; },
+                    [&](int j) { it[kt++] = j; });
+}
+
+void lists2_count(hipStream_t st, const Grid& g, const int* sta, const int* end, const double* xs, const double* ys,
+                  const double* zs, int* count_b, int* count_t) {
+  int nb = (int)((g.ncell + 127) / 128);
+  if (g.dim == 2)
+    hipLaunchKernelGGL(k_lists2_count<2>, dim3(nb), dim3(128), 0, st, g, g.nb_buffer, g.buffer_len, g.nb_trunc,
+                       g.trunc_len, sta, end, xs, ys, zs, count_b, count_t);
+  else
+    hipLaunchKernelGGL(k_lists2_count<3>, dim3(nb), dim3(128), 0, st, g, g.nb_buffer, g.buffer_len, g.nb_trunc,
+                       g.trunc_len, sta, end, xs, ys, zs, count_b, count_t);
+}
+
+void lists2_fill(hipStream_t st, const Grid& g, const int* sta, const int* end, const double* xs, const double* ys,
+                 const double* zs, const long long* ptr_b, int* idx_b, const long long* ptr_t, int* idx_t) {
+  int nb = (int)((g.ncell + 127) / 128);
+  if (g.dim == 2)
+    hipLaunchKernelGGL(k_lists2_fill<2>, dim3(nb), dim3(128), 0, st, g, g.nb_buffer, g.buffer_len, g.nb_trunc,
+                       g.trunc_len, sta, end, xs, ys, zs, ptr_b, idx_b, ptr_t, idx_t);
+  else
+    hipLaunchKernelGGL(k_lists2_fill<3>, dim3(nb), dim3(128), 0, st, g, g.nb_buffer, g.buffer_len, g.nb_trunc,
+                       g.trunc_len, sta, end, xs, ys, zs, ptr_b, idx_b, ptr_t, idx_t);
+}
+
 void list_count(hipStream_t st, const Grid& g, int reach, double len, const int* tsta, const int* tend,
                 const int* sta, const int* end, const double* xs, const double* ys, const double* zs,
                 int* count) {

@@ -19,6 +19,12 @@ void list_fill(hipStream_t st, const Grid& g, int reach, double len, const int* 
                const int* sta, const int* end, const double* xs, const double* ys, const double* zs,
                const long long* ptr, int* idx);
 
+// both lists in one scan when the targets are the sources
+void lists2_count(hipStream_t st, const Grid& g, const int* sta, const int* end, const double* xs, const double* ys,
+                  const double* zs, int* count_b, int* count_t);
+void lists2_fill(hipStream_t st, const Grid& g, const int* sta, const int* end, const double* xs, const double* ys,
+                 const double* zs, const long long* ptr_b, int* idx_b, const long long* ptr_t, int* idx_t);
+
 // matvec.hip ------------------------------------------------------------------
 void matvec(hipStream_t st, int dim, int nrhs, long long ncell, const int* tsta, const int* tend,
             const long long* tptr, const int* tidx, const double* xi, const double* yi, const double* zi,
