@@ -132,7 +132,7 @@ struct prbf_ctx {
   DBuf<double> M64fb;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // block factorisation: work list, counter and workspace slots
-  DBuf<int> fb_order, fb_next, fb_status;
+  DBuf<int> fb_order, fb_next, fb_status, fb_failed;
   DBuf<unsigned char> fb_ws;
   // Arnoldi steps captured as graphs: device state of the cycle, one
   // executable graph per step index, and the launch arguments they captured
@@ -527,28 +527,40 @@ void rasm_setup_impl(prbf_ctx* c, int pc) {
     status.ensure(nc);
     PRBF_HIP(hipMemsetAsync(status, 0, sizeof(int) * nc, c->st));
     factor_blocks<float>(c, true, order, c->max_buffer, c->max_core, c->mptr, c->M32.p, status, true);
-    std::vector<int> h(nc);
-    PRBF_HIP(hipMemcpy(h.data(), status, sizeof(int) * nc, hipMemcpyDeviceToHost));
+    // the blocks whose FP32 factorisation broke down, listed on the device
     std::vector<int> failed;
-    for (int k : order)
-      if (h[k]) failed.push_back(k);
-    // FP64 redo of the blocks whose FP32 factorisation broke down
-    c->fb.ensure(nc);
-    std::vector<long long> h_fb(nc, -1), h_m64(nc + 1, 0);
-    long long tot = 0;
-    int nmax = 0, tmax = 0;
-    for (int k : failed) {
-      long long n = c->h_bptr[k + 1] - c->h_bptr[k];
-      long long T = c->h_iend[k] - c->h_ista[k] + 1;
-      h_fb[k] = tot;
-      h_m64[k] = tot;
-      tot += apply_elems((int)n, (int)T);
-      nmax = std::max<int>(nmax, (int)n);
-      tmax = std::max<int>(tmax, (int)T);
+    {
+      DBuf<int>& cnt = c->fb_next;  // the counter of the finished launch
+      c->fb_failed.ensure(std::max<size_t>(1, order.size()));
+      PRBF_HIP(hipMemsetAsync(cnt, 0, sizeof(int), c->st));
+      failed_blocks(c->st, (long long)order.size(), c->fb_order, status, c->fb_failed, cnt);
+      int nf = 0;
+      PRBF_HIP(hipMemcpyAsync(&nf, cnt, sizeof(int), hipMemcpyDeviceToHost, c->st));
+      sync(c);
+      if (nf > 0) {
+        failed.resize(nf);
+        PRBF_HIP(hipMemcpy(failed.data(), c->fb_failed, sizeof(int) * nf, hipMemcpyDeviceToHost));
+        std::sort(failed.begin(), failed.end());
+      }
     }
+    // FP64 redo of the blocks whose FP32 factorisation broke down; with none,
+    // the apply is given no per-cluster offsets at all
     c->n_fallback = (int)failed.size();
-    PRBF_HIP(hipMemcpyAsync(c->fb, h_fb.data(), sizeof(long long) * nc, hipMemcpyHostToDevice, c->st));
     if (!failed.empty()) {
+      c->fb.ensure(nc);
+      std::vector<long long> h_fb(nc, -1), h_m64(nc + 1, 0);
+      long long tot = 0;
+      int nmax = 0, tmax = 0;
+      for (int k : failed) {
+        long long n = c->h_bptr[k + 1] - c->h_bptr[k];
+        long long T = c->h_iend[k] - c->h_ista[k] + 1;
+        h_fb[k] = tot;
+        h_m64[k] = tot;
+        tot += apply_elems((int)n, (int)T);
+        nmax = std::max<int>(nmax, (int)n);
+        tmax = std::max<int>(tmax, (int)T);
+      }
+      PRBF_HIP(hipMemcpyAsync(c->fb, h_fb.data(), sizeof(long long) * nc, hipMemcpyHostToDevice, c->st));
       c->M64fb.ensure(tot);
       DBuf<long long> mp;
       mp.ensure(nc + 1);
@@ -586,12 +598,12 @@ void rasm_apply_impl(prbf_ctx* c, const double* r, double* z) {
   need(c->rasm_ready, PRBF_ERR_STATE, "rasm_setup first");
   if (c->rasm_pc == PRBF_PC_LEARNED) {
     need(c->learned_ready && c->net, PRBF_ERR_STATE, "prbf_learned_attach first");
-    rasm_apply_fp32(c->st, c->g.ncell, c->bptr, c->bidx, c->ista, c->iend, c->mptr, c->M32, c->fb, c->M64fb.p,
-                    c->skip, r, z);
+    rasm_apply_fp32(c->st, c->g.ncell, c->bptr, c->bidx, c->ista, c->iend, c->mptr, c->M32,
+                    c->n_fallback ? c->fb.p : nullptr, c->M64fb.p, c->skip, r, z);
     network_apply(c, c->mlp_list, c->n_mlp, r, z);
   } else if (c->rasm_pc == PRBF_PC_RASM_FP32)
-    rasm_apply_fp32(c->st, c->g.ncell, c->bptr, c->bidx, c->ista, c->iend, c->mptr, c->M32, c->fb, c->M64fb.p,
-                    nullptr, r, z);
+    rasm_apply_fp32(c->st, c->g.ncell, c->bptr, c->bidx, c->ista, c->iend, c->mptr, c->M32,
+                    c->n_fallback ? c->fb.p : nullptr, c->M64fb.p, nullptr, r, z);
   else
     rasm_apply_fp64(c->st, c->g.ncell, c->bptr, c->bidx, c->ista, c->iend, c->mptr, c->M, r, z, c->max_core);
   PRBF_HIP(hipGetLastError());
@@ -706,7 +718,7 @@ void ensure_graphs(prbf_ctx* c, int pc, bool flex, long long n, int m) {
       (unsigned long long)c->nj, (unsigned long long)c->prm.dim, sig, u(c->V.p), u(c->Z.p), u(c->v1.p),
       u(c->part.p), u(c->scal.p), u(c->kst.p), u(c->pin), u(c->ista.p), u(c->iend.p), u(c->tptr.p), u(c->tidx.p),
       u(c->xi.p), u(c->yi.p), u(c->zi.p), u(c->xj_()), u(c->yj_()), u(c->zj_()), u(c->bptr.p), u(c->bidx.p),
-      u(c->mptr.p), u(c->M.p), u(c->M32.p), u(c->fb.p), u(c->M64fb.p), u(c->skip.p), u(c->mlp_list.p),
+      u(c->mptr.p), u(c->M.p), u(c->M32.p), u(c->n_fallback ? c->fb.p : nullptr), u(c->M64fb.p), u(c->skip.p), u(c->mlp_list.p),
       (unsigned long long)c->n_mlp, u(c->pidx.p), u(c->cidx.p), u(c->pctr.p), u(c->lX.p), u(c->ls.p), u(c->net)};
   if (c->net) {
     unsigned long long os;

@@ -548,6 +548,22 @@ static void launch_block_chol(hipStream_t st, int dim, int nw, int npc, long lon
 #undef PRBF_BC
 }
 
+// list[0..*count) = the clusters of order[0..nwork) whose block reported a
+// breakdown (in no particular order; *count must be zero on entry)
+__global__ void k_failed_blocks(long long nwork, const int* __restrict__ order, const int* __restrict__ status,
+                                int* __restrict__ list, int* __restrict__ count) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nwork) return;
+  const int c = order[i];
+  if (status[c]) list[atomicAdd(count, 1)] = c;
+}
+
+void failed_blocks(hipStream_t st, long long nwork, const int* order, const int* status, int* list, int* count) {
+  if (nwork <= 0) return;
+  hipLaunchKernelGGL(k_failed_blocks, dim3((unsigned)((nwork + 255) / 256)), dim3(256), 0, st, nwork, order, status,
+                     list, count);
+}
+
 long long block_workspace(int n, int T) { return blk_ws(n, T); }
 
 size_t block_lds_bytes(bool fp32, int nmax, int tmax, int nw, int npc) {

@@ -54,6 +54,8 @@ __host__ __device__ inline long long apply_offset(int n, int t, int b) {
 // Workspace (elements) and LDS bytes of one block factorisation.
 long long block_workspace(int n, int T);
 size_t block_lds_bytes(bool fp32, int nmax, int tmax, int nw, int npc);
+// Clusters of order[0..nwork) with status[c] != 0 into list (unordered), their number in *count.
+void failed_blocks(hipStream_t st, long long nwork, const int* order, const int* status, int* list, int* count);
 // Workgroups of the persistent block factorisation that fit on the device at once.
 int block_chol_workgroups(bool fp32, int dim, int nw, int npc, int nmax, int tmax);
 // Factorises the blocks of clusters order[0..nwork) and writes their core rows:
