@@ -681,7 +681,10 @@ inline long long ks_rs(int m) { return (long long)(m + 1) * (m + 1) + 2 * (m + 1
 inline long long ks_inv(int m) { return ks_rs(m) + m + 2; }
 inline long long ks_out(int m) { return ks_inv(m) + 1; }
 constexpr int kPinStep = 256;  // pinned slots receiving a step's residual estimate and flags
-constexpr double kNetCycleDrop = 0.1;  // least core-residual reduction per cycle on the network
+// least core-residual reduction per restart cycle for a cluster to stay on the
+// network: 1000x (exact FP32 blocks reduce the residual by far more per cycle,
+// so a cluster that does not is moved at the first restart that exposes it)
+constexpr double kNetCycleDrop = 1e-3;
 
 // Enqueues Arnoldi step j of a cycle: the new basis vector, its Gram-Schmidt
 // coefficients and norm, the Givens update of H and the residual estimate,

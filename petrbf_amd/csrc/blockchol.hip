@@ -76,17 +76,6 @@ __device__ inline void tri_decode(int t, int* I, int* J) {
 
 __device__ inline void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
 
-// Value of v in lane l (l uniform) via readlane.
-__device__ inline double bcast(double v, int l) {
-  long long b = __double_as_longlong(v);
-  int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
-  int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-__device__ inline float bcast(float v, int l) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
-
 template <typename T>
 __device__ inline T gauss_entry(double dx, double dy, double dz, double inv_two_s2, double norm);
 template <>
@@ -97,6 +86,127 @@ template <>
 __device__ inline float gauss_entry<float>(double dx, double dy, double dz, double inv, double norm) {
   float a = (float)((dx * dx + dy * dy + dz * dz) * inv);
   return __expf(-a) * (float)norm;
+}
+
+// Value of v in lane P of this lane's 16-lane row (DPP row_newbcast:P, one
+// VALU move, no LDS and no readlane).
+template <int P>
+__device__ __forceinline__ float row_bcast(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x150 + P, 0xf, 0xf, true));
+}
+
+// 1/sqrt: the hardware instruction in FP32 (the FP32 blocks are a
+// preconditioner), the library's full-precision one in FP64.
+__device__ __forceinline__ float pivot_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double pivot_rsqrt(double x) { return rsqrt(x); }
+
+// Step P of the symmetric elimination on [D | I]: every 16-lane row of the
+// wavefront holds the whole 16x16 problem, lane i of the row holding row i.
+// Row P is read through row_newbcast; rows below P subtract
+// (a_iP / a_PP) x row P from both halves (rows at or above P multiply by 0),
+// and lane P keeps 1/sqrt(a_PP), the scale of its row of L^{-1}.
+template <typename Real, int P>
+struct DiagStep {
+  __device__ static __forceinline__ void run(Real (&a)[16], Real (&e)[16], Real& rs, bool& bad, int i) {
+    e[P] = (i == P) ? Real(1) : Real(0);  // column P of the right half is untouched until now
+    const Real piv = row_bcast<P>(a[P]);
+    const bool b = !(piv > Real(0));
+    bad |= b;
+    const Real rinv = pivot_rsqrt(b ? Real(1) : piv);  // (branch-free: one block per step)
+    rs = i == P ? rinv : rs;
+    const Real m = i > P ? a[P] * (rinv * rinv) : Real(0);
+    // one broadcast value alive at a time (the scheduler would otherwise
+    // hoist all sixteen moves and push the caller's registers into spills)
+#pragma unroll
+    for (int q = P + 1; q < 16; ++q) {
+      a[q] -= m * row_bcast<P>(a[q]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int q = 0; q <= P; ++q) {
+      e[q] -= m * row_bcast<P>(e[q]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    DiagStep<Real, P + 1>::run(a, e, rs, bad, i);
+  }
+};
+template <typename Real>
+struct DiagStep<Real, 16> {
+  __device__ static __forceinline__ void run(Real (&)[16], Real (&)[16], Real&, bool&, int) {}
+};
+
+// Value of v in lane l (l uniform) via readlane.
+__device__ inline double bcast(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// L^{-1} of the 16x16 diagonal tile D (row-major, row stride DS) into Li (row
+// stride OS, zero above the diagonal), by one wavefront.  Returns true (in
+// every lane) when a pivot is not positive.
+//
+// FP32: the elimination of DiagStep on every 16-lane row at once, without
+// scaling rows as it goes (row i of L^{-1} is the right half's row i times
+// 1/sqrt of its pivot, applied at the end).
+//
+// FP64: the same elimination on lanes 0-15 with readlane broadcasts of row p
+// and row p scaled at its step: the DPP form needs ~30 more VGPRs in FP64,
+// which the FP64 factorisation (at its 256-VGPR cap) would spill.
+template <typename Real, int DS, int OS>
+__device__ __forceinline__ bool diag_inverse(const Real* D, Real* Li, int lane) {
+  if constexpr (sizeof(Real) == 4) {
+    const int i = lane & 15;
+    Real a[16], e[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) a[q] = D[i * DS + q];
+    Real rs = Real(1);
+    bool bad = false;
+    DiagStep<Real, 0>::run(a, e, rs, bad, i);
+    if (lane < 16) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) Li[i * OS + q] = q <= i ? e[q] * rs : Real(0);
+    }
+    return bad;
+  } else {
+    Real ra[16], re[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      ra[q] = lane < 16 ? D[lane * DS + q] : Real(0);
+      re[q] = (q == lane) ? Real(1) : Real(0);
+    }
+    bool bad = false;
+#pragma unroll
+    for (int p = 0; p < 16; ++p) {
+      const Real piv = bcast(ra[p], p);
+      const bool b = !(piv > Real(0));
+      bad |= b;
+      const Real rinv = b ? Real(1) : pivot_rsqrt(piv);
+      const Real m = ra[p] * rinv;
+      if (lane == p) {
+#pragma unroll
+        for (int q = p; q < 16; ++q) ra[q] *= rinv;
+#pragma unroll
+        for (int q = 0; q <= p; ++q) re[q] *= rinv;
+      }
+#pragma unroll
+      for (int q = p + 1; q < 16; ++q) {
+        const Real v = bcast(ra[q], p);
+        if (lane > p) ra[q] -= m * v;
+      }
+#pragma unroll
+      for (int q = 0; q <= p; ++q) {
+        const Real v = bcast(re[q], p);
+        if (lane > p) re[q] -= m * v;
+      }
+    }
+    if (lane < 16) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) Li[lane * OS + q] = q <= lane ? re[q] : Real(0);
+    }
+    return bad;
+  }
 }
 
 // Workspace of one block, in elements of T:
@@ -220,47 +330,7 @@ __device__ void factor_block(
           for (int r = 0; r < 4; ++r) D[MF<Real>::row(r, lq) * kLS + lr] = tkk[r * 64 + lane];
         }
         wave_fence();
-        // L_kk^{-1} directly: symmetric elimination on [A_kk | I] with row i in
-        // the registers of lane i; step p scales row p by 1/sqrt(pivot) and
-        // removes column p from the rows below.  The left half becomes L^T,
-        // the right half L^{-1}; only the latter is kept.  Broadcasts of row
-        // p are readlanes.
-        Real ra[16], re[16];
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-          ra[q] = lane < 16 ? D[lane * kLS + q] : Real(0);
-          re[q] = (q == lane) ? Real(1) : Real(0);
-        }
-        bool badany = false;
-#pragma unroll
-        for (int p = 0; p < 16; ++p) {
-          const Real piv = bcast(ra[p], p);
-          const bool bad = !(piv > Real(0));
-          badany |= bad;
-          const Real rinv = bad ? Real(1) : rsqrt(piv);
-          const Real m = ra[p] * rinv;
-          if (lane == p) {
-#pragma unroll
-            for (int q = p; q < 16; ++q) ra[q] *= rinv;
-#pragma unroll
-            for (int q = 0; q <= p; ++q) re[q] *= rinv;
-          }
-#pragma unroll
-          for (int q = p + 1; q < 16; ++q) {
-            const Real v = bcast(ra[q], p);
-            if (lane > p) ra[q] -= m * v;
-          }
-#pragma unroll
-          for (int q = 0; q <= p; ++q) {
-            const Real v = bcast(re[q], p);
-            if (lane > p) re[q] -= m * v;
-          }
-        }
-        if (lane == 0 && badany) *flag = 1;
-        if (lane < 16) {
-#pragma unroll
-          for (int q = 0; q < 16; ++q) Li[lane * kLS + q] = q <= lane ? re[q] : Real(0);
-        }
+        if (diag_inverse<Real, kLS, kLS>(D, Li, lane) && lane == 0) *flag = 1;
         wave_fence();
         Real* wd = Wd + (long long)kk * kTE;
 #pragma unroll
