@@ -64,6 +64,45 @@ constexpr int kTE = 256;   // elements per 16x16 tile
 constexpr int kLS = 17;    // LDS row stride (bank-conflict padding)
 constexpr int kLT = 16 * kLS;
 
+// Panel tiles (the current tile columns of L, read by every trailing
+// update) and the product -P_i P_j^T of two of them into an accumulator.
+// FP32: row stride 16 in 16-byte chunks, chunk c of row r stored at chunk
+// c ^ f(r), f(r) = ((r >> 1) + (r >> 3)) & 3.  Lane (lr, lq) reads chunk lq
+// of row lr of both tiles as one 16-byte load each and runs the four K steps
+// with K index 4 lq + q on both operands; the chunk swizzle keeps these loads
+// free of bank conflicts with 32 or 64 banks.  Dropping the padding column
+// of the stride-17 layout is what lets three FP32 workgroups share a CU's LDS.
+// FP64: stride 17, one 8-byte load per K step.
+template <typename Real>
+struct Pan;
+template <>
+struct Pan<float> {
+  static constexpr int kT = 256;
+  __device__ static inline int chunk(int r) { return ((r >> 1) + (r >> 3)) & 3; }
+  __device__ static inline int idx(int r, int c) { return r * 16 + (((c >> 2) ^ chunk(r)) << 2) + (c & 3); }
+  __device__ static inline MF<float>::v4 minus_prod(const float* pi, const float* pj, int lr, int lq,
+                                                    MF<float>::v4 acc) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const int o = lr * 16 + ((lq ^ chunk(lr)) << 2);
+    const f4 a = *(const f4*)(pi + o);
+    const f4 b = *(const f4*)(pj + o);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc = MF<float>::mma(-a[q], b[q], acc);
+    return acc;
+  }
+};
+template <>
+struct Pan<double> {
+  static constexpr int kT = kLT;
+  __device__ static inline int idx(int r, int c) { return r * kLS + c; }
+  __device__ static inline MF<double>::v4 minus_prod(const double* pi, const double* pj, int lr, int lq,
+                                                     MF<double>::v4 acc) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc = MF<double>::mma(-pi[lr * kLS + 4 * q + lq], pj[lr * kLS + 4 * q + lq], acc);
+    return acc;
+  }
+};
+
 __device__ inline int ltile(int I, int J) { return I * (I + 1) / 2 + J; }
 
 __device__ inline void tri_decode(int t, int* I, int* J) {
@@ -221,13 +260,13 @@ template <typename T>
 __host__ __device__ inline size_t blk_lds_bytes(int nmax, int tmax, int nw, int npc) {
   int KT = (nmax + 15) / 16, NP = 16 * KT;
   (void)tmax;
-  size_t pan = (size_t)npc * KT * kLT * sizeof(T);
+  size_t pan = (size_t)npc * KT * Pan<T>::kT * sizeof(T);
   size_t crd = (size_t)3 * NP * sizeof(double);
-  size_t red = (size_t)NP * kLS * sizeof(T);  // phase-2 right-hand sides
+  size_t red = ((size_t)NP * kLS + (size_t)nw * kLT) * sizeof(T);  // phase-2 right-hand sides, per-wave scratch
   size_t region = pan > crd ? pan : crd;
   region = region > red ? region : red;
   region = (region + 15) & ~(size_t)15;
-  return region + (size_t)(nw + 2) * kLT * sizeof(T) + 16;
+  return region + (size_t)2 * kLT * sizeof(T) + 16;
 }
 
 // One block: cluster c, workspace Wb (blk_ws elements), LDS as laid out below.
@@ -251,18 +290,21 @@ __device__ void factor_block(
 
   Real* Wd = Wb + (long long)nt * kTE;       // L_II^{-1}, row-major
 
-  // LDS carve-up
-  size_t pan = (size_t)NPC * KT * kLT * sizeof(Real), crd = (size_t)3 * NP * sizeof(double),
-         red = (size_t)NP * kLS * sizeof(Real);
+  // LDS carve-up: one region holding, by phase, the coordinates (0), the
+  // panel (1), and the right-hand sides X with the per-wave scratch tiles
+  // (2, 3); then the diagonal tile, its inverse and the breakdown flag
+  typedef Pan<Real> PT;
+  size_t pan = (size_t)NPC * KT * PT::kT * sizeof(Real), crd = (size_t)3 * NP * sizeof(double),
+         red = ((size_t)NP * kLS + (size_t)NW * kLT) * sizeof(Real);
   size_t region = pan > crd ? pan : crd;
   region = region > red ? region : red;
   region = (region + 15) & ~(size_t)15;
-  Real* P = (Real*)lds_raw;                     // panel: NPC x KT tiles, row-major stride 17
+  Real* P = (Real*)lds_raw;                     // panel: NPC x KT tiles (PT layout)
   double* cx = (double*)lds_raw;          // phase 0 coordinates (same region)
   double* cy = cx + NP;
   double* cz = cy + NP;
-  Real* S = (Real*)(lds_raw + region);          // per-wave scratch tile
-  Real* D = S + NW * kLT;                    // diagonal tile L_kk
+  Real* S = (Real*)lds_raw + (size_t)NP * kLS;  // per-wave scratch tiles, past X
+  Real* D = (Real*)(lds_raw + region);       // diagonal tile L_kk
   Real* Li = D + kLT;                        // L_kk^{-1}
   int* flag = (int*)(Li + kLT);
   Real* Sw = S + wave * kLT;
@@ -308,11 +350,7 @@ __device__ void factor_block(
 #pragma unroll
           for (int r = 0; r < 4; ++r) acc[r] = tik[r * 64 + lane];
           for (int pc = 0; pc < cc; ++pc) {
-            const Real* pi = P + (pc * KT + i) * kLT;
-            const Real* pk = P + (pc * KT + kk) * kLT;
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-              acc = MF<Real>::mma(-pi[lr * kLS + 4 * q + lq], pk[lr * kLS + 4 * q + lq], acc);
+            acc = PT::minus_prod(P + (pc * KT + i) * PT::kT, P + (pc * KT + kk) * PT::kT, lr, lq, acc);
           }
 #pragma unroll
           for (int r = 0; r < 4; ++r) tik[r * 64 + lane] = acc[r];
@@ -361,11 +399,11 @@ __device__ void factor_block(
 #pragma unroll
             for (int q = 0; q < 4; ++q) acc = MF<Real>::mma(av[g][q], Li[lr * kLS + 4 * q + lq], acc);
             Real* tik = Wb + (long long)ltile(i, kk) * kTE;
-            Real* pi = P + (cc * KT + i) * kLT;
+            Real* pi = P + (cc * KT + i) * PT::kT;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
               tik[r * 64 + lane] = acc[r];
-              pi[MF<Real>::row(r, lq) * kLS + lr] = acc[r];
+              pi[PT::idx(MF<Real>::row(r, lq), lr)] = acc[r];
             }
           }
         }
@@ -373,11 +411,11 @@ __device__ void factor_block(
       __syncthreads();
     }
     // trailing update: A_ij -= sum_c L_ic L_jc^T for k0+npc <= j <= i < KT,
-    // kG tiles in flight per wavefront
+    // kG tiles in flight per wavefront (FP32: 4, within its 168-VGPR cap)
     const int j0 = k0 + npc;
     const int m = KT - j0;
     const int ntr = m * (m + 1) / 2;
-    constexpr int kG = 8;
+    constexpr int kG = sizeof(Real) == 4 ? 4 : 8;
     for (int u0 = wave; u0 < ntr; u0 += NW * kG) {
       v4 acc[kG];
       Real* tp[kG];
@@ -400,11 +438,7 @@ __device__ void factor_block(
       for (int g = 0; g < kG; ++g) {
         if (tp[g]) {
           for (int pc = 0; pc < npc; ++pc) {
-            const Real* pi = P + (pc * KT + ti[g]) * kLT;
-            const Real* pj = P + (pc * KT + tj[g]) * kLT;
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-              acc[g] = MF<Real>::mma(-pi[lr * kLS + 4 * q + lq], pj[lr * kLS + 4 * q + lq], acc[g]);
+            acc[g] = PT::minus_prod(P + (pc * KT + ti[g]) * PT::kT, P + (pc * KT + tj[g]) * PT::kT, lr, lq, acc[g]);
           }
 #pragma unroll
           for (int r = 0; r < 4; ++r) tp[g][r * 64 + lane] = acc[g][r];
