@@ -159,8 +159,11 @@ prbf_status prbf_rasm_setup(prbf_ctx* ctx);
  * counterpart for FP32: it is the mixed-precision variant of the same PC. */
 prbf_status prbf_rasm_setup_pc(prbf_ctx* ctx, int pc);
 /* Force a launch configuration of the block factorisation instead of the
- * built-in autotuner: PRBF_TUNE_CHOL_WAVES (4 or 8 wavefronts per workgroup),
- * PRBF_TUNE_CHOL_PANEL (1, 2 or 4 16-wide tile columns per panel).  No
+ * built-in autotuner: PRBF_TUNE_CHOL_WAVES (4, 6 or 8 wavefronts per
+ * workgroup), PRBF_TUNE_CHOL_PANEL (1 to 4 16-wide tile columns per panel).
+ * FP32 has every pair; FP64 has 4 or 8 wavefronts with 1, 2 or 4 columns, and
+ * a forced pair without an FP64 kernel, or whose panel does not fit in LDS,
+ * runs as 4 wavefronts with 1 column.  No
  * reference counterpart. */
 prbf_status prbf_set_tuning(prbf_ctx* ctx, int key, int value);
 /* PCApply_ASM (restrict): z = sum_c P_core A_c^{-1} R_buffer r, sorted order. */

@@ -374,7 +374,10 @@ struct CholCfg {
 
 // Launch configurations the tuner chooses among (wavefronts per workgroup,
 // tile columns per panel).
-const CholCfg kCholCfgs[] = {{4, 1}, {8, 1}, {4, 2}, {8, 2}, {4, 4}, {8, 4}};
+// (FP32 also 6 wavefronts and 3-column panels; a candidate without a kernel
+// for the precision is skipped.)
+const CholCfg kCholCfgs[] = {{4, 1}, {8, 1}, {4, 2}, {8, 2}, {4, 4}, {8, 4},
+                             {4, 3}, {6, 1}, {6, 2}, {6, 3}, {6, 4}, {8, 3}};
 
 // Process-wide cache of tuned configurations, keyed by precision and the
 // largest block's tile count.
@@ -445,6 +448,7 @@ void factor_blocks(prbf_ctx* c, bool fp32, const std::vector<int>& order, int nm
     float best = 1e30f;
     for (const CholCfg& cand : kCholCfgs) {
       if (block_lds_bytes(fp32, nmax, tmax, cand.nw, cand.npc) > kMaxLdsBytes) continue;
+      if (!block_chol_has_config(fp32, cand.nw, cand.npc)) continue;
       const int nwg = workgroups(cand);
       W.ensure((size_t)nwg * ws_slot);
       PRBF_HIP(hipEventRecord(c->ev0, c->st));
@@ -457,7 +461,9 @@ void factor_blocks(prbf_ctx* c, bool fp32, const std::vector<int>& order, int nm
     }
     tuned_cache().push_back({key, cfg});
   }
-  if (block_lds_bytes(fp32, nmax, tmax, cfg.nw, cfg.npc) > kMaxLdsBytes) cfg = CholCfg{4, 1};
+  if (block_lds_bytes(fp32, nmax, tmax, cfg.nw, cfg.npc) > kMaxLdsBytes ||
+      !block_chol_has_config(fp32, cfg.nw, cfg.npc))
+    cfg = CholCfg{4, 1};
   need(block_lds_bytes(fp32, nmax, tmax, cfg.nw, cfg.npc) <= kMaxLdsBytes, PRBF_ERR_LIMIT,
        fp32 ? "RASM block too large for the LDS panel of the block factorisation"
             : "RASM block too large for the LDS panel of the FP64 block factorisation");
@@ -1227,10 +1233,10 @@ prbf_status prbf_rasm_setup_pc(prbf_ctx* c, int pc) {
 prbf_status prbf_set_tuning(prbf_ctx* c, int key, int value) {
   return guarded(c, [&] {
     if (key == PRBF_TUNE_CHOL_WAVES) {
-      need(value == 4 || value == 8, PRBF_ERR_ARG, "waves per block factorisation must be 4 or 8");
+      need(value == 4 || value == 6 || value == 8, PRBF_ERR_ARG, "waves per block factorisation must be 4, 6 or 8");
       c->chol_waves = value;
     } else if (key == PRBF_TUNE_CHOL_PANEL) {
-      need(value == 1 || value == 2 || value == 4, PRBF_ERR_ARG, "panel width must be 1, 2 or 4 tile columns");
+      need(value >= 1 && value <= 4, PRBF_ERR_ARG, "panel width must be 1 to 4 tile columns");
       c->chol_panel = value;
     } else {
       need(false, PRBF_ERR_ARG, "unknown tuning key");

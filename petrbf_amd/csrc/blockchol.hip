@@ -28,6 +28,9 @@
 // T = double is the plain path (exact FP64 block solves); T = float runs the
 // same kernel in FP32 on the FP32 MFMA (mixed precision), with a per-block
 // status so a block whose FP32 factorisation breaks down is redone in FP64.
+#include <vector>
+
+#include "../../include/petrbf.h"
 #include "common.h"
 #include "kernels.h"
 
@@ -588,39 +591,43 @@ __global__ __launch_bounds__(NW * 64, sizeof(Real) == 4 ? 3 : 2) void k_block_ch
   }
 }
 
-template <typename T>
-static void set_lds_attr() {
-  static bool attr = false;
-  if (!attr) {
-    const void* fns[] = {(const void*)k_block_chol<T, 2, 4, 1>, (const void*)k_block_chol<T, 3, 4, 1>,
-                         (const void*)k_block_chol<T, 2, 8, 1>, (const void*)k_block_chol<T, 3, 8, 1>,
-                         (const void*)k_block_chol<T, 2, 4, 2>, (const void*)k_block_chol<T, 3, 4, 2>,
-                         (const void*)k_block_chol<T, 2, 8, 2>, (const void*)k_block_chol<T, 3, 8, 2>,
-                         (const void*)k_block_chol<T, 2, 4, 4>, (const void*)k_block_chol<T, 3, 4, 4>,
-                         (const void*)k_block_chol<T, 2, 8, 4>, (const void*)k_block_chol<T, 3, 8, 4>};
-    for (const void* f : fns)
-      PRBF_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    attr = true;
+// Instantiated launch configurations (wavefronts per workgroup, tile columns
+// per panel): FP32 also has 6-wavefront workgroups (two per CU fill its
+// three wavefronts per SIMD) and 3-column panels (two per CU in LDS).
+template <typename T, int D>
+static const void* chol_fn_dim(int nw, int npc) {
+#define PRBF_FN(NWV, NPCV) \
+  if (nw == NWV && npc == NPCV) return (const void*)k_block_chol<T, D, NWV, NPCV>;
+  PRBF_FN(4, 1) PRBF_FN(4, 2) PRBF_FN(4, 4) PRBF_FN(8, 1) PRBF_FN(8, 2) PRBF_FN(8, 4)
+  if constexpr (sizeof(T) == 4) {
+    PRBF_FN(4, 3) PRBF_FN(6, 1) PRBF_FN(6, 2) PRBF_FN(6, 3) PRBF_FN(6, 4) PRBF_FN(8, 3)
   }
+#undef PRBF_FN
+  return nullptr;
 }
 
 template <typename T>
 static const void* chol_fn(int dim, int nw, int npc) {
-#define PRBF_FN(D, NWV, NPCV) (const void*)k_block_chol<T, D, NWV, NPCV>
-#define PRBF_FN_NPC(D, NWV) \
-  (npc == 4 ? PRBF_FN(D, NWV, 4) : npc == 2 ? PRBF_FN(D, NWV, 2) : PRBF_FN(D, NWV, 1))
-  if (dim == 2) return nw == 8 ? PRBF_FN_NPC(2, 8) : PRBF_FN_NPC(2, 4);
-  return nw == 8 ? PRBF_FN_NPC(3, 8) : PRBF_FN_NPC(3, 4);
-#undef PRBF_FN_NPC
-#undef PRBF_FN
+  return dim == 3 ? chol_fn_dim<T, 3>(nw, npc) : chol_fn_dim<T, 2>(nw, npc);
+}
+
+// Raises the dynamic-LDS limit of a kernel to the CU's 160 KB, once.
+static void set_lds_attr(const void* fn) {
+  static std::vector<const void*> done;
+  for (const void* f : done)
+    if (f == fn) return;
+  PRBF_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+  done.push_back(fn);
 }
 
 template <typename T>
 static int resident_workgroups(int dim, int nw, int npc, int nmax, int tmax) {
-  set_lds_attr<T>();
+  const void* fn = chol_fn<T>(dim, nw, npc);
+  if (!fn) return 0;
+  set_lds_attr(fn);
   size_t lds = blk_lds_bytes<T>(nmax, tmax, nw, npc);
   int per_cu = 0, dev = 0, ncu = 0;
-  PRBF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, chol_fn<T>(dim, nw, npc), nw * 64, lds));
+  PRBF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, nw * 64, lds));
   PRBF_HIP(hipGetDevice(&dev));
   PRBF_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   return (per_cu < 1 ? 1 : per_cu) * ncu;
@@ -631,25 +638,17 @@ static void launch_block_chol(hipStream_t st, int dim, int nw, int npc, long lon
                               int nwg, long long ws_slot, int nmax, int tmax, const long long* bptr, const int* bidx,
                               const int* ista, const int* iend, const double* xs, const double* ys, const double* zs,
                               T* W, const long long* mptr, T* M, double sigma, int* err, int* blk_status) {
+  const void* fn = chol_fn<T>(dim, nw, npc);
+  if (!fn) throw ApiError(PRBF_ERR_ARG, "no block factorisation kernel for this launch configuration");
   Kern k = make_kern(sigma);
-  set_lds_attr<T>();
+  set_lds_attr(fn);
   size_t lds = blk_lds_bytes<T>(nmax, tmax, nw, npc);
-  const size_t item_slot = lds - 8;  // past the flag word of the largest block
+  size_t item_slot = lds - 8;  // past the flag word of the largest block
   PRBF_HIP(hipMemsetAsync(next, 0, sizeof(int), st));
-  dim3 grid((unsigned)nwg);
-#define PRBF_BC(D, NWV, NPCV)                                                                                     \
-  hipLaunchKernelGGL((k_block_chol<T, D, NWV, NPCV>), grid, dim3(NWV * 64), lds, st, nwork, order, next, ws_slot, \
-                     bptr, bidx, ista, iend, xs, ys, zs, W, mptr, M, k.inv_two_s2, k.norm, item_slot, err,         \
-                     blk_status)
-#define PRBF_BC_NPC(D, NWV) \
-  if (npc == 4) PRBF_BC(D, NWV, 4); else if (npc == 2) PRBF_BC(D, NWV, 2); else PRBF_BC(D, NWV, 1)
-  if (dim == 2) {
-    if (nw == 8) { PRBF_BC_NPC(2, 8); } else { PRBF_BC_NPC(2, 4); }
-  } else {
-    if (nw == 8) { PRBF_BC_NPC(3, 8); } else { PRBF_BC_NPC(3, 4); }
-  }
-#undef PRBF_BC_NPC
-#undef PRBF_BC
+  double inv_two_s2 = k.inv_two_s2, norm = k.norm;
+  void* args[] = {&nwork, &order, &next, &ws_slot, &bptr, &bidx,       &ista, &iend, &xs,  &ys,
+                  &zs,    &W,     &mptr, &M,       &inv_two_s2, &norm, &item_slot, &err, &blk_status};
+  PRBF_HIP(hipLaunchKernel(fn, dim3((unsigned)nwg), dim3(nw * 64), args, lds, st));
 }
 
 // list[0..*count) = the clusters of order[0..nwork) whose block reported a
@@ -669,6 +668,10 @@ void failed_blocks(hipStream_t st, long long nwork, const int* order, const int*
 }
 
 long long block_workspace(int n, int T) { return blk_ws(n, T); }
+
+bool block_chol_has_config(bool fp32, int nw, int npc) {
+  return (fp32 ? chol_fn<float>(2, nw, npc) : chol_fn<double>(2, nw, npc)) != nullptr;
+}
 
 size_t block_lds_bytes(bool fp32, int nmax, int tmax, int nw, int npc) {
   return fp32 ? blk_lds_bytes<float>(nmax, tmax, nw, npc) : blk_lds_bytes<double>(nmax, tmax, nw, npc);

@@ -57,6 +57,8 @@ size_t block_lds_bytes(bool fp32, int nmax, int tmax, int nw, int npc);
 // Clusters of order[0..nwork) with status[c] != 0 into list (unordered), their number in *count.
 void failed_blocks(hipStream_t st, long long nwork, const int* order, const int* status, int* list, int* count);
 // Workgroups of the persistent block factorisation that fit on the device at once.
+// Whether the block factorisation has a kernel for (precision, wavefronts, panel columns).
+bool block_chol_has_config(bool fp32, int nw, int npc);
 int block_chol_workgroups(bool fp32, int dim, int nw, int npc, int nmax, int tmax);
 // Factorises the blocks of clusters order[0..nwork) and writes their core rows:
 // nwg persistent workgroups, each with a workspace slot of ws_slot elements

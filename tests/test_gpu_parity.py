@@ -238,15 +238,17 @@ def test_mixed_core_rows_close_to_fp64():
 
 @pytest.mark.parametrize("prec", ["rasm", "rasm32"])
 def test_block_factorisation_launch_configs_agree(prec):
-    """Every launch configuration the autotuner may pick (4/8 wavefronts,
-    1/2/4 tile columns per panel) yields the same preconditioner."""
+    """Every launch configuration the autotuner may pick (FP64: 4/8
+    wavefronts, 1/2/4 tile columns per panel; FP32 also 6 wavefronts and
+    3-column panels) yields the same preconditioner."""
     _announce(f"chol configs {prec}")
     p = lattice(40)
     rng = np.random.default_rng(2)
     r = rng.standard_normal(p.n)
     outs = []
-    for nw in (4, 8):
-        for npc in (1, 2, 4):
+    fp32 = prec == "rasm32"
+    for nw in ((4, 6, 8) if fp32 else (4, 8)):
+        for npc in ((1, 2, 3, 4) if fp32 else (1, 2, 4)):
             ctx = Context.from_problem(p)
             ctx.set_tuning(chol_waves=nw, chol_panel=npc)
             ctx.set_points(p.x, p.y)
