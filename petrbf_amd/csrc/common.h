@@ -108,6 +108,25 @@ __device__ inline double exp_nonpos(double x) {
   return ldexp(p, (int)kf);
 }
 
+// exp(x) for -700 < x <= 0 in FP64, table-driven: x = (64 m + j) ln2/64 + r
+// with a two-part ln2/64 (high part 33 significant bits), |r| <= ln2/128,
+// e^r by its degree-5 Taylor polynomial (truncation < 4e-17 relative), times
+// tab[j] = 2^(j/64) and 2^m.  Within 3 ulp of exp over [-80, 0] (checked
+// against long double expl on 2e7 points); 11 FP64 operations against the
+// 17 of exp_nonpos.  tab is a 64-entry table in LDS.
+__device__ inline double exp_nonpos_tab(double x, const double* tab) {
+  const double kf = rint(x * 92.33248261689366);
+  double r = fma(-kf, 0.010830424695086549, x);
+  r = fma(-kf, 1.162596423439437e-12, r);
+  const int k = (int)kf;
+  double p = fma(r, 0.008333333333333333, 0.041666666666666664);
+  p = fma(p, r, 0.16666666666666666);
+  p = fma(p, r, 0.5);
+  p = fma(p, r, 1.0);
+  p = fma(p, r, 1.0);
+  return ldexp(p * tab[k & 63], k >> 6);
+}
+
 }  // namespace prbf
 
 #define PRBF_HIP(call)                                                         \

@@ -8,7 +8,8 @@
 // 2d/get_trunc.h:63) is staged in LDS as SoA FP64 tiles; each lane owns two
 // targets and a residue class of the source tile, accumulates in FP64 with
 // FMA, and the per-target partial sums are combined in a fixed order, so the
-// result is deterministic.  FP64 VALU-bound: one exp (exp_nonpos) per pair.
+// result is deterministic.  FP64 VALU-bound: one exp (exp_nonpos_tab, with
+// its 2^(j/64) table in LDS) per pair.
 #include "common.h"
 #include "kernels.h"
 
@@ -30,6 +31,7 @@ __global__ __launch_bounds__(64) void k_matvec_wave(
   __shared__ double sx[kTile], sy[kTile], sz[DIM == 3 ? kTile : 1];
   __shared__ double sg[NRHS][kTile];
   __shared__ double part[NRHS][128];
+  __shared__ double etab[64];
   const long long c = blockIdx.x;
   const int lane = threadIdx.x;
   const int t0 = tsta[c], t1 = tend[c];
@@ -37,6 +39,7 @@ __global__ __launch_bounds__(64) void k_matvec_wave(
   const long long s0 = tptr[c], s1 = tptr[c + 1];
   const int T = t1 - t0 + 1;
   const double nc = -inv_two_s2;
+  etab[lane] = exp2(lane * (1.0 / 64));  // (visible after the first tile's barrier)
   for (int tb = 0; tb < T; tb += 128) {
     const int Tc = min(128, T - tb);
     const int TP = (Tc + 1) / 2;       // target pairs
@@ -86,8 +89,8 @@ __global__ __launch_bounds__(64) void k_matvec_wave(
             ra2 = fma(az - z2, az - z2, ra2);
             rb2 = fma(bz - z2, bz - z2, rb2);
           }
-          const double ea1 = exp_nonpos(ra1 * nc), eb1 = exp_nonpos(rb1 * nc);
-          const double ea2 = exp_nonpos(ra2 * nc), eb2 = exp_nonpos(rb2 * nc);
+          const double ea1 = exp_nonpos_tab(ra1 * nc, etab), eb1 = exp_nonpos_tab(rb1 * nc, etab);
+          const double ea2 = exp_nonpos_tab(ra2 * nc, etab), eb2 = exp_nonpos_tab(rb2 * nc, etab);
 #pragma unroll
           for (int r = 0; r < NRHS; ++r) {
             const double g1 = sg[r][k], g2 = sg[r][k2];
@@ -106,7 +109,7 @@ __global__ __launch_bounds__(64) void k_matvec_wave(
             ra1 = fma(az - z1, az - z1, ra1);
             rb1 = fma(bz - z1, bz - z1, rb1);
           }
-          const double ea1 = exp_nonpos(ra1 * nc), eb1 = exp_nonpos(rb1 * nc);
+          const double ea1 = exp_nonpos_tab(ra1 * nc, etab), eb1 = exp_nonpos_tab(rb1 * nc, etab);
 #pragma unroll
           for (int r = 0; r < NRHS; ++r) {
             accA[r] = fma(sg[r][k], ea1, accA[r]);
