@@ -405,7 +405,7 @@ void launch_blocks(prbf_ctx* c, bool fp32, const CholCfg& cfg, long long nwork, 
 // fp32: blocks in FP32 on the FP32 MFMA, per-block status in `status`.  The
 // launch configuration is forced, cached, or, on the first call for a
 // (precision, block size), timed for every configuration that fits in LDS on
-// the first min(4096, nwork) blocks of the list, keeping the fastest (the
+// the first min(12288, nwork) blocks of the list, keeping the fastest (the
 // project's autotuner).
 template <typename Real>
 void factor_blocks(prbf_ctx* c, bool fp32, const std::vector<int>& order, int nmax, int tmax, const long long* mptr,
@@ -444,7 +444,10 @@ void factor_blocks(prbf_ctx* c, bool fp32, const std::vector<int>& order, int nm
     }
   } W{c->fb_ws};
   if (tune) {
-    const long long ns = std::min<long long>(4096, nwork);
+    // ~16 blocks per resident workgroup: fewer let the launch's tail decide
+    // (at 4096 the 4x2 and 8x4 FP32 configurations timed within 7% of each
+    // other, against 12% over the full launch)
+    const long long ns = std::min<long long>(12288, nwork);
     float best = 1e30f;
     for (const CholCfg& cand : kCholCfgs) {
       if (block_lds_bytes(fp32, nmax, tmax, cand.nw, cand.npc) > kMaxLdsBytes) continue;

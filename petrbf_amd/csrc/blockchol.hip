@@ -346,17 +346,19 @@ __device__ void factor_block(
     for (int cc = 0; cc < npc; ++cc) {
       const int kk = k0 + cc;
       if (cc > 0) {
-        // bring column kk up to date with the earlier columns of this panel
+        // bring column kk up to date with the earlier columns of this panel;
+        // the result waits for the TRSM in its panel slot, not in the workspace
         for (int i = kk + wave; i < KT; i += NW) {
-          Real* tik = Wb + (long long)ltile(i, kk) * kTE;
+          const Real* tik = Wb + (long long)ltile(i, kk) * kTE;
           v4 acc;
 #pragma unroll
           for (int r = 0; r < 4; ++r) acc[r] = tik[r * 64 + lane];
           for (int pc = 0; pc < cc; ++pc) {
             acc = PT::minus_prod(P + (pc * KT + i) * PT::kT, P + (pc * KT + kk) * PT::kT, lr, lq, acc);
           }
+          Real* pi = P + (cc * KT + i) * PT::kT;
 #pragma unroll
-          for (int r = 0; r < 4; ++r) tik[r * 64 + lane] = acc[r];
+          for (int r = 0; r < 4; ++r) pi[PT::idx(MF<Real>::row(r, lq), lr)] = acc[r];
           if (i == kk) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) D[MF<Real>::row(r, lq) * kLS + lr] = acc[r];
@@ -382,16 +384,24 @@ __device__ void factor_block(
       }
       __syncthreads();
       // panel column: L_ik = A_ik L_kk^{-T}; the A operand is read straight
-      // from the stored tile, all of a wavefront's tiles issued before use
+      // from the stored tile (the first column of a panel) or from the panel
+      // slot the in-panel update left it in, all of a wavefront's tiles
+      // issued before use; L_ik replaces A_ik in the slot
       for (int i0 = kk + 1 + wave; i0 < KT; i0 += NW * 4) {
         Real av[4][4];
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
           const int i = i0 + g * NW;
           if (i < KT) {
-            const Real* tik = Wb + (long long)ltile(i, kk) * kTE;
+            if (cc == 0) {
+              const Real* tik = Wb + (long long)ltile(i, kk) * kTE;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) av[g][q] = tik[MF<Real>::off(lr, 4 * q + lq)];
+              for (int q = 0; q < 4; ++q) av[g][q] = tik[MF<Real>::off(lr, 4 * q + lq)];
+            } else {
+              const Real* pi = P + (cc * KT + i) * PT::kT;
+#pragma unroll
+              for (int q = 0; q < 4; ++q) av[g][q] = pi[PT::idx(lr, 4 * q + lq)];
+            }
           }
         }
 #pragma unroll
