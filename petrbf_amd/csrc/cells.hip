@@ -227,99 +227,137 @@ __global__ void k_list_fill(Grid g, int reach, double len, const int* __restrict
   visit_list<DIM>(g, c, reach, len, tsta, tend, sta, end, xs, ys, zs, [&](int j) { idx[k++] = j; });
 }
 
-// Both lists of cluster c in one scan, for a cluster structure whose targets
-// are its sources (the solve): the buffer list (reach rb, half-width lb) and
-// the truncation list (rt, lt), each in the order visit_list gives it, with
-// the same box tests.  The scan covers the larger reach; a cell outside one
-// list's reach contributes nothing to that list.
-template <int DIM, class EmitB, class EmitT>
-__device__ inline void visit_lists2(const Grid& g, long long c, int rb, double lb, int rt, double lt,
-                                    const int* __restrict__ sta, const int* __restrict__ end,
-                                    const double* __restrict__ xs, const double* __restrict__ ys,
-                                    const double* __restrict__ zs, EmitB&& eb, EmitT&& et) {
-  if (sta[c] > end[c]) return;
-  int ix, iy, iz;
-  cell_coords(g, c, &ix, &iy, &iz);
-  const double xc = cell_centre(g.xmin, ix, g.box);
-  const double yc = cell_centre(g.ymin, iy, g.box);
-  const double zc = DIM == 3 ? cell_centre(g.zmin, iz, g.box) : 0.0;
-  for (int j = sta[c]; j <= end[c]; ++j) {
-    eb(j);
-    et(j);
-  }
-  const int R = max(rb, rt);
-  const int x0 = max(0, ix - R), x1 = min(g.nx - 1, ix + R);
-  const int y0 = max(0, iy - R), y1 = min(g.ny - 1, iy + R);
-  const int z0 = DIM == 3 ? max(0, iz - R) : 0, z1 = DIM == 3 ? min(g.nz - 1, iz + R) : 0;
-  for (int jx = x0; jx <= x1; ++jx)
-    for (int jy = y0; jy <= y1; ++jy)
-      for (int jz = z0; jz <= z1; ++jz) {
-        if (jx == ix && jy == iy && jz == iz) continue;
-        const int ax = abs(jx - ix), ay = abs(jy - iy), az = abs(jz - iz);
-        const bool cb = ax <= rb && ay <= rb && az <= rb;
-        const bool ct = ax <= rt && ay <= rt && az <= rt;
-        if (!cb && !ct) continue;
-        const long long jc = cell_index(g, jx, jy, jz);
-        for (int j = sta[jc]; j <= end[jc]; ++j) {
-          const double dx = fabs(xs[j] - xc), dy = fabs(ys[j] - yc), dz = DIM == 3 ? fabs(zs[j] - zc) : 0.0;
-          bool ib = cb && dx < lb && dy < lb;
-          bool it = ct && dx < lt && dy < lt;
-          if (DIM == 3) {
-            ib = ib && dz < lb;
-            it = it && dz < lt;
-          }
-          if (ib) eb(j);
-          if (it) et(j);
+// Both lists of one particle set, one wavefront per cluster.  The
+// candidates are the particles of the cluster's own cell, then those of the
+// neighbour cells in the reference's loop order (jx outer, jy, jz inner),
+// restricted to cells one of the two lists reaches; lanes take consecutive
+// candidates, test them against the buffer and truncation boxes with the
+// same FP64 operations as visit_list, and ballots place the hits, so each
+// list comes out in the reference's order with coalesced writes.  Count pass
+// (FILL false): per-cluster sizes; fill pass: the indices at pb / pt.
+template <int DIM, bool FILL>
+__global__ __launch_bounds__(256) void k_lists2(Grid g, int rb, double lb, int rt, double lt,
+                                                const int* __restrict__ sta, const int* __restrict__ end,
+                                                const double* __restrict__ xs, const double* __restrict__ ys,
+                                                const double* __restrict__ zs, int* __restrict__ cb,
+                                                int* __restrict__ ct, const long long* __restrict__ pb,
+                                                int* __restrict__ ib, const long long* __restrict__ pt,
+                                                int* __restrict__ it) {
+  __shared__ int s_off[4][65];           // per wavefront: first candidate of each cell of a chunk
+  __shared__ int s_j0[4][64];            // the cell's first particle
+  __shared__ unsigned char s_fl[4][64];  // 1: buffer list reaches it, 2: truncation list, 4: own cell
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const long long c = __builtin_amdgcn_readfirstlane((int)((long long)blockIdx.x * 4 + w));
+  if (c >= g.ncell) return;  // (wavefronts never wait for each other)
+  const unsigned long long below = (1ull << lane) - 1;
+  long long kb = 0, kt = 0;
+  const int s0 = sta[c], e0 = end[c];
+  if (s0 <= e0) {
+    int ix, iy, iz;
+    cell_coords(g, c, &ix, &iy, &iz);
+    const double xc = cell_centre(g.xmin, ix, g.box);
+    const double yc = cell_centre(g.ymin, iy, g.box);
+    const double zc = DIM == 3 ? cell_centre(g.zmin, iz, g.box) : 0.0;
+    const int R = max(rb, rt), W = 2 * R + 1;
+    const int M = DIM == 3 ? W * W * W : W * W;             // cells visited, own first
+    const int own = DIM == 3 ? (R * W + R) * W + R : R * W + R;  // own cell's place in loop order
+    const long long bb = FILL ? pb[c] : 0, bt = FILL ? pt[c] : 0;
+    for (int q0 = 0; q0 < M; q0 += 64) {
+      const int q = q0 + lane;
+      int cnt = 0, j0 = 0, fl = 0;
+      if (q < M) {
+        const int id = q == 0 ? own : (q - 1 < own ? q - 1 : q);
+        const int ox = (DIM == 3 ? id / (W * W) : id / W) - R;
+        const int oy = (DIM == 3 ? (id / W) % W : id % W) - R;
+        const int oz = DIM == 3 ? id % W - R : 0;
+        const int jx = ix + ox, jy = iy + oy, jz = iz + oz;
+        const bool inside = jx >= 0 && jx < g.nx && jy >= 0 && jy < g.ny && (DIM == 2 || (jz >= 0 && jz < g.nz));
+        const int ax = abs(ox), ay = abs(oy), az = abs(oz);
+        const bool fb = ax <= rb && ay <= rb && az <= rb;
+        const bool ft = ax <= rt && ay <= rt && az <= rt;
+        if (inside && (q == 0 || fb || ft)) {
+          const long long jc = cell_index(g, jx, jy, jz);
+          j0 = sta[jc];
+          cnt = max(0, end[jc] - j0 + 1);
+          fl = q == 0 ? 4 : (fb ? 1 : 0) | (ft ? 2 : 0);
         }
       }
-}
-
-template <int DIM>
-__global__ void k_lists2_count(Grid g, int rb, double lb, int rt, double lt, const int* __restrict__ sta,
-                               const int* __restrict__ end, const double* __restrict__ xs,
-                               const double* __restrict__ ys, const double* __restrict__ zs, int* __restrict__ cb,
-                               int* __restrict__ ct) {
-  long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= g.ncell) return;
-  int kb = 0, kt = 0;
-  visit_lists2<DIM>(g, c, rb, lb, rt, lt, sta, end, xs, ys, zs, [&](int) { ++kb; }, [&](int) { ++kt; });
-  cb[c] = kb;
-  ct[c] = kt;
-}
-
-template <int DIM>
-__global__ void k_lists2_fill(Grid g, int rb, double lb, int rt, double lt, const int* __restrict__ sta,
-                              const int* __restrict__ end, const double* __restrict__ xs,
-                              const double* __restrict__ ys, const double* __restrict__ zs,
-                              const long long* __restrict__ pb, int* __restrict__ ib, const long long* __restrict__ pt,
-                              int* __restrict__ it) {
-  long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= g.ncell) return;
-  long long kb = pb[c], kt = pt[c];
-  visit_lists2<DIM>(g, c, rb, lb, rt, lt, sta, end, xs, ys, zs, [&](int j) { ib[kb++] = j; },
-                    [&](int j) { it[kt++] = j; });
+      int incl = cnt;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const int v = __shfl_up(incl, d);
+        if (lane >= d) incl += v;
+      }
+      const int tot = __shfl(incl, 63);
+      s_off[w][lane] = incl - cnt;
+      s_j0[w][lane] = j0;
+      s_fl[w][lane] = (unsigned char)fl;
+      if (lane == 63) s_off[w][64] = tot;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      for (int k0 = 0; k0 < tot; k0 += 64) {
+        const int k = k0 + lane;
+        bool hb = false, ht = false;
+        int j = 0;
+        if (k < tot) {
+          int lo = 0, hi = 63;  // the last cell whose first candidate is <= k
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_off[w][mid] <= k) lo = mid;
+            else hi = mid - 1;
+          }
+          j = s_j0[w][lo] + (k - s_off[w][lo]);
+          const int f = s_fl[w][lo];
+          if (f & 4) {
+            hb = ht = true;
+          } else {
+            const double dx = fabs(xs[j] - xc), dy = fabs(ys[j] - yc), dz = DIM == 3 ? fabs(zs[j] - zc) : 0.0;
+            hb = (f & 1) && dx < lb && dy < lb;
+            ht = (f & 2) && dx < lt && dy < lt;
+            if (DIM == 3) {
+              hb = hb && dz < lb;
+              ht = ht && dz < lt;
+            }
+          }
+        }
+        const unsigned long long mb = __ballot(hb), mt = __ballot(ht);
+        if (FILL) {
+          if (hb) ib[bb + kb + __popcll(mb & below)] = j;
+          if (ht) it[bt + kt + __popcll(mt & below)] = j;
+        }
+        kb += __popcll(mb);
+        kt += __popcll(mt);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  if (!FILL && lane == 0) {
+    cb[c] = (int)kb;
+    ct[c] = (int)kt;
+  }
 }
 
 void lists2_count(hipStream_t st, const Grid& g, const int* sta, const int* end, const double* xs, const double* ys,
                   const double* zs, int* count_b, int* count_t) {
-  int nb = (int)((g.ncell + 127) / 128);
+  const unsigned nb = (unsigned)((g.ncell + 3) / 4);
   if (g.dim == 2)
-    hipLaunchKernelGGL(k_lists2_count<2>, dim3(nb), dim3(128), 0, st, g, g.nb_buffer, g.buffer_len, g.nb_trunc,
-                       g.trunc_len, sta, end, xs, ys, zs, count_b, count_t);
+    hipLaunchKernelGGL((k_lists2<2, false>), dim3(nb), dim3(256), 0, st, g, g.nb_buffer, g.buffer_len, g.nb_trunc,
+                       g.trunc_len, sta, end, xs, ys, zs, count_b, count_t, nullptr, nullptr, nullptr, nullptr);
   else
-    hipLaunchKernelGGL(k_lists2_count<3>, dim3(nb), dim3(128), 0, st, g, g.nb_buffer, g.buffer_len, g.nb_trunc,
-                       g.trunc_len, sta, end, xs, ys, zs, count_b, count_t);
+    hipLaunchKernelGGL((k_lists2<3, false>), dim3(nb), dim3(256), 0, st, g, g.nb_buffer, g.buffer_len, g.nb_trunc,
+                       g.trunc_len, sta, end, xs, ys, zs, count_b, count_t, nullptr, nullptr, nullptr, nullptr);
 }
 
 void lists2_fill(hipStream_t st, const Grid& g, const int* sta, const int* end, const double* xs, const double* ys,
                  const double* zs, const long long* ptr_b, int* idx_b, const long long* ptr_t, int* idx_t) {
-  int nb = (int)((g.ncell + 127) / 128);
+  const unsigned nb = (unsigned)((g.ncell + 3) / 4);
   if (g.dim == 2)
-    hipLaunchKernelGGL(k_lists2_fill<2>, dim3(nb), dim3(128), 0, st, g, g.nb_buffer, g.buffer_len, g.nb_trunc,
-                       g.trunc_len, sta, end, xs, ys, zs, ptr_b, idx_b, ptr_t, idx_t);
+    hipLaunchKernelGGL((k_lists2<2, true>), dim3(nb), dim3(256), 0, st, g, g.nb_buffer, g.buffer_len, g.nb_trunc,
+                       g.trunc_len, sta, end, xs, ys, zs, nullptr, nullptr, ptr_b, idx_b, ptr_t, idx_t);
   else
-    hipLaunchKernelGGL(k_lists2_fill<3>, dim3(nb), dim3(128), 0, st, g, g.nb_buffer, g.buffer_len, g.nb_trunc,
-                       g.trunc_len, sta, end, xs, ys, zs, ptr_b, idx_b, ptr_t, idx_t);
+    hipLaunchKernelGGL((k_lists2<3, true>), dim3(nb), dim3(256), 0, st, g, g.nb_buffer, g.buffer_len, g.nb_trunc,
+                       g.trunc_len, sta, end, xs, ys, zs, nullptr, nullptr, ptr_b, idx_b, ptr_t, idx_t);
 }
 
 void list_count(hipStream_t st, const Grid& g, int reach, double len, const int* tsta, const int* tend,
