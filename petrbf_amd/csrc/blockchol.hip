@@ -126,7 +126,10 @@ __device__ inline double gauss_entry<double>(double dx, double dy, double dz, do
 }
 template <>
 __device__ inline float gauss_entry<float>(double dx, double dy, double dz, double inv, double norm) {
-  float a = (float)((dx * dx + dy * dy + dz * dz) * inv);
+  // differences in FP64 (coordinates ~1, distances ~sigma), the rest in FP32:
+  // the exponent's relative error stays within a few FP32 ulp
+  const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
+  const float a = (fx * fx + fy * fy + fz * fz) * (float)inv;
   return __expf(-a) * (float)norm;
 }
 
