@@ -160,17 +160,13 @@ struct DiagStep {
     const Real rinv = pivot_rsqrt(b ? Real(1) : piv);  // (branch-free: one block per step)
     rs = i == P ? rinv : rs;
     const Real m = i > P ? a[P] * (rinv * rinv) : Real(0);
-    // one broadcast value alive at a time (the scheduler would otherwise
-    // hoist all sixteen moves and push the caller's registers into spills)
 #pragma unroll
     for (int q = P + 1; q < 16; ++q) {
       a[q] -= m * row_bcast<P>(a[q]);
-      __builtin_amdgcn_sched_barrier(0);
     }
 #pragma unroll
     for (int q = 0; q <= P; ++q) {
       e[q] -= m * row_bcast<P>(e[q]);
-      __builtin_amdgcn_sched_barrier(0);
     }
     DiagStep<Real, P + 1>::run(a, e, rs, bad, i);
   }
