@@ -505,29 +505,54 @@ __device__ void factor_block(
     }
     __syncthreads();
     // backward: X_I = L_II^{-T} (Y_I - sum_{J>I} L_JI^T X_J); the sum is split
-    // over the wavefronts and combined by wavefront 0 through the scratch tiles
+    // over the wavefronts and combined by wavefront 0 through the scratch
+    // tiles.  Each wavefront's first kPF tiles of column I-1 (and wavefront
+    // 0's L_{I-1,I-1}^{-1}) are loaded while step I runs, so the global
+    // loads leave the step's critical path.
+    constexpr int kPF = 4;
+    Real nx[kPF][4], nli[4];
+    auto load_col = [&](int I) {
+#pragma unroll
+      for (int g = 0; g < kPF; ++g) {
+        const int J = I + 1 + wave + g * NW;
+        if (J < KT) {
+          const Real* tji = Wb + (long long)ltile(J, I) * kTE;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) nx[g][q] = tji[MF<Real>::off(4 * q + lq, lr)];
+        }
+      }
+      if (wave == 0) {
+        const Real* li = Wd + (long long)I * kTE;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) nli[q] = li[(4 * q + lq) * 16 + lr];
+      }
+    };
+    load_col(KT - 1);
     for (int I = KT - 1; I >= 0; --I) {
+      Real av[kPF][4], al[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        al[q] = nli[q];
+#pragma unroll
+        for (int g = 0; g < kPF; ++g) av[g][q] = nx[g][q];
+      }
+      if (I > 0) load_col(I - 1);
       v4 acc = {0, 0, 0, 0};
-      for (int J0 = I + 1 + wave; J0 < KT; J0 += NW * 4) {
-        Real av[4][4];
 #pragma unroll
-        for (int g = 0; g < 4; ++g) {
-          const int J = J0 + g * NW;
-          if (J < KT) {
-            const Real* tji = Wb + (long long)ltile(J, I) * kTE;
+      for (int g = 0; g < kPF; ++g) {
+        const int J = I + 1 + wave + g * NW;
+        if (J < KT) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) av[g][q] = tji[MF<Real>::off(4 * q + lq, lr)];
-          }
+          for (int q = 0; q < 4; ++q) acc = MF<Real>::mma(av[g][q], X[(16 * J + 4 * q + lq) * kLS + lr], acc);
         }
+      }
+      for (int J = I + 1 + wave + kPF * NW; J < KT; J += NW) {  // (blocks beyond kPF x NW tiles)
+        const Real* tji = Wb + (long long)ltile(J, I) * kTE;
+        Real a[4];
 #pragma unroll
-        for (int g = 0; g < 4; ++g) {
-          const int J = J0 + g * NW;
-          if (J < KT) {
+        for (int q = 0; q < 4; ++q) a[q] = tji[MF<Real>::off(4 * q + lq, lr)];
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-              acc = MF<Real>::mma(av[g][q], X[(16 * J + 4 * q + lq) * kLS + lr], acc);
-          }
-        }
+        for (int q = 0; q < 4; ++q) acc = MF<Real>::mma(a[q], X[(16 * J + 4 * q + lq) * kLS + lr], acc);
       }
 #pragma unroll
       for (int r = 0; r < 4; ++r) Sw[r * 64 + lane] = acc[r];
@@ -545,10 +570,9 @@ __device__ void factor_block(
 #pragma unroll
         for (int r = 0; r < 4; ++r) Sw[MF<Real>::row(r, lq) * kLS + lr] = rv[r];
         wave_fence();
-        const Real* li = Wd + (long long)I * kTE;
         v4 x = {0, 0, 0, 0};
 #pragma unroll
-        for (int q = 0; q < 4; ++q) x = MF<Real>::mma(li[(4 * q + lq) * 16 + lr], Sw[(4 * q + lq) * kLS + lr], x);
+        for (int q = 0; q < 4; ++q) x = MF<Real>::mma(al[q], Sw[(4 * q + lq) * kLS + lr], x);
 #pragma unroll
         for (int r = 0; r < 4; ++r) X[(16 * I + MF<Real>::row(r, lq)) * kLS + lr] = x[r];
       }
