@@ -170,73 +170,23 @@ __global__ void k_copy_ll_to_int(long long n, const long long* __restrict__ a, i
 }
 
 // ---------------------------------------------------------------- cluster lists
-// Visits the list of cluster c in the reference order and calls emit(j) for
-// each member.  `core_sta/core_end` are the cluster's own particles of the
-// listed set; the candidate cells come from `sta/end` of the same set.
-template <int DIM, class Emit>
-__device__ inline void visit_list(const Grid& g, long long c, int reach, double len,
-                                  const int* __restrict__ tsta, const int* __restrict__ tend,
-                                  const int* __restrict__ sta, const int* __restrict__ end,
-                                  const double* __restrict__ xs, const double* __restrict__ ys,
-                                  const double* __restrict__ zs, Emit&& emit) {
-  if (tsta[c] > tend[c]) return;  // cluster without targets: empty list (get_buffer.h:79-81)
-  int ix, iy, iz;
-  cell_coords(g, c, &ix, &iy, &iz);
-  double xc = cell_centre(g.xmin, ix, g.box);
-  double yc = cell_centre(g.ymin, iy, g.box);
-  double zc = DIM == 3 ? cell_centre(g.zmin, iz, g.box) : 0.0;
-  for (int j = sta[c]; j <= end[c]; ++j) emit(j);
-  int x0 = max(0, ix - reach), x1 = min(g.nx - 1, ix + reach);
-  int y0 = max(0, iy - reach), y1 = min(g.ny - 1, iy + reach);
-  int z0 = DIM == 3 ? max(0, iz - reach) : 0, z1 = DIM == 3 ? min(g.nz - 1, iz + reach) : 0;
-  for (int jx = x0; jx <= x1; ++jx)
-    for (int jy = y0; jy <= y1; ++jy)
-      for (int jz = z0; jz <= z1; ++jz) {
-        if (jx == ix && jy == iy && jz == iz) continue;
-        long long jc = cell_index(g, jx, jy, jz);
-        for (int j = sta[jc]; j <= end[jc]; ++j) {
-          bool in = fabs(xs[j] - xc) < len && fabs(ys[j] - yc) < len;
-          if (DIM == 3) in = in && fabs(zs[j] - zc) < len;
-          if (in) emit(j);
-        }
-      }
-}
-
-template <int DIM>
-__global__ void k_list_count(Grid g, int reach, double len, const int* __restrict__ tsta,
-                             const int* __restrict__ tend, const int* __restrict__ sta,
-                             const int* __restrict__ end, const double* __restrict__ xs,
-                             const double* __restrict__ ys, const double* __restrict__ zs,
-                             int* __restrict__ count) {
-  long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= g.ncell) return;
-  int k = 0;
-  visit_list<DIM>(g, c, reach, len, tsta, tend, sta, end, xs, ys, zs, [&](int) { ++k; });
-  count[c] = k;
-}
-
-template <int DIM>
-__global__ void k_list_fill(Grid g, int reach, double len, const int* __restrict__ tsta,
-                            const int* __restrict__ tend, const int* __restrict__ sta,
-                            const int* __restrict__ end, const double* __restrict__ xs,
-                            const double* __restrict__ ys, const double* __restrict__ zs,
-                            const long long* __restrict__ ptr, int* __restrict__ idx) {
-  long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= g.ncell) return;
-  long long k = ptr[c];
-  visit_list<DIM>(g, c, reach, len, tsta, tend, sta, end, xs, ys, zs, [&](int j) { idx[k++] = j; });
-}
-
-// Both lists of one particle set, one wavefront per cluster.  The
-// candidates are the particles of the cluster's own cell, then those of the
-// neighbour cells in the reference's loop order (jx outer, jy, jz inner),
-// restricted to cells one of the two lists reaches; lanes take consecutive
-// candidates, test them against the buffer and truncation boxes with the
-// same FP64 operations as visit_list, and ballots place the hits, so each
-// list comes out in the reference's order with coalesced writes.  Count pass
-// (FILL false): per-cluster sizes; fill pass: the indices at pb / pt.
-template <int DIM, bool FILL>
-__global__ __launch_bounds__(256) void k_lists2(Grid g, int rb, double lb, int rt, double lt,
+// Cluster lists, one wavefront per cluster: TWO, the buffer list (reach rb,
+// half-width lb) and the truncation list (rt, lt) of a cluster structure
+// whose targets are its sources (the solve), in one scan; otherwise one list
+// (rb, lb) of source particles for the clusters of a target set (evaluation
+// at distinct targets; tsta/tend are the targets, an empty cluster gets an
+// empty list, 2d/get_buffer.h:79-81).  The candidates are the source
+// particles of the cluster's own cell, taken unconditionally, then those of
+// the neighbour cells in the reference's loop order (jx outer, jy, jz inner)
+// that a list reaches, tested against the box around the cell centre with
+// the reference's FP64 test fabs(x - xc) < len per axis
+// (2d/get_buffer.h:64, 2d/get_trunc.h:63).  Lanes take consecutive
+// candidates and ballots place the hits, so each list comes out in the
+// reference's order with coalesced writes.  Count pass (FILL false):
+// per-cluster sizes; fill pass: the indices at pb / pt.
+template <int DIM, bool FILL, bool TWO>
+__global__ __launch_bounds__(256) void k_lists(Grid g, int rb, double lb, int rt, double lt,
+                                                const int* __restrict__ tsta, const int* __restrict__ tend,
                                                 const int* __restrict__ sta, const int* __restrict__ end,
                                                 const double* __restrict__ xs, const double* __restrict__ ys,
                                                 const double* __restrict__ zs, int* __restrict__ cb,
@@ -251,17 +201,16 @@ __global__ __launch_bounds__(256) void k_lists2(Grid g, int rb, double lb, int r
   if (c >= g.ncell) return;  // (wavefronts never wait for each other)
   const unsigned long long below = (1ull << lane) - 1;
   long long kb = 0, kt = 0;
-  const int s0 = sta[c], e0 = end[c];
-  if (s0 <= e0) {
+  if (tsta[c] <= tend[c]) {
     int ix, iy, iz;
     cell_coords(g, c, &ix, &iy, &iz);
     const double xc = cell_centre(g.xmin, ix, g.box);
     const double yc = cell_centre(g.ymin, iy, g.box);
     const double zc = DIM == 3 ? cell_centre(g.zmin, iz, g.box) : 0.0;
-    const int R = max(rb, rt), W = 2 * R + 1;
+    const int R = TWO ? max(rb, rt) : rb, W = 2 * R + 1;
     const int M = DIM == 3 ? W * W * W : W * W;             // cells visited, own first
     const int own = DIM == 3 ? (R * W + R) * W + R : R * W + R;  // own cell's place in loop order
-    const long long bb = FILL ? pb[c] : 0, bt = FILL ? pt[c] : 0;
+    const long long bb = FILL ? pb[c] : 0, bt = (FILL && TWO) ? pt[c] : 0;
     for (int q0 = 0; q0 < M; q0 += 64) {
       const int q = q0 + lane;
       int cnt = 0, j0 = 0, fl = 0;
@@ -274,7 +223,7 @@ __global__ __launch_bounds__(256) void k_lists2(Grid g, int rb, double lb, int r
         const bool inside = jx >= 0 && jx < g.nx && jy >= 0 && jy < g.ny && (DIM == 2 || (jz >= 0 && jz < g.nz));
         const int ax = abs(ox), ay = abs(oy), az = abs(oz);
         const bool fb = ax <= rb && ay <= rb && az <= rb;
-        const bool ft = ax <= rt && ay <= rt && az <= rt;
+        const bool ft = TWO && ax <= rt && ay <= rt && az <= rt;
         if (inside && (q == 0 || fb || ft)) {
           const long long jc = cell_index(g, jx, jy, jz);
           j0 = sta[jc];
@@ -320,10 +269,10 @@ __global__ __launch_bounds__(256) void k_lists2(Grid g, int rb, double lb, int r
             }
           }
         }
-        const unsigned long long mb = __ballot(hb), mt = __ballot(ht);
+        const unsigned long long mb = __ballot(hb), mt = TWO ? __ballot(ht) : 0ull;
         if (FILL) {
           if (hb) ib[bb + kb + __popcll(mb & below)] = j;
-          if (ht) it[bt + kt + __popcll(mt & below)] = j;
+          if (TWO && ht) it[bt + kt + __popcll(mt & below)] = j;
         }
         kb += __popcll(mb);
         kt += __popcll(mt);
@@ -334,7 +283,7 @@ __global__ __launch_bounds__(256) void k_lists2(Grid g, int rb, double lb, int r
   }
   if (!FILL && lane == 0) {
     cb[c] = (int)kb;
-    ct[c] = (int)kt;
+    if (TWO) ct[c] = (int)kt;
   }
 }
 
@@ -342,42 +291,50 @@ void lists2_count(hipStream_t st, const Grid& g, const int* sta, const int* end,
                   const double* zs, int* count_b, int* count_t) {
   const unsigned nb = (unsigned)((g.ncell + 3) / 4);
   if (g.dim == 2)
-    hipLaunchKernelGGL((k_lists2<2, false>), dim3(nb), dim3(256), 0, st, g, g.nb_buffer, g.buffer_len, g.nb_trunc,
-                       g.trunc_len, sta, end, xs, ys, zs, count_b, count_t, nullptr, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL((k_lists<2, false, true>), dim3(nb), dim3(256), 0, st, g, g.nb_buffer, g.buffer_len,
+                       g.nb_trunc, g.trunc_len, sta, end, sta, end, xs, ys, zs, count_b, count_t, nullptr, nullptr,
+                       nullptr, nullptr);
   else
-    hipLaunchKernelGGL((k_lists2<3, false>), dim3(nb), dim3(256), 0, st, g, g.nb_buffer, g.buffer_len, g.nb_trunc,
-                       g.trunc_len, sta, end, xs, ys, zs, count_b, count_t, nullptr, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL((k_lists<3, false, true>), dim3(nb), dim3(256), 0, st, g, g.nb_buffer, g.buffer_len,
+                       g.nb_trunc, g.trunc_len, sta, end, sta, end, xs, ys, zs, count_b, count_t, nullptr, nullptr,
+                       nullptr, nullptr);
 }
 
 void lists2_fill(hipStream_t st, const Grid& g, const int* sta, const int* end, const double* xs, const double* ys,
                  const double* zs, const long long* ptr_b, int* idx_b, const long long* ptr_t, int* idx_t) {
   const unsigned nb = (unsigned)((g.ncell + 3) / 4);
   if (g.dim == 2)
-    hipLaunchKernelGGL((k_lists2<2, true>), dim3(nb), dim3(256), 0, st, g, g.nb_buffer, g.buffer_len, g.nb_trunc,
-                       g.trunc_len, sta, end, xs, ys, zs, nullptr, nullptr, ptr_b, idx_b, ptr_t, idx_t);
+    hipLaunchKernelGGL((k_lists<2, true, true>), dim3(nb), dim3(256), 0, st, g, g.nb_buffer, g.buffer_len,
+                       g.nb_trunc, g.trunc_len, sta, end, sta, end, xs, ys, zs, nullptr, nullptr, ptr_b, idx_b,
+                       ptr_t, idx_t);
   else
-    hipLaunchKernelGGL((k_lists2<3, true>), dim3(nb), dim3(256), 0, st, g, g.nb_buffer, g.buffer_len, g.nb_trunc,
-                       g.trunc_len, sta, end, xs, ys, zs, nullptr, nullptr, ptr_b, idx_b, ptr_t, idx_t);
+    hipLaunchKernelGGL((k_lists<3, true, true>), dim3(nb), dim3(256), 0, st, g, g.nb_buffer, g.buffer_len,
+                       g.nb_trunc, g.trunc_len, sta, end, sta, end, xs, ys, zs, nullptr, nullptr, ptr_b, idx_b,
+                       ptr_t, idx_t);
 }
 
 void list_count(hipStream_t st, const Grid& g, int reach, double len, const int* tsta, const int* tend,
                 const int* sta, const int* end, const double* xs, const double* ys, const double* zs,
                 int* count) {
-  int nb = (int)((g.ncell + 127) / 128);
+  const unsigned nb = (unsigned)((g.ncell + 3) / 4);
   if (g.dim == 2)
-    hipLaunchKernelGGL(k_list_count<2>, dim3(nb), dim3(128), 0, st, g, reach, len, tsta, tend, sta, end, xs, ys, zs, count);
+    hipLaunchKernelGGL((k_lists<2, false, false>), dim3(nb), dim3(256), 0, st, g, reach, len, 0, 0.0, tsta, tend,
+                       sta, end, xs, ys, zs, count, nullptr, nullptr, nullptr, nullptr, nullptr);
   else
-    hipLaunchKernelGGL(k_list_count<3>, dim3(nb), dim3(128), 0, st, g, reach, len, tsta, tend, sta, end, xs, ys, zs, count);
+    hipLaunchKernelGGL((k_lists<3, false, false>), dim3(nb), dim3(256), 0, st, g, reach, len, 0, 0.0, tsta, tend,
+                       sta, end, xs, ys, zs, count, nullptr, nullptr, nullptr, nullptr, nullptr);
 }
 
 void list_fill(hipStream_t st, const Grid& g, int reach, double len, const int* tsta, const int* tend,
                const int* sta, const int* end, const double* xs, const double* ys, const double* zs,
                const long long* ptr, int* idx) {
-  int nb = (int)((g.ncell + 127) / 128);
+  const unsigned nb = (unsigned)((g.ncell + 3) / 4);
   if (g.dim == 2)
-    hipLaunchKernelGGL(k_list_fill<2>, dim3(nb), dim3(128), 0, st, g, reach, len, tsta, tend, sta, end, xs, ys, zs, ptr, idx);
+    hipLaunchKernelGGL((k_lists<2, true, false>), dim3(nb), dim3(256), 0, st, g, reach, len, 0, 0.0, tsta, tend,
+                       sta, end, xs, ys, zs, nullptr, nullptr, ptr, idx, nullptr, nullptr);
   else
-    hipLaunchKernelGGL(k_list_fill<3>, dim3(nb), dim3(128), 0, st, g, reach, len, tsta, tend, sta, end, xs, ys, zs, ptr, idx);
+    hipLaunchKernelGGL((k_lists<3, true, false>), dim3(nb), dim3(256), 0, st, g, reach, len, 0, 0.0, tsta, tend,
+                       sta, end, xs, ys, zs, nullptr, nullptr, ptr, idx, nullptr, nullptr);
 }
 
 }  // namespace prbf
