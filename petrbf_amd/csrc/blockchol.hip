@@ -17,8 +17,9 @@
 //            (in LDS) with L_kk^{-1} beside it and the column as
 //            A_ik L_kk^{-T} on MFMA; then one trailing update
 //            A_ij -= sum L_ic L_jc^T per panel on MFMA with the panel in LDS
-//            (NPC = 2 or 4 halves or quarters the passes over the trailing
-//            tiles at the cost of LDS);
+//            (NPC = 2, 3 or 4 divides the passes over the trailing tiles at
+//            the cost of LDS; a panel's later columns stay in LDS from their
+//            in-panel update to their TRSM);
 //   phase 2  with the core last, L^{-1} E_core is zero above the core, so
 //            A'^{-1} E_core = L^{-T} (L^{-1} E_core) is a short forward and a
 //            full backward tile substitution, both MFMA with the
