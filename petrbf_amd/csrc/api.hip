@@ -8,6 +8,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -379,8 +380,13 @@ struct CholCfg {
 const CholCfg kCholCfgs[] = {{4, 1}, {8, 1}, {4, 2}, {8, 2}, {4, 4}, {8, 4},
                              {4, 3}, {6, 1}, {6, 2}, {6, 3}, {6, 4}, {8, 3}};
 
-// Process-wide cache of tuned configurations, keyed by precision and the
-// largest block's tile count.
+// Process-wide cache of tuned configurations, keyed by device, precision and
+// the largest block's tile count; shared by every context, so guarded (two
+// threads tuning the same key at once both tune, and the first result stays).
+std::mutex& tuned_mutex() {
+  static std::mutex m;
+  return m;
+}
 std::vector<std::pair<long long, CholCfg>>& tuned_cache() {
   static std::vector<std::pair<long long, CholCfg>> v;
   return v;
@@ -428,12 +434,19 @@ void factor_blocks(prbf_ctx* c, bool fp32, const std::vector<int>& order, int nm
     return (int)std::max<long long>(1, std::min(std::min(w, nwork), cap));
   };
   const int ktmax = (nmax + 15) / 16;
-  const long long key = (fp32 ? 1000000LL : 0LL) + ktmax * 1000LL + g.dim;
+  int dev = 0;
+  PRBF_HIP(hipGetDevice(&dev));
+  const long long key = dev * 10000000LL + (fp32 ? 1000000LL : 0LL) + ktmax * 1000LL + g.dim;
   CholCfg cfg{c->chol_waves, c->chol_panel};
   bool tune = tune_allowed && !c->tuning_forced;
-  if (tune)
+  if (tune) {
+    std::lock_guard<std::mutex> lock(tuned_mutex());
     for (auto& kv : tuned_cache())
-      if (kv.first == key) { cfg = kv.second; tune = false; }
+      if (kv.first == key) {
+        cfg = kv.second;
+        tune = false;
+      }
+  }
   // the workspace slots, kept by the context from one setup to the next
   struct {
     DBuf<unsigned char>& raw;
@@ -462,7 +475,14 @@ void factor_blocks(prbf_ctx* c, bool fp32, const std::vector<int>& order, int nm
       PRBF_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
       if (ms < best) { best = ms; cfg = cand; }
     }
-    tuned_cache().push_back({key, cfg});
+    std::lock_guard<std::mutex> lock(tuned_mutex());
+    bool have = false;
+    for (auto& kv : tuned_cache())
+      if (kv.first == key) {
+        cfg = kv.second;
+        have = true;
+      }
+    if (!have) tuned_cache().push_back({key, cfg});
   }
   if (block_lds_bytes(fp32, nmax, tmax, cfg.nw, cfg.npc) > kMaxLdsBytes ||
       !block_chol_has_config(fp32, cfg.nw, cfg.npc))

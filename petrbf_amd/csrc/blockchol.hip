@@ -29,6 +29,7 @@
 // T = double is the plain path (exact FP64 block solves); T = float runs the
 // same kernel in FP32 on the FP32 MFMA (mixed precision), with a per-block
 // status so a block whose FP32 factorisation breaks down is redone in FP64.
+#include <mutex>
 #include <vector>
 
 #include "../../include/petrbf.h"
@@ -645,13 +646,18 @@ static const void* chol_fn(int dim, int nw, int npc) {
   return dim == 3 ? chol_fn_dim<T, 3>(nw, npc) : chol_fn_dim<T, 2>(nw, npc);
 }
 
-// Raises the dynamic-LDS limit of a kernel to the CU's 160 KB, once.
+// Raises the dynamic-LDS limit of a kernel to the CU's 160 KB, once per
+// (device, kernel); the record is process-wide, hence the lock.
 static void set_lds_attr(const void* fn) {
-  static std::vector<const void*> done;
-  for (const void* f : done)
-    if (f == fn) return;
+  static std::mutex m;
+  static std::vector<std::pair<int, const void*>> done;
+  int dev = 0;
+  PRBF_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(m);
+  for (const auto& d : done)
+    if (d.first == dev && d.second == fn) return;
   PRBF_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-  done.push_back(fn);
+  done.push_back({dev, fn});
 }
 
 template <typename T>

@@ -287,3 +287,38 @@ def test_graph_captured_iterations_match_kernel_by_kernel(ksp, pc):
         if pc != "none" and restart == 30:
             assert i0.reason == 2
     ctx.close()
+
+
+def test_contexts_in_concurrent_threads_agree():
+    """One context per host thread: contexts set up and solving at the same
+    time in four threads (sharing the process-wide tuner and kernel-attribute
+    caches) give the single-threaded gamma bit for bit."""
+    import threading
+    _announce("four contexts in four threads")
+    p = lattice(48, seed=0x5EED0031)
+
+    def run():
+        ctx = Context.from_problem(p)
+        ctx.set_points(p.x, p.y)
+        g, inf = ctx.solve(p.w[0], ksp="fgmres", pc="rasm32", rtol=1e-10)
+        ctx.close()
+        return g, inf.its
+
+    ref = run()
+    out = [None] * 4
+    errs = []
+
+    def worker(k):
+        try:
+            out[k] = run()
+        except Exception as e:  # reported below, not swallowed
+            errs.append(repr(e))
+
+    th = [threading.Thread(target=worker, args=(k,)) for k in range(4)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errs, errs
+    for g, its in out:
+        assert its == ref[1] and np.array_equal(g, ref[0])
