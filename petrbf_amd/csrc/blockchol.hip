@@ -188,7 +188,8 @@ __device__ inline double bcast(double v, int l) {
 
 // L^{-1} of the 16x16 diagonal tile D (row-major, row stride DS) into Li (row
 // stride OS, zero above the diagonal), by one wavefront.  Returns true (in
-// every lane) when a pivot is not positive.
+// every lane) when a pivot is not positive.  Li may be D: every lane holds
+// its rows in registers before any lane writes.
 //
 // FP32: the elimination of DiagStep on every 16-lane row at once, without
 // scaling rows as it goes (row i of L^{-1} is the right half's row i times
@@ -260,17 +261,24 @@ __host__ __device__ inline long long blk_ws(int n, int T) {
   return (long long)(KT * (KT + 1) / 2 + KT) * kTE;
 }
 
+// Panel tiles held in LDS for m columns of a block of KT tile rows: column pc
+// of the panel starting at tile column k0 keeps the rows below its diagonal
+// tile, k0+pc+1 .. KT-1 (the diagonal tile itself goes to D), and the first
+// panel (k0 = 0) is the largest
+__host__ __device__ inline int pan_tiles(int KT, int m) { return m * (KT - 1) - m * (m - 1) / 2; }
+
 template <typename T>
 __host__ __device__ inline size_t blk_lds_bytes(int nmax, int tmax, int nw, int npc) {
   int KT = (nmax + 15) / 16, NP = 16 * KT;
   (void)tmax;
-  size_t pan = (size_t)npc * KT * Pan<T>::kT * sizeof(T);
+  const int m = npc < KT ? npc : KT;
+  size_t pan = (size_t)pan_tiles(KT, m) * Pan<T>::kT * sizeof(T);
   size_t crd = (size_t)3 * NP * sizeof(double);
   size_t red = ((size_t)NP * kLS + (size_t)nw * kLT) * sizeof(T);  // phase-2 right-hand sides, per-wave scratch
   size_t region = pan > crd ? pan : crd;
   region = region > red ? region : red;
   region = (region + 15) & ~(size_t)15;
-  return region + (size_t)2 * kLT * sizeof(T) + 16;
+  return region + (size_t)kLT * sizeof(T) + 16;
 }
 
 // One block: cluster c, workspace Wb (blk_ws elements), LDS as laid out below.
@@ -296,21 +304,22 @@ __device__ void factor_block(
 
   // LDS carve-up: one region holding, by phase, the coordinates (0), the
   // panel (1), and the right-hand sides X with the per-wave scratch tiles
-  // (2, 3); then the diagonal tile, its inverse and the breakdown flag
+  // (2, 3); then the diagonal tile, which its inverse overwrites in place,
+  // and the breakdown flag
   typedef Pan<Real> PT;
-  size_t pan = (size_t)NPC * KT * PT::kT * sizeof(Real), crd = (size_t)3 * NP * sizeof(double),
+  size_t pan = (size_t)pan_tiles(KT, min(NPC, KT)) * PT::kT * sizeof(Real), crd = (size_t)3 * NP * sizeof(double),
          red = ((size_t)NP * kLS + (size_t)NW * kLT) * sizeof(Real);
   size_t region = pan > crd ? pan : crd;
   region = region > red ? region : red;
   region = (region + 15) & ~(size_t)15;
-  Real* P = (Real*)lds_raw;                     // panel: NPC x KT tiles (PT layout)
+  Real* P = (Real*)lds_raw;                     // panel: pan_tiles tiles (PT layout)
   double* cx = (double*)lds_raw;          // phase 0 coordinates (same region)
   double* cy = cx + NP;
   double* cz = cy + NP;
   Real* S = (Real*)lds_raw + (size_t)NP * kLS;  // per-wave scratch tiles, past X
-  Real* D = (Real*)(lds_raw + region);       // diagonal tile L_kk
-  Real* Li = D + kLT;                        // L_kk^{-1}
-  int* flag = (int*)(Li + kLT);
+  Real* D = (Real*)(lds_raw + region);       // diagonal tile L_kk, then L_kk^{-1}
+  Real* Li = D;
+  int* flag = (int*)(D + kLT);
   Real* Sw = S + wave * kLT;
 
   // ---- phase 0: coordinates (core last) and the lower tiles of A'
@@ -344,6 +353,9 @@ __device__ void factor_block(
   // ---- phase 1: tiled right-looking Cholesky, NPC tile columns per panel
   for (int k0 = 0; k0 < KT; k0 += NPC) {
     const int npc = min(NPC, KT - k0);
+    // panel slot of tile (i, k0 + pc), i > k0 + pc: the columns' row ranges
+    // packed one after another
+    auto pan = [&](int pc, int i) { return P + (pc * (KT - k0 - 2) - (pc * (pc - 1) >> 1) + i - k0 - 1) * PT::kT; };
     for (int cc = 0; cc < npc; ++cc) {
       const int kk = k0 + cc;
       if (cc > 0) {
@@ -354,15 +366,14 @@ __device__ void factor_block(
           v4 acc;
 #pragma unroll
           for (int r = 0; r < 4; ++r) acc[r] = tik[r * 64 + lane];
-          for (int pc = 0; pc < cc; ++pc) {
-            acc = PT::minus_prod(P + (pc * KT + i) * PT::kT, P + (pc * KT + kk) * PT::kT, lr, lq, acc);
-          }
-          Real* pi = P + (cc * KT + i) * PT::kT;
-#pragma unroll
-          for (int r = 0; r < 4; ++r) pi[PT::idx(MF<Real>::row(r, lq), lr)] = acc[r];
+          for (int pc = 0; pc < cc; ++pc) acc = PT::minus_prod(pan(pc, i), pan(pc, kk), lr, lq, acc);
           if (i == kk) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) D[MF<Real>::row(r, lq) * kLS + lr] = acc[r];
+          } else {
+            Real* pi = pan(cc, i);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) pi[PT::idx(MF<Real>::row(r, lq), lr)] = acc[r];
           }
         }
         __syncthreads();
@@ -399,7 +410,7 @@ __device__ void factor_block(
 #pragma unroll
               for (int q = 0; q < 4; ++q) av[g][q] = tik[MF<Real>::off(lr, 4 * q + lq)];
             } else {
-              const Real* pi = P + (cc * KT + i) * PT::kT;
+              const Real* pi = pan(cc, i);
 #pragma unroll
               for (int q = 0; q < 4; ++q) av[g][q] = pi[PT::idx(lr, 4 * q + lq)];
             }
@@ -413,7 +424,7 @@ __device__ void factor_block(
 #pragma unroll
             for (int q = 0; q < 4; ++q) acc = MF<Real>::mma(av[g][q], Li[lr * kLS + 4 * q + lq], acc);
             Real* tik = Wb + (long long)ltile(i, kk) * kTE;
-            Real* pi = P + (cc * KT + i) * PT::kT;
+            Real* pi = pan(cc, i);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
               tik[r * 64 + lane] = acc[r];
@@ -451,9 +462,7 @@ __device__ void factor_block(
 #pragma unroll
       for (int g = 0; g < kG; ++g) {
         if (tp[g]) {
-          for (int pc = 0; pc < npc; ++pc) {
-            acc[g] = PT::minus_prod(P + (pc * KT + ti[g]) * PT::kT, P + (pc * KT + tj[g]) * PT::kT, lr, lq, acc[g]);
-          }
+          for (int pc = 0; pc < npc; ++pc) acc[g] = PT::minus_prod(pan(pc, ti[g]), pan(pc, tj[g]), lr, lq, acc[g]);
 #pragma unroll
           for (int r = 0; r < 4; ++r) tp[g][r * 64 + lane] = acc[g][r];
           if (ti[g] == j0 && tj[g] == j0) {  // next diagonal tile: hand it over in D
