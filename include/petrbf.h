@@ -61,7 +61,10 @@ typedef enum { PRBF_KSP_GMRES = 0, PRBF_KSP_FGMRES = 1 } prbf_ksp;
 /* LEARNED: the network of prbf_learned_attach() on the clusters it accepted,
  * exact FP32 blocks on the others; FGMRES only. */
 typedef enum { PRBF_PC_NONE = 0, PRBF_PC_RASM_FP64 = 1, PRBF_PC_RASM_FP32 = 2, PRBF_PC_LEARNED = 3 } prbf_pc;
-typedef enum { PRBF_TUNE_CHOL_WAVES = 1, PRBF_TUNE_CHOL_PANEL = 2 } prbf_tune_key;
+typedef enum { PRBF_TUNE_CHOL_WAVES = 1, PRBF_TUNE_CHOL_PANEL = 2, PRBF_TUNE_CHOL_ORDER = 3 } prbf_tune_key;
+/* Values of PRBF_TUNE_CHOL_ORDER: right-looking (trailing updates) or
+ * left-looking (each panel updated from the finished L rows). */
+typedef enum { PRBF_CHOL_RIGHT = 1, PRBF_CHOL_LEFT = 2 } prbf_chol_order;
 
 /* Solver options; prbf_solve_opts_default() gives the reference's run line
  * (-ksp_type gmres default restart 30, -pc_type asm -sub_pc_type lu,
@@ -118,8 +121,9 @@ prbf_status prbf_set_points_dev(prbf_ctx* ctx, int64_t ni, const double* xi, con
                                 int64_t nj, const double* xj, const double* yj, const double* zj);
 
 /* info[0..15]: ncell, nx, ny, nz, nb_buffer, nb_trunc, buffer_nnz, trunc_nnz,
- * max_buffer, max_core, max_trunc, ni, nj, and the wavefronts and panel width
- * the last block factorisation used (0 before any), 0.
+ * max_buffer, max_core, max_trunc, ni, nj, and the wavefronts, panel width
+ * and order (prbf_chol_order) the last block factorisation used (0 before
+ * any).
  * geom[0..7]: xmin, ymin, zmin, box, buffer half-width, trunc half-width, the
  * device time (s) of the last block-factorisation launch and its algorithmic
  * FLOPs, sum over blocks of n^3/3 + 2 n^2 T (Cholesky and the two triangular
@@ -160,11 +164,13 @@ prbf_status prbf_rasm_setup(prbf_ctx* ctx);
 prbf_status prbf_rasm_setup_pc(prbf_ctx* ctx, int pc);
 /* Force a launch configuration of the block factorisation instead of the
  * built-in autotuner: PRBF_TUNE_CHOL_WAVES (4, 6 or 8 wavefronts per
- * workgroup), PRBF_TUNE_CHOL_PANEL (1 to 4 16-wide tile columns per panel).
- * FP32 has every pair; FP64 has 4 or 8 wavefronts with 1, 2 or 4 columns, and
- * a forced pair without an FP64 kernel, or whose panel does not fit in LDS,
- * runs as 4 wavefronts with 1 column.  No
- * reference counterpart. */
+ * workgroup), PRBF_TUNE_CHOL_PANEL (1 to 4 16-wide tile columns per panel),
+ * PRBF_TUNE_CHOL_ORDER (prbf_chol_order; right-looking unless set).
+ * Right-looking: FP32 has every pair; FP64 has 4 or 8 wavefronts with 1, 2
+ * or 4 columns.  Left-looking: 4 or 8 wavefronts with 1 or 2 columns in both
+ * precisions.  A forced configuration without a kernel, or whose LDS does not
+ * fit, runs right-looking as 4 wavefronts with 1 column.  No reference
+ * counterpart. */
 prbf_status prbf_set_tuning(prbf_ctx* ctx, int key, int value);
 /* PCApply_ASM (restrict): z = sum_c P_core A_c^{-1} R_buffer r, sorted order. */
 prbf_status prbf_rasm_apply(prbf_ctx* ctx, const double* r_sorted, double* z_sorted);

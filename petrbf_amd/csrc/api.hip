@@ -123,8 +123,9 @@ struct prbf_ctx {
   DBuf<double> V, Z;
   int chol_waves = 8;  // wavefronts per block-factorisation workgroup (when forced)
   int chol_panel = 2;  // tile columns per panel (when forced)
+  int chol_order = PRBF_CHOL_RIGHT;  // right- or left-looking (when forced)
   bool tuning_forced = false;
-  int used_nw = 0, used_npc = 0;  // configuration of the last factorisation
+  int used_nw = 0, used_npc = 0, used_order = 0;  // configuration of the last factorisation
   double t_chol_s = 0, chol_flops = 0;  // its kernel time and algorithmic FLOPs
   int rasm_pc = 0;
   int n_fallback = 0;
@@ -371,14 +372,17 @@ constexpr size_t kMaxLdsBytes = 160 * 1024;
 
 struct CholCfg {
   int nw, npc;
+  bool ll;  // left-looking kernel
 };
 
 // Launch configurations the tuner chooses among (wavefronts per workgroup,
-// tile columns per panel).
+// tile columns per panel, left-looking).
 // (FP32 also 6 wavefronts and 3-column panels; a candidate without a kernel
 // for the precision is skipped.)
-const CholCfg kCholCfgs[] = {{4, 1}, {8, 1}, {4, 2}, {8, 2}, {4, 4}, {8, 4},
-                             {4, 3}, {6, 1}, {6, 2}, {6, 3}, {6, 4}, {8, 3}};
+const CholCfg kCholCfgs[] = {{4, 1, false}, {8, 1, false}, {4, 2, false}, {8, 2, false}, {4, 4, false},
+                             {8, 4, false}, {4, 3, false}, {6, 1, false}, {6, 2, false}, {6, 3, false},
+                             {6, 4, false}, {8, 3, false}, {4, 1, true},  {4, 2, true},  {8, 1, true},
+                             {8, 2, true}};
 
 // Process-wide cache of tuned configurations, keyed by device, precision and
 // the largest block's tile count; shared by every context, so guarded (two
@@ -397,10 +401,10 @@ void launch_blocks(prbf_ctx* c, bool fp32, const CholCfg& cfg, long long nwork, 
                    long long ws_slot, int nmax, int tmax, Real* W, const long long* mptr, Real* Mout, int* status) {
   const Grid& g = c->g;
   if (fp32)
-    block_chol_fp32(c->st, g.dim, cfg.nw, cfg.npc, nwork, order, next, nwg, ws_slot, nmax, tmax, c->bptr, c->bidx,
+    block_chol_fp32(c->st, g.dim, cfg.nw, cfg.npc, cfg.ll, nwork, order, next, nwg, ws_slot, nmax, tmax, c->bptr, c->bidx,
                     c->ista, c->iend, c->xi, c->yi, c->zi, (float*)W, mptr, (float*)Mout, c->prm.sigma, status);
   else
-    block_chol_fp64(c->st, g.dim, cfg.nw, cfg.npc, nwork, order, next, nwg, ws_slot, nmax, tmax, c->bptr, c->bidx,
+    block_chol_fp64(c->st, g.dim, cfg.nw, cfg.npc, cfg.ll, nwork, order, next, nwg, ws_slot, nmax, tmax, c->bptr, c->bidx,
                     c->ista, c->iend, c->xi, c->yi, c->zi, (double*)W, mptr, (double*)Mout, c->prm.sigma,
                     c->d_err);
   PRBF_HIP(hipGetLastError());
@@ -430,14 +434,14 @@ void factor_blocks(prbf_ctx* c, bool fp32, const std::vector<int>& order, int nm
   const long long cap = (long long)(free_b / 3 / sizeof(Real)) / ws_slot;
   need(cap >= 1, PRBF_ERR_NOMEM, "not enough device memory for one RASM block");
   auto workgroups = [&](const CholCfg& cf) {
-    long long w = block_chol_workgroups(fp32, g.dim, cf.nw, cf.npc, nmax, tmax);
+    long long w = block_chol_workgroups(fp32, g.dim, cf.nw, cf.npc, cf.ll, nmax, tmax);
     return (int)std::max<long long>(1, std::min(std::min(w, nwork), cap));
   };
   const int ktmax = (nmax + 15) / 16;
   int dev = 0;
   PRBF_HIP(hipGetDevice(&dev));
   const long long key = dev * 10000000LL + (fp32 ? 1000000LL : 0LL) + ktmax * 1000LL + g.dim;
-  CholCfg cfg{c->chol_waves, c->chol_panel};
+  CholCfg cfg{c->chol_waves, c->chol_panel, c->chol_order == PRBF_CHOL_LEFT};
   bool tune = tune_allowed && !c->tuning_forced;
   if (tune) {
     std::lock_guard<std::mutex> lock(tuned_mutex());
@@ -463,8 +467,8 @@ void factor_blocks(prbf_ctx* c, bool fp32, const std::vector<int>& order, int nm
     const long long ns = std::min<long long>(12288, nwork);
     float best = 1e30f;
     for (const CholCfg& cand : kCholCfgs) {
-      if (block_lds_bytes(fp32, nmax, tmax, cand.nw, cand.npc) > kMaxLdsBytes) continue;
-      if (!block_chol_has_config(fp32, cand.nw, cand.npc)) continue;
+      if (block_lds_bytes(fp32, nmax, tmax, cand.nw, cand.npc, cand.ll) > kMaxLdsBytes) continue;
+      if (!block_chol_has_config(fp32, cand.nw, cand.npc, cand.ll)) continue;
       const int nwg = workgroups(cand);
       W.ensure((size_t)nwg * ws_slot);
       PRBF_HIP(hipEventRecord(c->ev0, c->st));
@@ -484,10 +488,10 @@ void factor_blocks(prbf_ctx* c, bool fp32, const std::vector<int>& order, int nm
       }
     if (!have) tuned_cache().push_back({key, cfg});
   }
-  if (block_lds_bytes(fp32, nmax, tmax, cfg.nw, cfg.npc) > kMaxLdsBytes ||
-      !block_chol_has_config(fp32, cfg.nw, cfg.npc))
-    cfg = CholCfg{4, 1};
-  need(block_lds_bytes(fp32, nmax, tmax, cfg.nw, cfg.npc) <= kMaxLdsBytes, PRBF_ERR_LIMIT,
+  if (block_lds_bytes(fp32, nmax, tmax, cfg.nw, cfg.npc, cfg.ll) > kMaxLdsBytes ||
+      !block_chol_has_config(fp32, cfg.nw, cfg.npc, cfg.ll))
+    cfg = CholCfg{4, 1, false};
+  need(block_lds_bytes(fp32, nmax, tmax, cfg.nw, cfg.npc, cfg.ll) <= kMaxLdsBytes, PRBF_ERR_LIMIT,
        fp32 ? "RASM block too large for the LDS panel of the block factorisation"
             : "RASM block too large for the LDS panel of the FP64 block factorisation");
   const int nwg = workgroups(cfg);
@@ -497,6 +501,7 @@ void factor_blocks(prbf_ctx* c, bool fp32, const std::vector<int>& order, int nm
   PRBF_HIP(hipEventRecord(c->ev1, c->st));
   c->used_nw = cfg.nw;
   c->used_npc = cfg.npc;
+  c->used_order = cfg.ll ? PRBF_CHOL_LEFT : PRBF_CHOL_RIGHT;
   sync(c);
   float ms = 0;
   PRBF_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
@@ -540,7 +545,7 @@ void rasm_setup_impl(prbf_ctx* c, int pc) {
     for (int k : order) sorted[cnt[nb - 1 - (int)(c->h_bptr[k + 1] - c->h_bptr[k])]++] = k;
     order.swap(sorted);
   }
-  need(block_lds_bytes(fp32, c->max_buffer, c->max_core, 4, 1) <= kMaxLdsBytes, PRBF_ERR_LIMIT,
+  need(block_lds_bytes(fp32, c->max_buffer, c->max_core, 4, 1, false) <= kMaxLdsBytes, PRBF_ERR_LIMIT,
        "RASM block too large for the LDS panel of the block factorisation");
   c->rasm_ready = false;
   c->mptr.ensure(nc + 1);
@@ -594,10 +599,11 @@ void rasm_setup_impl(prbf_ctx* c, int pc) {
       DBuf<long long> mp;
       mp.ensure(nc + 1);
       PRBF_HIP(hipMemcpyAsync(mp, h_m64.data(), sizeof(long long) * (nc + 1), hipMemcpyHostToDevice, c->st));
-      const int saved_nw = c->used_nw, saved_npc = c->used_npc;
+      const int saved_nw = c->used_nw, saved_npc = c->used_npc, saved_order = c->used_order;
       factor_blocks<double>(c, false, failed, nmax, tmax, mp, c->M64fb.p, nullptr, false);
       c->used_nw = saved_nw;
       c->used_npc = saved_npc;
+      c->used_order = saved_order;
     }
   }
   int herr = 0;
@@ -1134,7 +1140,8 @@ prbf_status prbf_get_info(prbf_ctx* c, int64_t* info, double* geom) {
     const Grid& g = c->g;
     if (info) {
       int64_t v[16] = {g.ncell, g.nx, g.ny, g.nz, g.nb_buffer, g.nb_trunc, c->buffer_nnz, c->trunc_nnz,
-                       c->max_buffer, c->max_core, c->max_trunc, c->ni, c->nj, c->used_nw, c->used_npc, 0};
+                       c->max_buffer, c->max_core, c->max_trunc, c->ni, c->nj, c->used_nw, c->used_npc,
+                       c->used_order};
       std::memcpy(info, v, sizeof(v));
     }
     if (geom) {
@@ -1261,6 +1268,10 @@ prbf_status prbf_set_tuning(prbf_ctx* c, int key, int value) {
     } else if (key == PRBF_TUNE_CHOL_PANEL) {
       need(value >= 1 && value <= 4, PRBF_ERR_ARG, "panel width must be 1 to 4 tile columns");
       c->chol_panel = value;
+    } else if (key == PRBF_TUNE_CHOL_ORDER) {
+      need(value == PRBF_CHOL_RIGHT || value == PRBF_CHOL_LEFT, PRBF_ERR_ARG,
+           "factorisation order must be PRBF_CHOL_RIGHT or PRBF_CHOL_LEFT");
+      c->chol_order = value;
     } else {
       need(false, PRBF_ERR_ARG, "unknown tuning key");
     }

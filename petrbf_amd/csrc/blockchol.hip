@@ -69,6 +69,27 @@ constexpr int kTE = 256;   // elements per 16x16 tile
 constexpr int kLS = 17;    // LDS row stride (bank-conflict padding)
 constexpr int kLT = 16 * kLS;
 
+// Lane-major tiles (the left-looking factorisation): lane l keeps its four
+// values X[l%16][row(q, l/16)], q = 0..3, contiguous at 4 l.  One vector
+// access per lane is then the A operand of X, the B operand of X^T (both with
+// K index row(q, l/16)) and the accumulator of X^T.  lm_off(a, b) is where
+// element (a, b) sits.
+template <typename Real>
+__device__ inline int lm_off(int a, int b) {
+  if constexpr (sizeof(Real) == 4)
+    return (16 * (b >> 2) + a) * 4 + (b & 3);
+  else
+    return (16 * (b & 3) + a) * 4 + (b >> 2);
+}
+template <typename Real, bool LM>
+__device__ inline int tile_off(int a, int b) {
+  if constexpr (LM)
+    return lm_off<Real>(a, b);
+  else
+    return MF<Real>::off(a, b);
+}
+
+
 // Panel tiles (the current tile columns of L, read by every trailing
 // update) and the product -P_i P_j^T of two of them into an accumulator.
 // FP32: row stride 16 in 16-byte chunks, chunk c of row r stored at chunk
@@ -281,6 +302,152 @@ __host__ __device__ inline size_t blk_lds_bytes(int nmax, int tmax, int nw, int 
   return region + (size_t)kLT * sizeof(T) + 16;
 }
 
+// Phases 2 and 3 of one block whose factor L (tiles in Wb, L_II^{-1} in Wd)
+// is complete: the core rows of A'^{-1} into M.  LM: tiles stored lane-major
+// (lm_off) instead of register-major (MF::off).
+template <typename Real, int NW, bool LM>
+__device__ void core_rows(long long c, const Real* __restrict__ Wb, const Real* __restrict__ Wd,
+                          unsigned char* __restrict__ lds_raw, int n, int T, const long long* __restrict__ mptr,
+                          Real* __restrict__ M, int* __restrict__ blk_status) {
+  typedef typename MF<Real>::v4 v4;
+  const int KT = (n + 15) / 16, TC = (T + 15) / 16;
+  const int ncore0 = n - T;
+  const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+  const int lr = lane & 15, lq = lane >> 4;
+  Real* S = (Real*)lds_raw + (size_t)(16 * KT) * kLS;  // per-wave scratch tiles, past X
+  Real* Sw = S + wave * kLT;
+  // ---- phase 2: X = L^{-T} L^{-1} E_core, 16 right-hand sides at a time,
+  // X in LDS (row stride 17) in the region the panel used
+  Real* X = (Real*)lds_raw;
+  const int I0 = ncore0 / 16;
+  Real* Mc = M + mptr[c];
+  for (int tc = 0; tc < TC; ++tc) {
+    // forward: Y_I = L_II^{-1} (E_I - sum_{I0<=J<I} L_IJ Y_J), tile rows I >= I0
+    if (wave == 0) {
+      for (int I = I0; I < KT; ++I) {
+        v4 acc = {0, 0, 0, 0};
+        for (int J = I0; J < I; ++J) {
+          const Real* tij = Wb + (long long)ltile(I, J) * kTE;
+          if constexpr (LM) {
+            const v4 t = *(const v4*)(tij + lane * 4);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+              acc = MF<Real>::mma(t[q], X[(16 * J + MF<Real>::row(q, lq)) * kLS + lr], acc);
+          } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+              acc = MF<Real>::mma(tij[MF<Real>::off(lr, 4 * q + lq)], X[(16 * J + 4 * q + lq) * kLS + lr], acc);
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int row = 16 * I + MF<Real>::row(r, lq), col = 16 * tc + lr;
+          Real e = (col < T && row == ncore0 + col) ? Real(1) : Real(0);
+          Sw[MF<Real>::row(r, lq) * kLS + lr] = e - acc[r];
+        }
+        wave_fence();
+        const Real* li = Wd + (long long)I * kTE;
+        v4 y = {0, 0, 0, 0};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) y = MF<Real>::mma(li[lr * 16 + 4 * q + lq], Sw[(4 * q + lq) * kLS + lr], y);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) X[(16 * I + MF<Real>::row(r, lq)) * kLS + lr] = y[r];
+        wave_fence();
+      }
+    }
+    __syncthreads();
+    // backward: X_I = L_II^{-T} (Y_I - sum_{J>I} L_JI^T X_J); the sum is split
+    // over the wavefronts and combined by wavefront 0 through the scratch
+    // tiles.  Each wavefront's first kPF tiles of column I-1 (and wavefront
+    // 0's L_{I-1,I-1}^{-1}) are loaded while step I runs, so the global
+    // loads leave the step's critical path.
+    constexpr int kPF = 4;
+    Real nx[kPF][4], nli[4];
+    auto load_col = [&](int I) {
+#pragma unroll
+      for (int g = 0; g < kPF; ++g) {
+        const int J = I + 1 + wave + g * NW;
+        if (J < KT) {
+          const Real* tji = Wb + (long long)ltile(J, I) * kTE;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) nx[g][q] = tji[tile_off<Real, LM>(4 * q + lq, lr)];
+        }
+      }
+      if (wave == 0) {
+        const Real* li = Wd + (long long)I * kTE;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) nli[q] = li[(4 * q + lq) * 16 + lr];
+      }
+    };
+    load_col(KT - 1);
+    for (int I = KT - 1; I >= 0; --I) {
+      Real av[kPF][4], al[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        al[q] = nli[q];
+#pragma unroll
+        for (int g = 0; g < kPF; ++g) av[g][q] = nx[g][q];
+      }
+      if (I > 0) load_col(I - 1);
+      v4 acc = {0, 0, 0, 0};
+#pragma unroll
+      for (int g = 0; g < kPF; ++g) {
+        const int J = I + 1 + wave + g * NW;
+        if (J < KT) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) acc = MF<Real>::mma(av[g][q], X[(16 * J + 4 * q + lq) * kLS + lr], acc);
+        }
+      }
+      for (int J = I + 1 + wave + kPF * NW; J < KT; J += NW) {  // (blocks beyond kPF x NW tiles)
+        const Real* tji = Wb + (long long)ltile(J, I) * kTE;
+        Real a[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a[q] = tji[tile_off<Real, LM>(4 * q + lq, lr)];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc = MF<Real>::mma(a[q], X[(16 * J + 4 * q + lq) * kLS + lr], acc);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Sw[r * 64 + lane] = acc[r];
+      __syncthreads();
+      if (wave == 0) {
+        Real rv[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          Real sum = S[r * 64 + lane];
+          for (int w = 1; w < NW; ++w) sum += S[w * kLT + r * 64 + lane];
+          Real yv = (I >= I0) ? X[(16 * I + MF<Real>::row(r, lq)) * kLS + lr] : Real(0);
+          rv[r] = yv - sum;
+        }
+        wave_fence();
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Sw[MF<Real>::row(r, lq) * kLS + lr] = rv[r];
+        wave_fence();
+        v4 x = {0, 0, 0, 0};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) x = MF<Real>::mma(al[q], Sw[(4 * q + lq) * kLS + lr], x);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) X[(16 * I + MF<Real>::row(r, lq)) * kLS + lr] = x[r];
+      }
+      __syncthreads();
+    }
+    // ---- phase 3: M_c[16 tc + tt][b] = X[pos(b)][tt], rows past T zero, in
+    // the apply's storage order (apply_offset), written contiguously
+    const int tcn = min(16, T - 16 * tc);
+    constexpr int V = 16 / (int)sizeof(Real);
+    Real* Mk = Mc + (long long)tc * apply_chunk_elems(n);
+    for (int e = tid; e < 16 * n; e += NW * 64) {
+      const int g = e / 1024, rr = e % 1024;
+      const int ng = min(64, n - 64 * g);
+      const int q = rr / (ng * V), rem = rr % (ng * V);
+      const int b = 64 * g + rem / V, tt = q * V + rem % V;
+      const int p = b < T ? ncore0 + b : b - T;
+      Mk[e] = tt < tcn ? X[p * kLS + tt] : Real(0);
+    }
+    __syncthreads();
+  }
+  if (blk_status && tid == 0) blk_status[c] = 0;
+}
+
 // One block: cluster c, workspace Wb (blk_ws elements), LDS as laid out below.
 template <typename Real, int DIM, int NW, int NPC>
 __device__ void factor_block(
@@ -294,7 +461,7 @@ __device__ void factor_block(
   const int n = (int)(bptr[c + 1] - b0);
   if (n == 0) return;
   const int T = iend[c] - ista[c] + 1;
-  const int KT = (n + 15) / 16, NP = 16 * KT, TC = (T + 15) / 16;
+  const int KT = (n + 15) / 16, NP = 16 * KT;
   const int nt = KT * (KT + 1) / 2;
   const int ncore0 = n - T;
   const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
@@ -316,11 +483,9 @@ __device__ void factor_block(
   double* cx = (double*)lds_raw;          // phase 0 coordinates (same region)
   double* cy = cx + NP;
   double* cz = cy + NP;
-  Real* S = (Real*)lds_raw + (size_t)NP * kLS;  // per-wave scratch tiles, past X
   Real* D = (Real*)(lds_raw + region);       // diagonal tile L_kk, then L_kk^{-1}
   Real* Li = D;
   int* flag = (int*)(D + kLT);
-  Real* Sw = S + wave * kLT;
 
   // ---- phase 0: coordinates (core last) and the lower tiles of A'
   for (int p = tid; p < n; p += NW * 64) {
@@ -482,129 +647,270 @@ __device__ void factor_block(
     return;
   }
 
-  // ---- phase 2: X = L^{-T} L^{-1} E_core, 16 right-hand sides at a time,
-  // X in LDS (row stride 17) in the region the panel used
-  Real* X = (Real*)lds_raw;
-  const int I0 = ncore0 / 16;
-  Real* Mc = M + mptr[c];
-  for (int tc = 0; tc < TC; ++tc) {
-    // forward: Y_I = L_II^{-1} (E_I - sum_{I0<=J<I} L_IJ Y_J), tile rows I >= I0
-    if (wave == 0) {
-      for (int I = I0; I < KT; ++I) {
-        v4 acc = {0, 0, 0, 0};
-        for (int J = I0; J < I; ++J) {
-          const Real* tij = Wb + (long long)ltile(I, J) * kTE;
+  core_rows<Real, NW, false>(c, Wb, Wd, lds_raw, n, T, mptr, M, blk_status);
+}
+
+// ---- left-looking variant ------------------------------------------------
+// Tiles of L rows the left-looking factorisation holds in LDS: rows k0 ..
+// k0+npc-1 of L left of the panel at tile column k0, the most over its panels.
+__host__ __device__ inline int ll_row_tiles(int KT, int npc) {
+  int m = 0;
+  for (int k0 = 0; k0 < KT; k0 += npc) {
+    const int c = KT - k0 < npc ? KT - k0 : npc;
+    if (c * k0 > m) m = c * k0;
+  }
+  return m;
+}
+
+template <typename T>
+__host__ __device__ inline size_t ll_region(int KT, int nw, int npc) {
+  size_t rows = (size_t)ll_row_tiles(KT, npc) * kTE * sizeof(T);
+  size_t crd = (size_t)3 * 16 * KT * sizeof(double);
+  size_t red = ((size_t)16 * KT * kLS + (size_t)nw * kLT) * sizeof(T);
+  size_t region = rows > crd ? rows : crd;
+  region = region > red ? region : red;
+  return (region + 31) & ~(size_t)31;
+}
+
+template <typename T>
+__host__ __device__ inline size_t ll_lds_bytes(int nmax, int nw, int npc) {
+  const int KT = (nmax + 15) / 16;
+  return ll_region<T>(KT, nw, npc) + (size_t)npc * kLT * sizeof(T) + (size_t)(npc * (npc - 1) / 2) * kTE * sizeof(T) +
+         16;
+}
+
+// One block, left-looking: for each panel of NPC tile columns k0.., the L
+// rows k0..k0+npc-1 left of the panel go to LDS; the panel's diagonal block
+// A_{k0+r, k0+c} - sum_j L_{k0+r,j} L_{k0+c,j}^T is formed from LDS alone and
+// factorised by one wavefront (inverses of the diagonal tiles, L of the
+// block below them); then every tile row i below the diagonal block is
+// brought up to date, A_{i,k} - sum_{j<k0} L_ij L_kj^T, with L_ij streamed
+// once from the workspace and the panel rows from LDS, and finished in
+// registers (the panel's earlier columns, then L_kk^{-1}).  Each L tile is
+// written once and read once per later panel, against a read and a write of
+// every trailing tile per panel in the right-looking order.  All products
+// run transposed, on X^T in accumulators, so every workspace and panel tile
+// access is one vector access per lane (lane-major tiles, lm_off).
+template <typename Real, int DIM, int NW, int NPC>
+__device__ void factor_block_ll(
+    long long c, Real* __restrict__ Wb, unsigned char* __restrict__ lds_raw, const long long* __restrict__ bptr,
+    const int* __restrict__ bidx, const int* __restrict__ ista, const int* __restrict__ iend,
+    const double* __restrict__ xs, const double* __restrict__ ys, const double* __restrict__ zs,
+    const long long* __restrict__ mptr, Real* __restrict__ M, double inv_two_s2, double norm, int* __restrict__ err,
+    int* __restrict__ blk_status) {
+  typedef typename MF<Real>::v4 v4;
+  constexpr int G = 4;  // tile rows per wavefront pass
+  const long long b0 = bptr[c];
+  const int n = (int)(bptr[c + 1] - b0);
+  if (n == 0) return;
+  const int T = iend[c] - ista[c] + 1;
+  const int KT = (n + 15) / 16, NP = 16 * KT;
+  const int nt = KT * (KT + 1) / 2;
+  const int ncore0 = n - T;
+  const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+  const int lr = lane & 15, lq = lane >> 4;
+  Real* Wd = Wb + (long long)nt * kTE;  // L_II^{-1}, row-major
+
+  // LDS: one region holding, by phase, the coordinates (0), the L rows of the
+  // current panel (1), and the right-hand sides with the per-wave scratch
+  // tiles (2, 3); then the panel's diagonal tiles (row-major, each replaced
+  // by its inverse), the L tiles of its diagonal block below the diagonal
+  // (lane-major) and the breakdown flag
+  const size_t region = ll_region<Real>(KT, NW, NPC);
+  Real* Rw = (Real*)lds_raw;
+  double* cx = (double*)lds_raw;
+  double* cy = cx + NP;
+  double* cz = cy + NP;
+  Real* Dg = (Real*)(lds_raw + region);
+  Real* Pd = Dg + NPC * kLT;
+  int* flag = (int*)(Pd + (NPC * (NPC - 1) / 2) * kTE);
+  auto pd = [&](int r, int q) { return Pd + ((r * (r - 1)) / 2 + q) * kTE; };  // tile (r, q), r > q
+  auto wtile = [&](int I, int J) { return Wb + (long long)ltile(I, J) * kTE; };
+
+  // ---- phase 0: coordinates (core last) and the lower tiles of A', lane-major
+  for (int p = tid; p < n; p += NW * 64) {
+    int b = p < ncore0 ? p + T : p - ncore0;
+    int j = bidx[b0 + b];
+    cx[p] = xs[j];
+    cy[p] = ys[j];
+    cz[p] = DIM == 3 ? zs[j] : 0.0;
+  }
+  if (tid == 0) *flag = 0;
+  __syncthreads();
+  for (int t = wave; t < nt; t += NW) {
+    int I, J;
+    tri_decode(t, &I, &J);
+    const int i = 16 * I + lr;
+    v4 v;
 #pragma unroll
-          for (int q = 0; q < 4; ++q)
-            acc = MF<Real>::mma(tij[MF<Real>::off(lr, 4 * q + lq)], X[(16 * J + 4 * q + lq) * kLS + lr], acc);
-        }
+    for (int q = 0; q < 4; ++q) {
+      const int j = 16 * J + MF<Real>::row(q, lq);
+      if (i < n && j < n)
+        v[q] = gauss_entry<Real>(cx[i] - cx[j], cy[i] - cy[j], DIM == 3 ? cz[i] - cz[j] : 0.0, inv_two_s2, norm);
+      else
+        v[q] = (i == j) ? Real(1) : Real(0);
+    }
+    *(v4*)(Wb + (long long)t * kTE + lane * 4) = v;
+  }
+  __syncthreads();
+
+  // ---- phase 1: left-looking Cholesky over panels of NPC tile columns
+  for (int k0 = 0; k0 < KT; k0 += NPC) {
+    const int npc = min(NPC, KT - k0);
+    // the panel's L rows left of it, row r at Rw + r k0 tiles (rows are
+    // contiguous in the workspace)
+    for (int r = 0; r < npc; ++r) {
+      const Real* src = wtile(k0 + r, 0);
+      Real* dst = Rw + (size_t)r * k0 * kTE;
+      for (int e = tid * 4; e < k0 * kTE; e += NW * 256) *(v4*)(dst + e) = *(const v4*)(src + e);
+    }
+    __syncthreads();
+    // diagonal block, one tile per wavefront: acc = (A_{k0+r,k0+q} - sum_j
+    // L_{k0+r,j} L_{k0+q,j}^T)^T
+    for (int u = wave; u < npc * (npc + 1) / 2; u += NW) {
+      int r, q2;
+      tri_decode(u, &r, &q2);
+      v4 acc = *(const v4*)(wtile(k0 + r, k0 + q2) + lane * 4);
+      const Real* ra = Rw + (size_t)q2 * k0 * kTE + lane * 4;
+      const Real* rb = Rw + (size_t)r * k0 * kTE + lane * 4;
+      for (int j = 0; j < k0; ++j) {
+        const v4 a = *(const v4*)(ra + j * kTE);
+        const v4 b = *(const v4*)(rb + j * kTE);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc = MF<Real>::mma(-a[q], b[q], acc);
+      }
+      if (r == q2) {
+        Real* D = Dg + r * kLT;
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) D[MF<Real>::row(rr, lq) * kLS + lr] = acc[rr];
+      } else {
+        *(v4*)(pd(r, q2) + lane * 4) = acc;
+      }
+    }
+    __syncthreads();
+    // factorise the diagonal block (one wavefront): per column q2 the inverse
+    // of its diagonal tile, L below it (L^T = L_kk^{-1} A^T), and the update
+    // of the block's later columns
+    if (wave == 0) {
+      for (int q2 = 0; q2 < npc; ++q2) {
+        Real* D = Dg + q2 * kLT;
+        wave_fence();
+        if (diag_inverse<Real, kLS, kLS>(D, D, lane) && lane == 0) *flag = 1;
+        wave_fence();
+        Real* wd = Wd + (long long)(k0 + q2) * kTE;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          int row = 16 * I + MF<Real>::row(r, lq), col = 16 * tc + lr;
-          Real e = (col < T && row == ncore0 + col) ? Real(1) : Real(0);
-          Sw[MF<Real>::row(r, lq) * kLS + lr] = e - acc[r];
+          const int e = r * 64 + lane;
+          wd[e] = D[(e >> 4) * kLS + (e & 15)];
+        }
+        Real li[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) li[q] = D[lr * kLS + MF<Real>::row(q, lq)];
+        for (int r = q2 + 1; r < npc; ++r) {
+          const v4 u = *(const v4*)(pd(r, q2) + lane * 4);
+          v4 v = {0, 0, 0, 0};
+#pragma unroll
+          for (int q = 0; q < 4; ++q) v = MF<Real>::mma(li[q], u[q], v);
+          *(v4*)(pd(r, q2) + lane * 4) = v;
+          *(v4*)(wtile(k0 + r, k0 + q2) + lane * 4) = v;
         }
         wave_fence();
-        const Real* li = Wd + (long long)I * kTE;
-        v4 y = {0, 0, 0, 0};
+        for (int r = q2 + 1; r < npc; ++r) {
+          const v4 b = *(const v4*)(pd(r, q2) + lane * 4);
+          for (int r2 = q2 + 1; r2 <= r; ++r2) {
+            const v4 a = *(const v4*)(pd(r2, q2) + lane * 4);
+            if (r2 == r) {
+              Real* Dr = Dg + r * kLT;
+              v4 acc;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) y = MF<Real>::mma(li[lr * 16 + 4 * q + lq], Sw[(4 * q + lq) * kLS + lr], y);
+              for (int rr = 0; rr < 4; ++rr) acc[rr] = Dr[MF<Real>::row(rr, lq) * kLS + lr];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) X[(16 * I + MF<Real>::row(r, lq)) * kLS + lr] = y[r];
+              for (int q = 0; q < 4; ++q) acc = MF<Real>::mma(-a[q], b[q], acc);
+#pragma unroll
+              for (int rr = 0; rr < 4; ++rr) Dr[MF<Real>::row(rr, lq) * kLS + lr] = acc[rr];
+            } else {
+              v4 acc = *(const v4*)(pd(r, r2) + lane * 4);
+#pragma unroll
+              for (int q = 0; q < 4; ++q) acc = MF<Real>::mma(-a[q], b[q], acc);
+              *(v4*)(pd(r, r2) + lane * 4) = acc;
+            }
+          }
+        }
         wave_fence();
       }
     }
     __syncthreads();
-    // backward: X_I = L_II^{-T} (Y_I - sum_{J>I} L_JI^T X_J); the sum is split
-    // over the wavefronts and combined by wavefront 0 through the scratch
-    // tiles.  Each wavefront's first kPF tiles of column I-1 (and wavefront
-    // 0's L_{I-1,I-1}^{-1}) are loaded while step I runs, so the global
-    // loads leave the step's critical path.
-    constexpr int kPF = 4;
-    Real nx[kPF][4], nli[4];
-    auto load_col = [&](int I) {
+    // tile rows below the diagonal block, G per wavefront pass: the update
+    // from the L rows left of the panel, then per panel column the update
+    // from the panel's earlier columns and the TRSM, all in registers
+    for (int ib = k0 + npc + wave; ib < KT; ib += NW * G) {
+      v4 acc[G][NPC];
 #pragma unroll
-      for (int g = 0; g < kPF; ++g) {
-        const int J = I + 1 + wave + g * NW;
-        if (J < KT) {
-          const Real* tji = Wb + (long long)ltile(J, I) * kTE;
+      for (int g = 0; g < G; ++g) {
+        const int i = ib + g * NW;
 #pragma unroll
-          for (int q = 0; q < 4; ++q) nx[g][q] = tji[MF<Real>::off(4 * q + lq, lr)];
+        for (int q2 = 0; q2 < NPC; ++q2)
+          if (i < KT && q2 < npc) acc[g][q2] = *(const v4*)(wtile(i, k0 + q2) + lane * 4);
+      }
+      for (int j = 0; j < k0; ++j) {
+        v4 b[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+          if (ib + g * NW < KT) b[g] = *(const v4*)(wtile(ib + g * NW, j) + lane * 4);
+#pragma unroll
+        for (int q2 = 0; q2 < NPC; ++q2) {
+          if (q2 < npc) {
+            const v4 a = *(const v4*)(Rw + ((size_t)q2 * k0 + j) * kTE + lane * 4);
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+              if (ib + g * NW < KT) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[g][q2] = MF<Real>::mma(-a[q], b[g][q], acc[g][q2]);
+              }
+          }
         }
       }
-      if (wave == 0) {
-        const Real* li = Wd + (long long)I * kTE;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) nli[q] = li[(4 * q + lq) * 16 + lr];
-      }
-    };
-    load_col(KT - 1);
-    for (int I = KT - 1; I >= 0; --I) {
-      Real av[kPF][4], al[4];
+      for (int q2 = 0; q2 < NPC; ++q2) {
+        if (q2 < npc) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        al[q] = nli[q];
+          for (int p2 = 0; p2 < q2; ++p2) {
+            const v4 a = *(const v4*)(pd(q2, p2) + lane * 4);
 #pragma unroll
-        for (int g = 0; g < kPF; ++g) av[g][q] = nx[g][q];
-      }
-      if (I > 0) load_col(I - 1);
-      v4 acc = {0, 0, 0, 0};
+            for (int g = 0; g < G; ++g)
+              if (ib + g * NW < KT) {
 #pragma unroll
-      for (int g = 0; g < kPF; ++g) {
-        const int J = I + 1 + wave + g * NW;
-        if (J < KT) {
+                for (int q = 0; q < 4; ++q) acc[g][q2] = MF<Real>::mma(-a[q], acc[g][p2][q], acc[g][q2]);
+              }
+          }
+          const Real* D = Dg + q2 * kLT;
+          Real li[4];
 #pragma unroll
-          for (int q = 0; q < 4; ++q) acc = MF<Real>::mma(av[g][q], X[(16 * J + 4 * q + lq) * kLS + lr], acc);
+          for (int q = 0; q < 4; ++q) li[q] = D[lr * kLS + MF<Real>::row(q, lq)];
+#pragma unroll
+          for (int g = 0; g < G; ++g) {
+            const int i = ib + g * NW;
+            if (i < KT) {
+              v4 v = {0, 0, 0, 0};
+#pragma unroll
+              for (int q = 0; q < 4; ++q) v = MF<Real>::mma(li[q], acc[g][q2][q], v);
+              acc[g][q2] = v;
+              *(v4*)(wtile(i, k0 + q2) + lane * 4) = v;
+            }
+          }
         }
       }
-      for (int J = I + 1 + wave + kPF * NW; J < KT; J += NW) {  // (blocks beyond kPF x NW tiles)
-        const Real* tji = Wb + (long long)ltile(J, I) * kTE;
-        Real a[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) a[q] = tji[MF<Real>::off(4 * q + lq, lr)];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) acc = MF<Real>::mma(a[q], X[(16 * J + 4 * q + lq) * kLS + lr], acc);
-      }
-#pragma unroll
-      for (int r = 0; r < 4; ++r) Sw[r * 64 + lane] = acc[r];
-      __syncthreads();
-      if (wave == 0) {
-        Real rv[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          Real sum = S[r * 64 + lane];
-          for (int w = 1; w < NW; ++w) sum += S[w * kLT + r * 64 + lane];
-          Real yv = (I >= I0) ? X[(16 * I + MF<Real>::row(r, lq)) * kLS + lr] : Real(0);
-          rv[r] = yv - sum;
-        }
-        wave_fence();
-#pragma unroll
-        for (int r = 0; r < 4; ++r) Sw[MF<Real>::row(r, lq) * kLS + lr] = rv[r];
-        wave_fence();
-        v4 x = {0, 0, 0, 0};
-#pragma unroll
-        for (int q = 0; q < 4; ++q) x = MF<Real>::mma(al[q], Sw[(4 * q + lq) * kLS + lr], x);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) X[(16 * I + MF<Real>::row(r, lq)) * kLS + lr] = x[r];
-      }
-      __syncthreads();
-    }
-    // ---- phase 3: M_c[16 tc + tt][b] = X[pos(b)][tt], rows past T zero, in
-    // the apply's storage order (apply_offset), written contiguously
-    const int tcn = min(16, T - 16 * tc);
-    constexpr int V = 16 / (int)sizeof(Real);
-    Real* Mk = Mc + (long long)tc * apply_chunk_elems(n);
-    for (int e = tid; e < 16 * n; e += NW * 64) {
-      const int g = e / 1024, rr = e % 1024;
-      const int ng = min(64, n - 64 * g);
-      const int q = rr / (ng * V), rem = rr % (ng * V);
-      const int b = 64 * g + rem / V, tt = q * V + rem % V;
-      const int p = b < T ? ncore0 + b : b - T;
-      Mk[e] = tt < tcn ? X[p * kLS + tt] : Real(0);
     }
     __syncthreads();
   }
-  if (blk_status && tid == 0) blk_status[c] = 0;
+  if (*flag) {
+    if (tid == 0) {
+      if (blk_status) blk_status[c] = 1;
+      else atomicOr(err, 2);
+    }
+    return;
+  }
+  core_rows<Real, NW, true>(c, Wb, Wd, lds_raw, n, T, mptr, M, blk_status);
 }
 
 // Persistent batched launch: each workgroup owns one workspace slot and takes
@@ -614,7 +920,7 @@ __device__ void factor_block(
 // Register caps the compiler meets without spilling: FP32 at most 168 VGPRs
 // (three wavefronts per SIMD), FP64 at most 256 (two), so that LDS, not
 // registers, bounds the workgroups per CU.
-template <typename Real, int DIM, int NW, int NPC>
+template <typename Real, int DIM, int NW, int NPC, bool LL>
 __global__ __launch_bounds__(NW * 64, sizeof(Real) == 4 ? 3 : 2) void k_block_chol(
     long long nwork, const int* __restrict__ order, int* __restrict__ next, long long ws_slot,
     const long long* __restrict__ bptr, const int* __restrict__ bidx, const int* __restrict__ ista,
@@ -630,29 +936,43 @@ __global__ __launch_bounds__(NW * 64, sizeof(Real) == 4 ? 3 : 2) void k_block_ch
     const long long item = *s_item;
     __syncthreads();
     if (item >= nwork) break;
-    factor_block<Real, DIM, NW, NPC>(order[item], Wb, lds_raw, bptr, bidx, ista, iend, xs, ys, zs, mptr, M,
-                                     inv_two_s2, norm, err, blk_status);
+    if constexpr (LL)
+      factor_block_ll<Real, DIM, NW, NPC>(order[item], Wb, lds_raw, bptr, bidx, ista, iend, xs, ys, zs, mptr, M,
+                                          inv_two_s2, norm, err, blk_status);
+    else
+      factor_block<Real, DIM, NW, NPC>(order[item], Wb, lds_raw, bptr, bidx, ista, iend, xs, ys, zs, mptr, M,
+                                       inv_two_s2, norm, err, blk_status);
   }
 }
 
 // Instantiated launch configurations (wavefronts per workgroup, tile columns
-// per panel): FP32 also has 6-wavefront workgroups (two per CU fill its
-// three wavefronts per SIMD) and 3-column panels (two per CU in LDS).
+// per panel, left-looking): FP32 right-looking also has 6-wavefront
+// workgroups (two per CU fill its three wavefronts per SIMD) and 3-column
+// panels (two per CU in LDS); the left-looking kernels have 4 or 8
+// wavefronts with 1 or 2 columns.
 template <typename T, int D>
-static const void* chol_fn_dim(int nw, int npc) {
-#define PRBF_FN(NWV, NPCV) \
-  if (nw == NWV && npc == NPCV) return (const void*)k_block_chol<T, D, NWV, NPCV>;
-  PRBF_FN(4, 1) PRBF_FN(4, 2) PRBF_FN(4, 4) PRBF_FN(8, 1) PRBF_FN(8, 2) PRBF_FN(8, 4)
+static const void* chol_fn_dim(int nw, int npc, bool ll) {
+#define PRBF_FN(NWV, NPCV, LLV) \
+  if (nw == NWV && npc == NPCV && ll == LLV) return (const void*)k_block_chol<T, D, NWV, NPCV, LLV>;
+  PRBF_FN(4, 1, false) PRBF_FN(4, 2, false) PRBF_FN(4, 4, false) PRBF_FN(8, 1, false) PRBF_FN(8, 2, false)
+  PRBF_FN(8, 4, false)
+  PRBF_FN(4, 1, true) PRBF_FN(4, 2, true) PRBF_FN(8, 1, true) PRBF_FN(8, 2, true)
   if constexpr (sizeof(T) == 4) {
-    PRBF_FN(4, 3) PRBF_FN(6, 1) PRBF_FN(6, 2) PRBF_FN(6, 3) PRBF_FN(6, 4) PRBF_FN(8, 3)
+    PRBF_FN(4, 3, false) PRBF_FN(6, 1, false) PRBF_FN(6, 2, false) PRBF_FN(6, 3, false) PRBF_FN(6, 4, false)
+    PRBF_FN(8, 3, false)
   }
 #undef PRBF_FN
   return nullptr;
 }
 
 template <typename T>
-static const void* chol_fn(int dim, int nw, int npc) {
-  return dim == 3 ? chol_fn_dim<T, 3>(nw, npc) : chol_fn_dim<T, 2>(nw, npc);
+static const void* chol_fn(int dim, int nw, int npc, bool ll) {
+  return dim == 3 ? chol_fn_dim<T, 3>(nw, npc, ll) : chol_fn_dim<T, 2>(nw, npc, ll);
+}
+
+template <typename T>
+static size_t chol_lds_bytes(int nmax, int tmax, int nw, int npc, bool ll) {
+  return ll ? ll_lds_bytes<T>(nmax, nw, npc) : blk_lds_bytes<T>(nmax, tmax, nw, npc);
 }
 
 // Raises the dynamic-LDS limit of a kernel to the CU's 160 KB, once per
@@ -670,11 +990,11 @@ static void set_lds_attr(const void* fn) {
 }
 
 template <typename T>
-static int resident_workgroups(int dim, int nw, int npc, int nmax, int tmax) {
-  const void* fn = chol_fn<T>(dim, nw, npc);
+static int resident_workgroups(int dim, int nw, int npc, bool ll, int nmax, int tmax) {
+  const void* fn = chol_fn<T>(dim, nw, npc, ll);
   if (!fn) return 0;
   set_lds_attr(fn);
-  size_t lds = blk_lds_bytes<T>(nmax, tmax, nw, npc);
+  size_t lds = chol_lds_bytes<T>(nmax, tmax, nw, npc, ll);
   int per_cu = 0, dev = 0, ncu = 0;
   PRBF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, nw * 64, lds));
   PRBF_HIP(hipGetDevice(&dev));
@@ -683,15 +1003,16 @@ static int resident_workgroups(int dim, int nw, int npc, int nmax, int tmax) {
 }
 
 template <typename T>
-static void launch_block_chol(hipStream_t st, int dim, int nw, int npc, long long nwork, const int* order, int* next,
+static void launch_block_chol(hipStream_t st, int dim, int nw, int npc, bool ll, long long nwork, const int* order,
+                              int* next,
                               int nwg, long long ws_slot, int nmax, int tmax, const long long* bptr, const int* bidx,
                               const int* ista, const int* iend, const double* xs, const double* ys, const double* zs,
                               T* W, const long long* mptr, T* M, double sigma, int* err, int* blk_status) {
-  const void* fn = chol_fn<T>(dim, nw, npc);
+  const void* fn = chol_fn<T>(dim, nw, npc, ll);
   if (!fn) throw ApiError(PRBF_ERR_ARG, "no block factorisation kernel for this launch configuration");
   Kern k = make_kern(sigma);
   set_lds_attr(fn);
-  size_t lds = blk_lds_bytes<T>(nmax, tmax, nw, npc);
+  size_t lds = chol_lds_bytes<T>(nmax, tmax, nw, npc, ll);
   size_t item_slot = lds - 8;  // past the flag word of the largest block
   PRBF_HIP(hipMemsetAsync(next, 0, sizeof(int), st));
   double inv_two_s2 = k.inv_two_s2, norm = k.norm;
@@ -718,32 +1039,32 @@ void failed_blocks(hipStream_t st, long long nwork, const int* order, const int*
 
 long long block_workspace(int n, int T) { return blk_ws(n, T); }
 
-bool block_chol_has_config(bool fp32, int nw, int npc) {
-  return (fp32 ? chol_fn<float>(2, nw, npc) : chol_fn<double>(2, nw, npc)) != nullptr;
+bool block_chol_has_config(bool fp32, int nw, int npc, bool ll) {
+  return (fp32 ? chol_fn<float>(2, nw, npc, ll) : chol_fn<double>(2, nw, npc, ll)) != nullptr;
 }
 
-size_t block_lds_bytes(bool fp32, int nmax, int tmax, int nw, int npc) {
-  return fp32 ? blk_lds_bytes<float>(nmax, tmax, nw, npc) : blk_lds_bytes<double>(nmax, tmax, nw, npc);
+size_t block_lds_bytes(bool fp32, int nmax, int tmax, int nw, int npc, bool ll) {
+  return fp32 ? chol_lds_bytes<float>(nmax, tmax, nw, npc, ll) : chol_lds_bytes<double>(nmax, tmax, nw, npc, ll);
 }
 
-int block_chol_workgroups(bool fp32, int dim, int nw, int npc, int nmax, int tmax) {
-  return fp32 ? resident_workgroups<float>(dim, nw, npc, nmax, tmax)
-              : resident_workgroups<double>(dim, nw, npc, nmax, tmax);
+int block_chol_workgroups(bool fp32, int dim, int nw, int npc, bool ll, int nmax, int tmax) {
+  return fp32 ? resident_workgroups<float>(dim, nw, npc, ll, nmax, tmax)
+              : resident_workgroups<double>(dim, nw, npc, ll, nmax, tmax);
 }
 
-void block_chol_fp64(hipStream_t st, int dim, int nw, int npc, long long nwork, const int* order, int* next, int nwg,
-                     long long ws_slot, int nmax, int tmax, const long long* bptr, const int* bidx, const int* ista,
-                     const int* iend, const double* xs, const double* ys, const double* zs, double* W,
+void block_chol_fp64(hipStream_t st, int dim, int nw, int npc, bool ll, long long nwork, const int* order, int* next,
+                     int nwg, long long ws_slot, int nmax, int tmax, const long long* bptr, const int* bidx,
+                     const int* ista, const int* iend, const double* xs, const double* ys, const double* zs, double* W,
                      const long long* mptr, double* M, double sigma, int* err) {
-  launch_block_chol<double>(st, dim, nw, npc, nwork, order, next, nwg, ws_slot, nmax, tmax, bptr, bidx, ista, iend,
+  launch_block_chol<double>(st, dim, nw, npc, ll, nwork, order, next, nwg, ws_slot, nmax, tmax, bptr, bidx, ista, iend,
                             xs, ys, zs, W, mptr, M, sigma, err, nullptr);
 }
 
-void block_chol_fp32(hipStream_t st, int dim, int nw, int npc, long long nwork, const int* order, int* next, int nwg,
-                     long long ws_slot, int nmax, int tmax, const long long* bptr, const int* bidx, const int* ista,
-                     const int* iend, const double* xs, const double* ys, const double* zs, float* W,
+void block_chol_fp32(hipStream_t st, int dim, int nw, int npc, bool ll, long long nwork, const int* order, int* next,
+                     int nwg, long long ws_slot, int nmax, int tmax, const long long* bptr, const int* bidx,
+                     const int* ista, const int* iend, const double* xs, const double* ys, const double* zs, float* W,
                      const long long* mptr, float* M, double sigma, int* blk_status) {
-  launch_block_chol<float>(st, dim, nw, npc, nwork, order, next, nwg, ws_slot, nmax, tmax, bptr, bidx, ista, iend,
+  launch_block_chol<float>(st, dim, nw, npc, ll, nwork, order, next, nwg, ws_slot, nmax, tmax, bptr, bidx, ista, iend,
                            xs, ys, zs, W, mptr, M, sigma, nullptr, blk_status);
 }
 

@@ -51,25 +51,27 @@ __host__ __device__ inline long long apply_offset(int n, int t, int b) {
   return kc * apply_chunk_elems(n) + 1024LL * g + (long long)((tt / V) * ng + l) * V + tt % V;
 }
 
-// Workspace (elements) and LDS bytes of one block factorisation.
+// Workspace (elements) and LDS bytes of one block factorisation (ll: the
+// left-looking kernels).
 long long block_workspace(int n, int T);
-size_t block_lds_bytes(bool fp32, int nmax, int tmax, int nw, int npc);
+size_t block_lds_bytes(bool fp32, int nmax, int tmax, int nw, int npc, bool ll);
 // Clusters of order[0..nwork) with status[c] != 0 into list (unordered), their number in *count.
 void failed_blocks(hipStream_t st, long long nwork, const int* order, const int* status, int* list, int* count);
+// Whether the block factorisation has a kernel for (precision, wavefronts,
+// panel columns, order).
+bool block_chol_has_config(bool fp32, int nw, int npc, bool ll);
 // Workgroups of the persistent block factorisation that fit on the device at once.
-// Whether the block factorisation has a kernel for (precision, wavefronts, panel columns).
-bool block_chol_has_config(bool fp32, int nw, int npc);
-int block_chol_workgroups(bool fp32, int dim, int nw, int npc, int nmax, int tmax);
+int block_chol_workgroups(bool fp32, int dim, int nw, int npc, bool ll, int nmax, int tmax);
 // Factorises the blocks of clusters order[0..nwork) and writes their core rows:
 // nwg persistent workgroups, each with a workspace slot of ws_slot elements
 // of W, take blocks from the counter *next (reset by the launcher).
-void block_chol_fp64(hipStream_t st, int dim, int nw, int npc, long long nwork, const int* order, int* next, int nwg,
-                     long long ws_slot, int nmax, int tmax, const long long* bptr, const int* bidx, const int* ista,
-                     const int* iend, const double* xs, const double* ys, const double* zs, double* W,
+void block_chol_fp64(hipStream_t st, int dim, int nw, int npc, bool ll, long long nwork, const int* order, int* next,
+                     int nwg, long long ws_slot, int nmax, int tmax, const long long* bptr, const int* bidx,
+                     const int* ista, const int* iend, const double* xs, const double* ys, const double* zs, double* W,
                      const long long* mptr, double* M, double sigma, int* err);
-void block_chol_fp32(hipStream_t st, int dim, int nw, int npc, long long nwork, const int* order, int* next, int nwg,
-                     long long ws_slot, int nmax, int tmax, const long long* bptr, const int* bidx, const int* ista,
-                     const int* iend, const double* xs, const double* ys, const double* zs, float* W,
+void block_chol_fp32(hipStream_t st, int dim, int nw, int npc, bool ll, long long nwork, const int* order, int* next,
+                     int nwg, long long ws_slot, int nmax, int tmax, const long long* bptr, const int* bidx,
+                     const int* ista, const int* iend, const double* xs, const double* ys, const double* zs, float* W,
                      const long long* mptr, float* M, double sigma, int* blk_status);
 void rasm_apply_fp64(hipStream_t st, long long ncell, const long long* bptr, const int* bidx,
                      const int* ista, const int* iend, const long long* mptr, const double* M,

@@ -118,7 +118,7 @@ class Context:
         g = np.zeros(8, np.float64)
         self._check(self._L.prbf_get_info(self._h, _p(a), _p(g)))
         keys = ["ncell", "nx", "ny", "nz", "nb_buffer", "nb_trunc", "buffer_nnz", "trunc_nnz",
-                "max_buffer", "max_core", "max_trunc", "ni", "nj", "chol_waves", "chol_panel"]
+                "max_buffer", "max_core", "max_trunc", "ni", "nj", "chol_waves", "chol_panel", "chol_order"]
         d = {k: int(a[i]) for i, k in enumerate(keys)}
         d.update(xmin=g[0], ymin=g[1], zmin=g[2], box=g[3], buffer_len=g[4], trunc_len=g[5], chol_kernel_s=g[6],
                  chol_flops=g[7])
@@ -194,12 +194,17 @@ class Context:
     def rasm_setup(self, pc="rasm"):
         self._check(self._L.prbf_rasm_setup_pc(self._h, _PC[pc]))
 
-    def set_tuning(self, chol_waves=None, chol_panel=None):
-        """Force the block-factorisation launch configuration (else autotuned)."""
+    def set_tuning(self, chol_waves=None, chol_panel=None, chol_order=None):
+        """Force the block-factorisation launch configuration (else autotuned).
+
+        chol_order: "right" or "left" (right-looking unless set)."""
         if chol_waves is not None:
             self._check(self._L.prbf_set_tuning(self._h, L.PRBF_TUNE_CHOL_WAVES, int(chol_waves)))
         if chol_panel is not None:
             self._check(self._L.prbf_set_tuning(self._h, L.PRBF_TUNE_CHOL_PANEL, int(chol_panel)))
+        if chol_order is not None:
+            v = {"right": L.PRBF_CHOL_RIGHT, "left": L.PRBF_CHOL_LEFT}[chol_order]
+            self._check(self._L.prbf_set_tuning(self._h, L.PRBF_TUNE_CHOL_ORDER, v))
 
     def rasm_apply(self, r_sorted):
         r = _f64(r_sorted)

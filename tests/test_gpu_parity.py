@@ -119,14 +119,21 @@ def test_rasm_core_rows_are_inverse_rows():
     ctx.close()
 
 
-def test_rasm_apply_matches_oracle_lu():
-    _announce("rasm apply")
+@pytest.mark.parametrize("order", [None, "right", "left"])
+def test_rasm_apply_matches_oracle_lu(order):
+    """The FP64 preconditioner (autotuned, or forced right- or left-looking)
+    against the oracle's LU of every block."""
+    _announce(f"rasm apply {order}")
     p = lattice(40)
     o = Oracle.from_problem(p)
     o.rasm_setup()
     ctx = Context.from_problem(p)
+    if order:
+        ctx.set_tuning(chol_waves=4, chol_panel=2, chol_order=order)
     ctx.set_points(p.x, p.y)
     ctx.rasm_setup()
+    if order:
+        assert ctx.info()["chol_order"] == (1 if order == "right" else 2)
     rng = np.random.default_rng(3)
     r = rng.standard_normal(p.n)
     want = o.rasm_apply(r)
@@ -238,27 +245,32 @@ def test_mixed_core_rows_close_to_fp64():
 
 @pytest.mark.parametrize("prec", ["rasm", "rasm32"])
 def test_block_factorisation_launch_configs_agree(prec):
-    """Every launch configuration the autotuner may pick (FP64: 4/8
-    wavefronts, 1/2/4 tile columns per panel; FP32 also 6 wavefronts and
-    3-column panels) yields the same preconditioner."""
+    """Every launch configuration the autotuner may pick (right-looking FP64:
+    4/8 wavefronts, 1/2/4 tile columns per panel; FP32 also 6 wavefronts and
+    3-column panels; left-looking: 4/8 wavefronts, 1/2 columns) yields the
+    same preconditioner."""
     _announce(f"chol configs {prec}")
     p = lattice(40)
     rng = np.random.default_rng(2)
     r = rng.standard_normal(p.n)
     outs = []
     fp32 = prec == "rasm32"
-    for nw in ((4, 6, 8) if fp32 else (4, 8)):
-        for npc in ((1, 2, 3, 4) if fp32 else (1, 2, 4)):
-            ctx = Context.from_problem(p)
-            ctx.set_tuning(chol_waves=nw, chol_panel=npc)
-            ctx.set_points(p.x, p.y)
-            ctx.rasm_setup(prec)
-            used = (ctx.info()["chol_waves"], ctx.info()["chol_panel"])
-            # a forced configuration whose panel does not fit in LDS (FP64 with
-            # 4-column panels at these block sizes) is replaced by one that does
-            assert used == (nw, npc) or (prec == "rasm" and npc == 4), used
-            outs.append(ctx.rasm_apply(r))
-            ctx.close()
+    cfgs = [("right", nw, npc) for nw in ((4, 6, 8) if fp32 else (4, 8))
+            for npc in ((1, 2, 3, 4) if fp32 else (1, 2, 4))]
+    cfgs += [("left", nw, npc) for nw in (4, 8) for npc in (1, 2)]
+    for order, nw, npc in cfgs:
+        ctx = Context.from_problem(p)
+        ctx.set_tuning(chol_waves=nw, chol_panel=npc, chol_order=order)
+        ctx.set_points(p.x, p.y)
+        ctx.rasm_setup(prec)
+        info = ctx.info()
+        used = (info["chol_order"], info["chol_waves"], info["chol_panel"])
+        # a forced configuration whose panel does not fit in LDS (FP64 with
+        # 4-column panels at these block sizes) is replaced by one that does
+        want = (1 if order == "right" else 2, nw, npc)
+        assert used == want or (prec == "rasm" and npc == 4), (used, want)
+        outs.append(ctx.rasm_apply(r))
+        ctx.close()
     tol = 1e-9 if prec == "rasm" else 1e-2
     for o in outs[1:]:
         assert np.linalg.norm(o - outs[0]) <= tol * np.linalg.norm(outs[0])
