@@ -787,6 +787,48 @@ __device__ void factor_block_ll(
         *(v4*)(pd(r, q2) + lane * 4) = acc;
       }
     }
+    // tile rows below the diagonal block, G per wavefront pass: the update
+    // from the L rows left of the panel (L_ij streamed from the workspace,
+    // the next column's tiles loaded during the current column's products),
+    // then per panel column the update from the panel's earlier columns and
+    // the TRSM, all in registers.  The first pass streams its products while
+    // the diagonal block is factorised.
+    v4 acc[G][NPC];
+    auto stream = [&](int ib) {
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int i = ib + g * NW;
+#pragma unroll
+        for (int q2 = 0; q2 < NPC; ++q2)
+          if (i < KT && q2 < npc) acc[g][q2] = *(const v4*)(wtile(i, k0 + q2) + lane * 4);
+      }
+      v4 b[G];
+#pragma unroll
+      for (int g = 0; g < G; ++g)
+        if (k0 > 0 && ib + g * NW < KT) b[g] = *(const v4*)(wtile(ib + g * NW, 0) + lane * 4);
+      for (int j = 0; j < k0; ++j) {
+        v4 bn[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+          if (j + 1 < k0 && ib + g * NW < KT) bn[g] = *(const v4*)(wtile(ib + g * NW, j + 1) + lane * 4);
+#pragma unroll
+        for (int q2 = 0; q2 < NPC; ++q2) {
+          if (q2 < npc) {
+            const v4 a = *(const v4*)(Rw + ((size_t)q2 * k0 + j) * kTE + lane * 4);
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+              if (ib + g * NW < KT) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[g][q2] = MF<Real>::mma(-a[q], b[g][q], acc[g][q2]);
+              }
+          }
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) b[g] = bn[g];
+      }
+    };
+    const int ib0 = k0 + npc + wave;
+    if (ib0 < KT) stream(ib0);
     __syncthreads();
     // factorise the diagonal block (one wavefront): per column q2 the inverse
     // of its diagonal tile, L below it (L^T = L_kk^{-1} A^T), and the update
@@ -840,36 +882,8 @@ __device__ void factor_block_ll(
       }
     }
     __syncthreads();
-    // tile rows below the diagonal block, G per wavefront pass: the update
-    // from the L rows left of the panel, then per panel column the update
-    // from the panel's earlier columns and the TRSM, all in registers
-    for (int ib = k0 + npc + wave; ib < KT; ib += NW * G) {
-      v4 acc[G][NPC];
-#pragma unroll
-      for (int g = 0; g < G; ++g) {
-        const int i = ib + g * NW;
-#pragma unroll
-        for (int q2 = 0; q2 < NPC; ++q2)
-          if (i < KT && q2 < npc) acc[g][q2] = *(const v4*)(wtile(i, k0 + q2) + lane * 4);
-      }
-      for (int j = 0; j < k0; ++j) {
-        v4 b[G];
-#pragma unroll
-        for (int g = 0; g < G; ++g)
-          if (ib + g * NW < KT) b[g] = *(const v4*)(wtile(ib + g * NW, j) + lane * 4);
-#pragma unroll
-        for (int q2 = 0; q2 < NPC; ++q2) {
-          if (q2 < npc) {
-            const v4 a = *(const v4*)(Rw + ((size_t)q2 * k0 + j) * kTE + lane * 4);
-#pragma unroll
-            for (int g = 0; g < G; ++g)
-              if (ib + g * NW < KT) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) acc[g][q2] = MF<Real>::mma(-a[q], b[g][q], acc[g][q2]);
-              }
-          }
-        }
-      }
+    for (int ib = ib0; ib < KT; ib += NW * G) {
+      if (ib != ib0) stream(ib);
 #pragma unroll
       for (int q2 = 0; q2 < NPC; ++q2) {
         if (q2 < npc) {
