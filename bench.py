@@ -95,6 +95,8 @@ def main():
     ap.add_argument("--mode", choices=["plain", "mixed", "both"], default="both")
     ap.add_argument("--chol-waves", type=int, default=0, help="4 or 8 wavefronts per block factorisation")
     ap.add_argument("--chol-panel", type=int, default=0, help="1, 2 or 4 tile columns per panel")
+    ap.add_argument("--chol-order", choices=["right", "left"], default=None,
+                    help="right- or left-looking block factorisation")
     ap.add_argument("--workload", choices=["c2", "c5"], default="c2",
                     help="c2: the 2D N=4M solve (the headline); c5: the 3D 160^3 three-RHS workload "
                          "(lists, batched matvec, FP32 setup, FGMRES progress), its own JSON line")
@@ -138,6 +140,8 @@ def main():
         ctx.set_tuning(chol_waves=args.chol_waves)
     if args.chol_panel:
         ctx.set_tuning(chol_panel=args.chol_panel)
+    if args.chol_order:
+        ctx.set_tuning(chol_order=args.chol_order)
     if args.setup_only:
         ctx.set_points_dev(x, y)
         t0 = time.perf_counter()
@@ -146,7 +150,8 @@ def main():
         ci = ctx.info()
         print(json.dumps({"setup": args.setup_only, "wall_s": time.perf_counter() - t0,
                           "kernel_s": ci["chol_kernel_s"], "flops": ci["chol_flops"],
-                          "chol_waves": ci["chol_waves"], "chol_panel": ci["chol_panel"]}), flush=True)
+                          "chol_waves": ci["chol_waves"], "chol_panel": ci["chol_panel"],
+                          "chol_order": ci["chol_order"]}), flush=True)
         return
 
     modes = {"plain": ("gmres", "rasm"), "mixed": ("fgmres", "rasm32")}
@@ -178,11 +183,13 @@ def main():
         true_rel = float(torch.linalg.norm(wv - w) / torch.linalg.norm(w))
         log(f"{mode}: {args.steps} steps {elapsed:.3f}s; its {inf.its} reason {inf.reason} true rel resid {true_rel:.3e}")
         ci = ctx.info()
-        log(f"{mode}: block factorisation config waves={ci['chol_waves']} panel={ci['chol_panel']}, "
+        log(f"{mode}: block factorisation config waves={ci['chol_waves']} panel={ci['chol_panel']} "
+            f"order={'left' if ci['chol_order'] == 2 else 'right'}, "
             f"kernel {ci['chol_kernel_s']*1e3:.1f} ms, {ci['chol_flops']/ci['chol_kernel_s']/1e12:.1f} TFLOP/s")
-        chol[mode] = (ci["chol_kernel_s"], ci["chol_flops"], ci["chol_waves"], ci["chol_panel"])
+        chol[mode] = (ci["chol_kernel_s"], ci["chol_flops"], ci["chol_waves"], ci["chol_panel"], ci["chol_order"])
         results[mode] = dict(ms_per_step=elapsed / args.steps * 1e3, its=inf.its, reason=inf.reason,
                              chol_waves=ci["chol_waves"], chol_panel=ci["chol_panel"],
+                             chol_order="left" if ci["chol_order"] == 2 else "right",
                              t_setup_s=inf.t_setup_s, t_solve_s=inf.t_solve_s, t_matvec_s=inf.t_matvec_s,
                              n_matvec=inf.n_matvec, t_pc_s=inf.t_pc_s, n_pc=inf.n_pc,
                              n_fallback=inf.n_fallback, true_rel_residual=true_rel)
@@ -262,13 +269,15 @@ def main():
     # the dominant kernel of the headline (mixed) step: the FP32 block
     # factorisation, one persistent launch per setup, timed with HIP events on
     # the library's stream; FLOPs = sum over blocks of n^3/3 + 2 n^2 T
-    for mode, key, peak, name in (("mixed", "roofline", PEAK_FP32_MFMA_TFLOPS, "k_block_chol<float,2,%d,%d>"),
+    # (template arguments: precision, dimension, wavefronts, panel columns,
+    # left-looking)
+    for mode, key, peak, name in (("mixed", "roofline", PEAK_FP32_MFMA_TFLOPS, "k_block_chol<float,2,%d,%d,%s>"),
                                   ("plain", "roofline_block_chol_fp64", PEAK_FP64_TFLOPS,
-                                   "k_block_chol<double,2,%d,%d>")):
+                                   "k_block_chol<double,2,%d,%d,%s>")):
         if mode not in chol:
             continue
-        t_k, fl, nw, npc = chol[mode]
-        kname = name % (nw, npc)
+        t_k, fl, nw, npc, order = chol[mode]
+        kname = name % (nw, npc, "true" if order == 2 else "false")
         res[key] = {"bound": "mfma", "kernel": kname, "achieved": fl / t_k / 1e12, "peak": peak,
                     "unit": "TFLOP/s", "frac": fl / t_k / 1e12 / peak, "avg_ms": t_k * 1e3,
                     "flops_per_launch": fl, "traffic": measured_traffic(kname)}
