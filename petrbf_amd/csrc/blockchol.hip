@@ -280,10 +280,11 @@ __device__ __forceinline__ bool diag_inverse(const Real* D, Real* Li, int lane) 
 
 // Workspace of one block, in elements of T:
 //   lower tiles (nt = KT(KT+1)/2, 256 each) | L_II^{-1} (KT x 256, row-major)
+//   | coordinates, core last (3 x 16 KT doubles, left-looking kernel)
 __host__ __device__ inline long long blk_ws(int n, int T) {
   int KT = (n + 15) / 16;
   (void)T;
-  return (long long)(KT * (KT + 1) / 2 + KT) * kTE;
+  return (long long)(KT * (KT + 1) / 2 + KT) * kTE + 6LL * 16 * KT;
 }
 
 // Panel tiles held in LDS for m columns of a block of KT tile rows: column pc
@@ -669,10 +670,8 @@ __host__ __device__ inline int ll_row_tiles(int KT, int npc) {
 template <typename T>
 __host__ __device__ inline size_t ll_region(int KT, int nw, int npc) {
   size_t rows = (size_t)ll_row_tiles(KT, npc) * kTE * sizeof(T);
-  size_t crd = (size_t)3 * 16 * KT * sizeof(double);
   size_t red = ((size_t)16 * KT * kLS + (size_t)nw * kLT) * sizeof(T);
-  size_t region = rows > crd ? rows : crd;
-  region = region > red ? region : red;
+  size_t region = rows > red ? rows : red;
   return (region + 31) & ~(size_t)31;
 }
 
@@ -680,7 +679,7 @@ template <typename T>
 __host__ __device__ inline size_t ll_lds_bytes(int nmax, int nw, int npc) {
   const int KT = (nmax + 15) / 16;
   return ll_region<T>(KT, nw, npc) + (size_t)npc * kLT * sizeof(T) + (size_t)(npc * (npc - 1) / 2) * kTE * sizeof(T) +
-         16;
+         (size_t)3 * 16 * npc * sizeof(double) + 16;
 }
 
 // One block, left-looking: for each panel of NPC tile columns k0.., the L
@@ -692,8 +691,11 @@ __host__ __device__ inline size_t ll_lds_bytes(int nmax, int nw, int npc) {
 // once from the workspace and the panel rows from LDS, and finished in
 // registers (the panel's earlier columns, then L_kk^{-1}).  Each L tile is
 // written once and read once per later panel, against a read and a write of
-// every trailing tile per panel in the right-looking order.  All products
-// run transposed, on X^T in accumulators, so every workspace and panel tile
+// every trailing tile per panel in the right-looking order.  A' is never
+// stored: its entries are made where they enter, from the block's
+// coordinates (staged in the workspace; the panel columns' copy in LDS),
+// while a pass's first tile loads are in flight.  All products run
+// transposed, on X^T in accumulators, so every workspace and panel tile
 // access is one vector access per lane (lane-major tiles, lm_off).
 template <typename Real, int DIM, int NW, int NPC>
 __device__ void factor_block_ll(
@@ -714,24 +716,25 @@ __device__ void factor_block_ll(
   const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
   const int lr = lane & 15, lq = lane >> 4;
   Real* Wd = Wb + (long long)nt * kTE;  // L_II^{-1}, row-major
-
-  // LDS: one region holding, by phase, the coordinates (0), the L rows of the
-  // current panel (1), and the right-hand sides with the per-wave scratch
-  // tiles (2, 3); then the panel's diagonal tiles (row-major, each replaced
-  // by its inverse), the L tiles of its diagonal block below the diagonal
-  // (lane-major) and the breakdown flag
-  const size_t region = ll_region<Real>(KT, NW, NPC);
-  Real* Rw = (Real*)lds_raw;
-  double* cx = (double*)lds_raw;
+  double* cx = (double*)(Wd + (long long)KT * kTE);  // coordinates, core last
   double* cy = cx + NP;
   double* cz = cy + NP;
+
+  // LDS: one region holding, by phase, the L rows of the current panel (1)
+  // and the right-hand sides with the per-wave scratch tiles (2, 3); then the
+  // panel's diagonal tiles (row-major, each replaced by its inverse), the L
+  // tiles of its diagonal block below the diagonal (lane-major), the panel
+  // columns' coordinates and the breakdown flag
+  const size_t region = ll_region<Real>(KT, NW, NPC);
+  Real* Rw = (Real*)lds_raw;
   Real* Dg = (Real*)(lds_raw + region);
   Real* Pd = Dg + NPC * kLT;
-  int* flag = (int*)(Pd + (NPC * (NPC - 1) / 2) * kTE);
+  double* Cc = (double*)(Pd + (NPC * (NPC - 1) / 2) * kTE);  // x, y, z of the panel's 16 NPC columns
+  int* flag = (int*)(Cc + 3 * 16 * NPC);
   auto pd = [&](int r, int q) { return Pd + ((r * (r - 1)) / 2 + q) * kTE; };  // tile (r, q), r > q
   auto wtile = [&](int I, int J) { return Wb + (long long)ltile(I, J) * kTE; };
 
-  // ---- phase 0: coordinates (core last) and the lower tiles of A', lane-major
+  // ---- phase 0: the block's coordinates, core last, into the workspace
   for (int p = tid; p < n; p += NW * 64) {
     int b = p < ncore0 ? p + T : p - ncore0;
     int j = bidx[b0 + b];
@@ -741,40 +744,48 @@ __device__ void factor_block_ll(
   }
   if (tid == 0) *flag = 0;
   __syncthreads();
-  for (int t = wave; t < nt; t += NW) {
-    int I, J;
-    tri_decode(t, &I, &J);
-    const int i = 16 * I + lr;
-    v4 v;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int j = 16 * J + MF<Real>::row(q, lq);
-      if (i < n && j < n)
-        v[q] = gauss_entry<Real>(cx[i] - cx[j], cy[i] - cy[j], DIM == 3 ? cz[i] - cz[j] : 0.0, inv_two_s2, norm);
-      else
-        v[q] = (i == j) ? Real(1) : Real(0);
-    }
-    *(v4*)(Wb + (long long)t * kTE + lane * 4) = v;
-  }
-  __syncthreads();
 
   // ---- phase 1: left-looking Cholesky over panels of NPC tile columns
   for (int k0 = 0; k0 < KT; k0 += NPC) {
     const int npc = min(NPC, KT - k0);
     // the panel's L rows left of it, row r at Rw + r k0 tiles (rows are
-    // contiguous in the workspace)
+    // contiguous in the workspace), and its columns' coordinates
     for (int r = 0; r < npc; ++r) {
       const Real* src = wtile(k0 + r, 0);
       Real* dst = Rw + (size_t)r * k0 * kTE;
       for (int e = tid * 4; e < k0 * kTE; e += NW * 256) *(v4*)(dst + e) = *(const v4*)(src + e);
     }
+    for (int p = tid; p < 16 * npc; p += NW * 64) {
+      const int gp = 16 * k0 + p;
+      const bool in = gp < n;
+      Cc[p] = in ? cx[gp] : 0.0;
+      Cc[16 * NPC + p] = in ? cy[gp] : 0.0;
+      Cc[32 * NPC + p] = (in && DIM == 3) ? cz[gp] : 0.0;
+    }
     __syncthreads();
+    // this lane's entries of tile (I, k0 + q2) of A' as the accumulator of
+    // its transpose, A'(16 I + lr, 16 (k0 + q2) + row(q, lq)), from the row's
+    // coordinates (xi, yi, zi) and the panel columns'; identity on the padding
+    auto aprime = [&](int I, int q2, double xi, double yi, double zi) {
+      const int i = 16 * I + lr;
+      v4 v;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int jj = 16 * q2 + MF<Real>::row(q, lq), j = 16 * k0 + jj;
+        if (i < n && j < n)
+          v[q] = gauss_entry<Real>(xi - Cc[jj], yi - Cc[16 * NPC + jj], DIM == 3 ? zi - Cc[32 * NPC + jj] : 0.0,
+                                   inv_two_s2, norm);
+        else
+          v[q] = (i == j) ? Real(1) : Real(0);
+      }
+      return v;
+    };
     // diagonal block, one tile per wavefront: acc = (A_{k0+r,k0+q} - sum_j
     // L_{k0+r,j} L_{k0+q,j}^T)^T
     for (int u = wave; u < npc * (npc + 1) / 2; u += NW) {
       int r, q2;
       tri_decode(u, &r, &q2);
-      v4 acc = *(const v4*)(wtile(k0 + r, k0 + q2) + lane * 4);
+      v4 acc = aprime(k0 + r, q2, Cc[16 * r + lr], Cc[16 * NPC + 16 * r + lr], Cc[32 * NPC + 16 * r + lr]);
       const Real* ra = Rw + (size_t)q2 * k0 * kTE + lane * 4;
       const Real* rb = Rw + (size_t)r * k0 * kTE + lane * 4;
       for (int j = 0; j < k0; ++j) {
@@ -799,17 +810,22 @@ __device__ void factor_block_ll(
     // the diagonal block is factorised.
     v4 acc[G][NPC];
     auto stream = [&](int ib) {
-#pragma unroll
-      for (int g = 0; g < G; ++g) {
-        const int i = ib + g * NW;
-#pragma unroll
-        for (int q2 = 0; q2 < NPC; ++q2)
-          if (i < KT && q2 < npc) acc[g][q2] = *(const v4*)(wtile(i, k0 + q2) + lane * 4);
-      }
       v4 b[G];
 #pragma unroll
       for (int g = 0; g < G; ++g)
         if (k0 > 0 && ib + g * NW < KT) b[g] = *(const v4*)(wtile(ib + g * NW, 0) + lane * 4);
+      // A' of this pass's tiles, made while the first tiles load
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int i = ib + g * NW, ii = 16 * i + lr;
+        if (i < KT) {
+          const bool in = ii < n;
+          const double xi = in ? cx[ii] : 0.0, yi = in ? cy[ii] : 0.0, zi = (in && DIM == 3) ? cz[ii] : 0.0;
+#pragma unroll
+          for (int q2 = 0; q2 < NPC; ++q2)
+            if (q2 < npc) acc[g][q2] = aprime(i, q2, xi, yi, zi);
+        }
+      }
       for (int j = 0; j < k0; ++j) {
         v4 bn[G];
 #pragma unroll
