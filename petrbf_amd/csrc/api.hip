@@ -382,7 +382,7 @@ struct CholCfg {
 const CholCfg kCholCfgs[] = {{4, 1, false}, {8, 1, false}, {4, 2, false}, {8, 2, false}, {4, 4, false},
                              {8, 4, false}, {4, 3, false}, {6, 1, false}, {6, 2, false}, {6, 3, false},
                              {6, 4, false}, {8, 3, false}, {4, 1, true},  {4, 2, true},  {8, 1, true},
-                             {8, 2, true}};
+                             {8, 2, true},  {4, 4, true},  {8, 4, true}};
 
 // Process-wide cache of tuned configurations, keyed by device, precision and
 // the largest block's tile count; shared by every context, so guarded (two

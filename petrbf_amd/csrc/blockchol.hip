@@ -667,9 +667,14 @@ __host__ __device__ inline int ll_row_tiles(int KT, int npc) {
   return m;
 }
 
+// Panels of more than two columns read their L rows straight from the
+// workspace (L2-resident: just written, and read by every wavefront of the
+// workgroup), so that LDS stays within three workgroups per CU.
+__host__ __device__ constexpr bool ll_rows_global(int npc) { return npc > 2; }
+
 template <typename T>
 __host__ __device__ inline size_t ll_region(int KT, int nw, int npc) {
-  size_t rows = (size_t)ll_row_tiles(KT, npc) * kTE * sizeof(T);
+  size_t rows = ll_rows_global(npc) ? 0 : (size_t)ll_row_tiles(KT, npc) * kTE * sizeof(T);
   size_t red = ((size_t)16 * KT * kLS + (size_t)nw * kLT) * sizeof(T);
   size_t region = rows > red ? rows : red;
   return (region + 31) & ~(size_t)31;
@@ -705,7 +710,8 @@ __device__ void factor_block_ll(
     const long long* __restrict__ mptr, Real* __restrict__ M, double inv_two_s2, double norm, int* __restrict__ err,
     int* __restrict__ blk_status) {
   typedef typename MF<Real>::v4 v4;
-  constexpr int G = 4;  // tile rows per wavefront pass
+  constexpr bool PG = ll_rows_global(NPC);  // panel rows from the workspace, not LDS
+  constexpr int G = PG ? 2 : 4;              // tile rows per wavefront pass
   const long long b0 = bptr[c];
   const int n = (int)(bptr[c + 1] - b0);
   if (n == 0) return;
@@ -750,7 +756,7 @@ __device__ void factor_block_ll(
     const int npc = min(NPC, KT - k0);
     // the panel's L rows left of it, row r at Rw + r k0 tiles (rows are
     // contiguous in the workspace), and its columns' coordinates
-    for (int r = 0; r < npc; ++r) {
+    for (int r = 0; r < npc && !PG; ++r) {
       const Real* src = wtile(k0 + r, 0);
       Real* dst = Rw + (size_t)r * k0 * kTE;
       for (int e = tid * 4; e < k0 * kTE; e += NW * 256) *(v4*)(dst + e) = *(const v4*)(src + e);
@@ -786,8 +792,8 @@ __device__ void factor_block_ll(
       int r, q2;
       tri_decode(u, &r, &q2);
       v4 acc = aprime(k0 + r, q2, Cc[16 * r + lr], Cc[16 * NPC + 16 * r + lr], Cc[32 * NPC + 16 * r + lr]);
-      const Real* ra = Rw + (size_t)q2 * k0 * kTE + lane * 4;
-      const Real* rb = Rw + (size_t)r * k0 * kTE + lane * 4;
+      const Real* ra = (PG ? wtile(k0 + q2, 0) : Rw + (size_t)q2 * k0 * kTE) + lane * 4;
+      const Real* rb = (PG ? wtile(k0 + r, 0) : Rw + (size_t)r * k0 * kTE) + lane * 4;
       for (int j = 0; j < k0; ++j) {
         const v4 a = *(const v4*)(ra + j * kTE);
         const v4 b = *(const v4*)(rb + j * kTE);
@@ -834,7 +840,7 @@ __device__ void factor_block_ll(
 #pragma unroll
         for (int q2 = 0; q2 < NPC; ++q2) {
           if (q2 < npc) {
-            const v4 a = *(const v4*)(Rw + ((size_t)q2 * k0 + j) * kTE + lane * 4);
+            const v4 a = *(const v4*)((PG ? wtile(k0 + q2, j) : Rw + ((size_t)q2 * k0 + j) * kTE) + lane * 4);
 #pragma unroll
             for (int g = 0; g < G; ++g)
               if (ib + g * NW < KT) {
@@ -990,7 +996,8 @@ static const void* chol_fn_dim(int nw, int npc, bool ll) {
   if (nw == NWV && npc == NPCV && ll == LLV) return (const void*)k_block_chol<T, D, NWV, NPCV, LLV>;
   PRBF_FN(4, 1, false) PRBF_FN(4, 2, false) PRBF_FN(4, 4, false) PRBF_FN(8, 1, false) PRBF_FN(8, 2, false)
   PRBF_FN(8, 4, false)
-  PRBF_FN(4, 1, true) PRBF_FN(4, 2, true) PRBF_FN(8, 1, true) PRBF_FN(8, 2, true)
+  PRBF_FN(4, 1, true) PRBF_FN(4, 2, true) PRBF_FN(4, 4, true) PRBF_FN(8, 1, true) PRBF_FN(8, 2, true)
+  PRBF_FN(8, 4, true)
   if constexpr (sizeof(T) == 4) {
     PRBF_FN(4, 3, false) PRBF_FN(6, 1, false) PRBF_FN(6, 2, false) PRBF_FN(6, 3, false) PRBF_FN(6, 4, false)
     PRBF_FN(8, 3, false)

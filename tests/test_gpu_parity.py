@@ -247,7 +247,7 @@ def test_mixed_core_rows_close_to_fp64():
 def test_block_factorisation_launch_configs_agree(prec):
     """Every launch configuration the autotuner may pick (right-looking FP64:
     4/8 wavefronts, 1/2/4 tile columns per panel; FP32 also 6 wavefronts and
-    3-column panels; left-looking: 4/8 wavefronts, 1/2 columns) yields the
+    3-column panels; left-looking: 4/8 wavefronts, 1/2/4 columns) yields the
     same preconditioner."""
     _announce(f"chol configs {prec}")
     p = lattice(40)
@@ -257,7 +257,7 @@ def test_block_factorisation_launch_configs_agree(prec):
     fp32 = prec == "rasm32"
     cfgs = [("right", nw, npc) for nw in ((4, 6, 8) if fp32 else (4, 8))
             for npc in ((1, 2, 3, 4) if fp32 else (1, 2, 4))]
-    cfgs += [("left", nw, npc) for nw in (4, 8) for npc in (1, 2)]
+    cfgs += [("left", nw, npc) for nw in (4, 8) for npc in (1, 2, 4)]
     for order, nw, npc in cfgs:
         ctx = Context.from_problem(p)
         ctx.set_tuning(chol_waves=nw, chol_panel=npc, chol_order=order)
