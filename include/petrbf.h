@@ -94,7 +94,7 @@ typedef struct {
   double t_pc_s;     /* summed device time of the preconditioner applies (0 with graph = 1) */
   int n_matvec;
   int n_pc;
-  int n_fallback;    /* RASM_FP32 / LEARNED: blocks redone in FP64 after an FP32 breakdown */
+  int n_fallback;    /* always 0: an FP32 block breakdown fails the setup with PRBF_ERR_FACTOR */
   int n_network;     /* LEARNED: clusters served by the network at the end of the solve */
   int n_net_fallback; /* LEARNED: clusters the solve moved from the network to exact blocks */
 } prbf_solve_info;

@@ -130,8 +130,6 @@ struct prbf_ctx {
   int rasm_pc = 0;
   int n_fallback = 0;
   DBuf<float> M32;
-  DBuf<long long> fb;
-  DBuf<double> M64fb;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // block factorisation: work list, counter and workspace slots
   DBuf<int> fb_order, fb_next, fb_status, fb_failed;
@@ -577,33 +575,12 @@ void rasm_setup_impl(prbf_ctx* c, int pc) {
         std::sort(failed.begin(), failed.end());
       }
     }
-    // FP64 redo of the blocks whose FP32 factorisation broke down; with none,
-    // the apply is given no per-cluster offsets at all
-    c->n_fallback = (int)failed.size();
+    // a block whose FP32 factorisation broke down is reported, not redone in
+    // FP64 (that path is switched off)
     if (!failed.empty()) {
-      c->fb.ensure(nc);
-      std::vector<long long> h_fb(nc, -1), h_m64(nc + 1, 0);
-      long long tot = 0;
-      int nmax = 0, tmax = 0;
-      for (int k : failed) {
-        long long n = c->h_bptr[k + 1] - c->h_bptr[k];
-        long long T = c->h_iend[k] - c->h_ista[k] + 1;
-        h_fb[k] = tot;
-        h_m64[k] = tot;
-        tot += apply_elems((int)n, (int)T);
-        nmax = std::max<int>(nmax, (int)n);
-        tmax = std::max<int>(tmax, (int)T);
-      }
-      PRBF_HIP(hipMemcpyAsync(c->fb, h_fb.data(), sizeof(long long) * nc, hipMemcpyHostToDevice, c->st));
-      c->M64fb.ensure(tot);
-      DBuf<long long> mp;
-      mp.ensure(nc + 1);
-      PRBF_HIP(hipMemcpyAsync(mp, h_m64.data(), sizeof(long long) * (nc + 1), hipMemcpyHostToDevice, c->st));
-      const int saved_nw = c->used_nw, saved_npc = c->used_npc, saved_order = c->used_order;
-      factor_blocks<double>(c, false, failed, nmax, tmax, mp, c->M64fb.p, nullptr, false);
-      c->used_nw = saved_nw;
-      c->used_npc = saved_npc;
-      c->used_order = saved_order;
+      const std::string msg = "the FP32 factorisation of " + std::to_string(failed.size()) +
+                              " RASM blocks broke down (not positive definite in FP32); use PRBF_PC_RASM_FP64";
+      need(false, PRBF_ERR_FACTOR, msg.c_str());
     }
   }
   int herr = 0;
@@ -633,12 +610,10 @@ void rasm_apply_impl(prbf_ctx* c, const double* r, double* z) {
   need(c->rasm_ready, PRBF_ERR_STATE, "rasm_setup first");
   if (c->rasm_pc == PRBF_PC_LEARNED) {
     need(c->learned_ready && c->net, PRBF_ERR_STATE, "prbf_learned_attach first");
-    rasm_apply_fp32(c->st, c->g.ncell, c->bptr, c->bidx, c->ista, c->iend, c->mptr, c->M32,
-                    c->n_fallback ? c->fb.p : nullptr, c->M64fb.p, c->skip, r, z);
+    rasm_apply_fp32(c->st, c->g.ncell, c->bptr, c->bidx, c->ista, c->iend, c->mptr, c->M32, c->skip, r, z);
     network_apply(c, c->mlp_list, c->n_mlp, r, z);
   } else if (c->rasm_pc == PRBF_PC_RASM_FP32)
-    rasm_apply_fp32(c->st, c->g.ncell, c->bptr, c->bidx, c->ista, c->iend, c->mptr, c->M32,
-                    c->n_fallback ? c->fb.p : nullptr, c->M64fb.p, nullptr, r, z);
+    rasm_apply_fp32(c->st, c->g.ncell, c->bptr, c->bidx, c->ista, c->iend, c->mptr, c->M32, nullptr, r, z);
   else
     rasm_apply_fp64(c->st, c->g.ncell, c->bptr, c->bidx, c->ista, c->iend, c->mptr, c->M, r, z, c->max_core);
   PRBF_HIP(hipGetLastError());
@@ -756,7 +731,7 @@ void ensure_graphs(prbf_ctx* c, int pc, bool flex, long long n, int m) {
       (unsigned long long)c->nj, (unsigned long long)c->prm.dim, sig, u(c->V.p), u(c->Z.p), u(c->v1.p),
       u(c->part.p), u(c->scal.p), u(c->kst.p), u(c->pin), u(c->ista.p), u(c->iend.p), u(c->tptr.p), u(c->tidx.p),
       u(c->xi.p), u(c->yi.p), u(c->zi.p), u(c->xj_()), u(c->yj_()), u(c->zj_()), u(c->bptr.p), u(c->bidx.p),
-      u(c->mptr.p), u(c->M.p), u(c->M32.p), u(c->n_fallback ? c->fb.p : nullptr), u(c->M64fb.p), u(c->skip.p), u(c->mlp_list.p),
+      u(c->mptr.p), u(c->M.p), u(c->M32.p), u(c->skip.p), u(c->mlp_list.p),
       (unsigned long long)c->n_mlp, u(c->pidx.p), u(c->cidx.p), u(c->pctr.p), u(c->lX.p), u(c->ls.p), u(c->net)};
   if (c->net) {
     unsigned long long os;
@@ -1309,7 +1284,7 @@ prbf_status prbf_rasm_get_core_rows(prbf_ctx* c, int64_t k, double* M, int32_t* 
       d2h(c, st.data(), c->M.p + c->h_mptr[k], sizeof(double) * st.size());
       sync(c);
       for (int t = 0; t < tt; ++t)
-        for (int b = 0; b < nn; ++b) M[(size_t)t * nn + b] = st[apply_offset<2>(nn, t, b)];
+        for (int b = 0; b < nn; ++b) M[(size_t)t * nn + b] = st[apply_offset<2>(nn, tt, t, b)];
     }
   });
 }

@@ -437,11 +437,11 @@ __device__ void core_rows(long long c, const Real* __restrict__ Wb, const Real* 
     }
     // ---- phase 3: M_c[16 tc + tt][b] = X[pos(b)][tt], rows past T zero, in
     // the apply's storage order (apply_offset), written contiguously
-    const int tcn = min(16, T - 16 * tc);
+    const int tcn = min(16, T - 16 * tc), R = apply_chunk_rows(T, tc);
     constexpr int V = 16 / (int)sizeof(Real);
-    Real* Mk = Mc + (long long)tc * apply_chunk_elems(n);
-    for (int e = tid; e < 16 * n; e += NW * 64) {
-      const int g = e / 1024, rr = e % 1024;
+    Real* Mk = Mc + apply_chunk_base(n, tc);
+    for (int e = tid; e < R * n; e += NW * 64) {
+      const int g = e / (R * 64), rr = e % (R * 64);
       const int ng = min(64, n - 64 * g);
       const int q = rr / (ng * V), rem = rr % (ng * V);
       const int b = 64 * g + rem / V, tt = q * V + rem % V;

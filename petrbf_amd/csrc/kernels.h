@@ -33,22 +33,31 @@ void matvec(hipStream_t st, int dim, int nrhs, long long ncell, const int* tsta,
 
 // blockchol.hip / rasm.hip ----------------------------------------------------
 // Storage of the core rows M_c (T x n) for the apply.  The rows come in
-// chunks of 16 (rows T..16*ceil(T/16)-1 are zero); inside a chunk the buffer
-// slots form groups of 64 (the last group holds ng = n - 64 g slots), and a
-// group stores its 16 x ng values as 16/V vector slabs [q][slot][V], V values
-// (16 bytes) per slot and slab.  Lane l of a wavefront thus reads 16 bytes per
-// slab at l*16 bytes and every slab load is one contiguous 1 KB access.
+// chunks of 16, the last one cut to R rows, T rounded up to a multiple of 4
+// (its rows past T are zero); inside a chunk of R rows the buffer slots form
+// groups of 64 (the last group holds ng = n - 64 g slots), and a group stores
+// its R x ng values as R/V vector slabs [q][slot][V], V values (16 bytes) per
+// slot and slab.  Lane l of a wavefront thus reads 16 bytes per slab at l*16
+// bytes and every slab load is one contiguous access of up to 1 KB.
 constexpr int kRowChunk = 16;
-__host__ __device__ inline long long apply_chunk_elems(int n) { return (long long)kRowChunk * n; }
+constexpr int kRowQuant = 4;
+// rows stored for chunk kc of T core rows
+__host__ __device__ inline int apply_chunk_rows(int T, int kc) {
+  const int r = T - kRowChunk * kc;
+  return r >= kRowChunk ? kRowChunk : (r + kRowQuant - 1) / kRowQuant * kRowQuant;
+}
+__host__ __device__ inline long long apply_chunk_base(int n, int kc) { return (long long)kRowChunk * kc * n; }
 __host__ __device__ inline long long apply_elems(int n, int T) {
-  return n ? (long long)((T + kRowChunk - 1) / kRowChunk) * apply_chunk_elems(n) : 0;
+  if (n == 0 || T <= 0) return 0;
+  const int kc = (T - 1) / kRowChunk;
+  return apply_chunk_base(n, kc) + (long long)apply_chunk_rows(T, kc) * n;
 }
 // offset of the value of row t (all chunks) and slot b; V = 16 / sizeof(element)
 template <int V>
-__host__ __device__ inline long long apply_offset(int n, int t, int b) {
-  const int kc = t / kRowChunk, tt = t % kRowChunk;
+__host__ __device__ inline long long apply_offset(int n, int T, int t, int b) {
+  const int kc = t / kRowChunk, tt = t % kRowChunk, R = apply_chunk_rows(T, kc);
   const int g = b / 64, l = b % 64, ng = n - 64 * g < 64 ? n - 64 * g : 64;
-  return kc * apply_chunk_elems(n) + 1024LL * g + (long long)((tt / V) * ng + l) * V + tt % V;
+  return apply_chunk_base(n, kc) + (long long)R * 64 * g + (long long)((tt / V) * ng + l) * V + tt % V;
 }
 
 // Workspace (elements) and LDS bytes of one block factorisation (ll: the
@@ -78,8 +87,7 @@ void rasm_apply_fp64(hipStream_t st, long long ncell, const long long* bptr, con
                      const double* r, double* z, int tmax);
 void rasm_apply_fp32(hipStream_t st, long long ncell, const long long* bptr, const int* bidx,
                      const int* ista, const int* iend, const long long* mptr, const float* M,
-                     const long long* fb, const double* M64, const unsigned char* skip, const double* r,
-                     double* z);
+                     const unsigned char* skip, const double* r, double* z);
 
 // krylov.hip ------------------------------------------------------------------
 constexpr int kRedBlocks = 1024;  // fixed partition => run-to-run identical sums

@@ -20,13 +20,13 @@ namespace prbf {
 // Apply: one wavefront per cluster.  Lane l owns the buffer slots l, l+64,
 // ... (one per slot group), gathers r at those slots once, and accumulates 16
 // row partials against the core rows, which it reads as 16-byte vectors from
-// contiguous 1 KB slabs (the storage order of apply_offset).  The 16 x 64
+// contiguous slabs of up to 1 KB (the storage order of apply_offset; only the
+// rows stored, T rounded up to 4, are read).  The 16 x 64
 // partials are combined by a butterfly over the wavefront (reduce-scatter by
 // halves, then two full exchanges): a fixed order, so the result is identical
 // run to run.  MT = double (plain) or float (mixed; products and sums still
-// in FP64).  In mixed mode a cluster whose FP32 block broke down carries FP64
-// rows in M64 at offset fb[c] (fb[c] < 0 otherwise); clusters with skip[c]
-// set are left to another part of the preconditioner.  Lanes past the end of
+// in FP64).  Clusters with skip[c] set are left to another part of the
+// preconditioner.  Lanes past the end of
 // the last group read a valid slot of the same group with r taken as 0, so
 // the loads need no branches.
 constexpr int kApplyWaves = 4;  // clusters per workgroup, one per wavefront
@@ -66,26 +66,26 @@ __device__ __forceinline__ void gather_slots(int n, int lane, const int* __restr
   }
 }
 
-// acc[t] += sum over this lane's slots of M[t][b] r[b], 16 rows of one chunk;
-// KG slot groups per pass (all loads of a pass are issued before the FMAs),
-// residuals from the LDS slice.  The core rows are read once per apply:
-// non-temporal loads keep them from displacing the residuals that
-// neighbouring clusters gather.
+// acc[t] += sum over this lane's slots of M[t][b] r[b], the R stored rows of
+// one chunk (rows R..15 of acc stay zero); KG slot groups per pass (all loads
+// of a pass are issued before the FMAs), residuals from the LDS slice.  The
+// core rows are read once per apply: non-temporal loads keep them from
+// displacing the residuals that neighbouring clusters gather.
 template <typename ET, int KG>
-__device__ __forceinline__ void chunk_times(const ET* __restrict__ Mk, int n, int lane, const double* __restrict__ rs,
-                                            double (&acc)[kRowChunk]) {
+__device__ __forceinline__ void chunk_times(const ET* __restrict__ Mk, int n, int R, int lane,
+                                            const double* __restrict__ rs, double (&acc)[kRowChunk]) {
   constexpr int V = 16 / (int)sizeof(ET), QN = kRowChunk / V;
   using VT = typename Vec16<ET>::type;
-  const int ngrp = (n + 63) / 64;
+  const int ngrp = (n + 63) / 64, qn = R / V;
   for (int g0 = 0; g0 < ngrp; g0 += KG) {
     VT m[KG][QN];
 #pragma unroll
     for (int k = 0; k < KG; ++k) {
       const int g = g0 + k < ngrp ? g0 + k : ngrp - 1;
       const int ng = min(64, n - 64 * g);
-      const VT* p = reinterpret_cast<const VT*>(Mk + 1024LL * g) + min(lane, ng - 1);
+      const VT* p = reinterpret_cast<const VT*>(Mk + (long long)R * 64 * g) + min(lane, ng - 1);
 #pragma unroll
-      for (int q = 0; q < QN; ++q) m[k][q] = __builtin_nontemporal_load(p + q * ng);
+      for (int q = 0; q < QN; ++q) m[k][q] = q < qn ? __builtin_nontemporal_load(p + q * ng) : VT{};
     }
 #pragma unroll
     for (int k = 0; k < KG; ++k) {
@@ -102,11 +102,12 @@ __device__ __forceinline__ void chunk_times(const ET* __restrict__ Mk, int n, in
 // The same for blocks of more than 64 kMaxGroups slots: indices and residuals
 // gathered pass by pass, KG slot groups per pass
 template <typename ET, int KG>
-__device__ __forceinline__ void chunk_times_gather(const ET* __restrict__ Mk, int n, int lane, const int* __restrict__ bi,
-                                   const double* __restrict__ r, double (&acc)[kRowChunk]) {
+__device__ __forceinline__ void chunk_times_gather(const ET* __restrict__ Mk, int n, int R, int lane,
+                                                   const int* __restrict__ bi, const double* __restrict__ r,
+                                                   double (&acc)[kRowChunk]) {
   constexpr int V = 16 / (int)sizeof(ET), QN = kRowChunk / V;
   using VT = typename Vec16<ET>::type;
-  const int ngrp = (n + 63) / 64;
+  const int ngrp = (n + 63) / 64, qn = R / V;
   for (int g0 = 0; g0 < ngrp; g0 += KG) {
     int idx[KG];
     bool ok[KG];
@@ -118,9 +119,9 @@ __device__ __forceinline__ void chunk_times_gather(const ET* __restrict__ Mk, in
       const int l = min(lane, ng - 1);
       ok[k] = (g0 + k < ngrp) && lane < ng;
       idx[k] = bi[64 * g + l];
-      const VT* p = reinterpret_cast<const VT*>(Mk + 1024LL * g) + l;
+      const VT* p = reinterpret_cast<const VT*>(Mk + (long long)R * 64 * g) + l;
 #pragma unroll
-      for (int q = 0; q < QN; ++q) m[k][q] = p[q * ng];
+      for (int q = 0; q < QN; ++q) m[k][q] = q < qn ? p[q * ng] : VT{};
     }
     double rb[KG];
 #pragma unroll
@@ -186,22 +187,28 @@ __device__ __forceinline__ void apply_cluster(const ET* __restrict__ Mc, int n, 
     double acc[kRowChunk];
 #pragma unroll
     for (int t = 0; t < kRowChunk; ++t) acc[t] = 0.0;
-    const ET* Mk = Mc + (long long)(t0 / kRowChunk) * apply_chunk_elems(n);
+    const int kc = t0 / kRowChunk, R = apply_chunk_rows(T, kc);
+    const ET* Mk = Mc + apply_chunk_base(n, kc);
     if (small)
-      chunk_times<ET, KG>(Mk, n, lane, rs, acc);
+      chunk_times<ET, KG>(Mk, n, R, lane, rs, acc);
     else
-      chunk_times_gather<ET, KG>(Mk, n, lane, bi, r, acc);
+      chunk_times_gather<ET, KG>(Mk, n, R, lane, bi, r, acc);
     const double s = wave_rows_sum(acc, lane);
     if ((lane & 3) == 0 && t0 + trow < T) zc[t0 + trow] = s;
   }
 }
 
+// One launch over all clusters; clusters with skip[c] set are left out.  One
+// slot group in flight per pass keeps the kernels at 96 (FP32, five
+// wavefronts per SIMD) or 128 VGPRs (FP64, four): the apply streams each core
+// row once, and more resident clusters hide the gathers' latency better than
+// more loads in flight per cluster.
 template <typename MT>
-__global__ __launch_bounds__(64 * kApplyWaves) void k_rasm_apply(
+__global__ __launch_bounds__(64 * kApplyWaves, sizeof(MT) == 4 ? 5 : 4) void k_rasm_apply(
     long long ncell, const long long* __restrict__ bptr, const int* __restrict__ bidx,
     const int* __restrict__ ista, const int* __restrict__ iend, const long long* __restrict__ mptr,
-    const MT* __restrict__ M, const long long* __restrict__ fb, const double* __restrict__ M64,
-    const unsigned char* __restrict__ skip, const double* __restrict__ r, double* __restrict__ z) {
+    const MT* __restrict__ M, const unsigned char* __restrict__ skip, const double* __restrict__ r,
+    double* __restrict__ z) {
   __shared__ double rs[kApplyWaves][kMaxGroups * 64];
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const long long c = (long long)xcd_block(blockIdx.x, gridDim.x) * kApplyWaves + wave;
@@ -212,11 +219,7 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_rasm_apply(
   if (n == 0) return;
   const int s = ista[c];
   const int T = iend[c] - s + 1;
-  const long long f = fb ? fb[c] : -1;
-  if (f >= 0)
-    apply_cluster<double, 2>(M64 + f, n, T, lane, bidx + b0, r, rs[wave], z + s);
-  else
-    apply_cluster<MT, sizeof(MT) == 4 ? 4 : 2>(M + mptr[c], n, T, lane, bidx + b0, r, rs[wave], z + s);
+  apply_cluster<MT, 1>(M + mptr[c], n, T, lane, bidx + b0, r, rs[wave], z + s);
 }
 
 static inline dim3 apply_grid(long long ncell) { return dim3((unsigned)((ncell + kApplyWaves - 1) / kApplyWaves)); }
@@ -226,15 +229,14 @@ void rasm_apply_fp64(hipStream_t st, long long ncell, const long long* bptr, con
                      const double* r, double* z, int tmax) {
   (void)tmax;
   hipLaunchKernelGGL((k_rasm_apply<double>), apply_grid(ncell), dim3(64 * kApplyWaves), 0, st, ncell, bptr, bidx,
-                     ista, iend, mptr, M, nullptr, nullptr, nullptr, r, z);
+                     ista, iend, mptr, M, nullptr, r, z);
 }
 
 void rasm_apply_fp32(hipStream_t st, long long ncell, const long long* bptr, const int* bidx,
                      const int* ista, const int* iend, const long long* mptr, const float* M,
-                     const long long* fb, const double* M64, const unsigned char* skip, const double* r,
-                     double* z) {
+                     const unsigned char* skip, const double* r, double* z) {
   hipLaunchKernelGGL((k_rasm_apply<float>), apply_grid(ncell), dim3(64 * kApplyWaves), 0, st, ncell, bptr, bidx,
-                     ista, iend, mptr, M, fb, M64, skip, r, z);
+                     ista, iend, mptr, M, skip, r, z);
 }
 
 }  // namespace prbf
