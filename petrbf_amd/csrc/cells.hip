@@ -73,34 +73,82 @@ __global__ void k_cell_assign(long long n, const double* __restrict__ x, const d
   atomicAdd(&count[c], 1);
 }
 
-// Exclusive scan of int counts into int64 offsets (single workgroup, chunked;
-// deterministic).  out has n+1 entries, out[n] = total.
-__global__ void k_scan_excl(long long n, const int* __restrict__ in, long long* __restrict__ out) {
-  __shared__ long long sh[1024];
-  __shared__ long long carry;
-  int t = threadIdx.x;
-  if (t == 0) carry = 0;
+// Exclusive scan of int counts into int64 offsets, out[n] = total; integer
+// sums, so the result does not depend on the order.  Three passes over
+// 1024-element blocks, with no scratch beyond out: (1) each block scans
+// itself by wavefront shuffles and leaves its total in the slot of its first
+// element (whose block-local exclusive value is 0); (2) one workgroup turns
+// those totals into block offsets in place and writes out[n]; (3) each block
+// adds its offset to its other elements.
+constexpr int kScanB = 1024;
+
+__device__ inline long long wave_incl_scan(long long v, int lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const long long u = __shfl_up(v, o);
+    if (lane >= o) v += u;
+  }
+  return v;
+}
+
+// inclusive scan of v over the workgroup's 1024 threads (wsum: 16 slots)
+__device__ inline long long block_incl_scan(long long v, long long* wsum) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  v = wave_incl_scan(v, lane);
+  if (lane == 63) wsum[w] = v;
   __syncthreads();
-  for (long long base = 0; base < n; base += 1024) {
-    long long v = (base + t < n) ? in[base + t] : 0;
-    sh[t] = v;
+  if (w == 0) {
+    long long s = lane < kScanB / 64 ? wsum[lane] : 0;
+    s = wave_incl_scan(s, lane);
+    if (lane < kScanB / 64) wsum[lane] = s;
+  }
+  __syncthreads();
+  if (w > 0) v += wsum[w - 1];
+  return v;
+}
+
+__global__ __launch_bounds__(kScanB) void k_scan_blocks(long long n, const int* __restrict__ in,
+                                                        long long* __restrict__ out) {
+  __shared__ long long wsum[kScanB / 64];
+  const long long i = (long long)blockIdx.x * kScanB + threadIdx.x;
+  const long long v = i < n ? in[i] : 0;
+  const long long incl = block_incl_scan(v, wsum);
+  if (threadIdx.x == kScanB - 1) out[(long long)blockIdx.x * kScanB] = incl;  // block total
+  else if (i + 1 < n) out[i + 1] = incl;                                       // exclusive of i + 1
+}
+
+__global__ __launch_bounds__(kScanB) void k_scan_offsets(long long n, long long nb, long long* __restrict__ out) {
+  __shared__ long long wsum[kScanB / 64];
+  __shared__ long long carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (long long b0 = 0; b0 < nb; b0 += kScanB) {
+    const long long b = b0 + threadIdx.x;
+    const long long v = b < nb ? out[b * kScanB] : 0;
+    const long long incl = block_incl_scan(v, wsum);
+    const long long base = carry;
+    if (b < nb) out[b * kScanB] = base + incl - v;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-      long long a = (t >= off) ? sh[t - off] : 0;
-      __syncthreads();
-      sh[t] += a;
-      __syncthreads();
-    }
-    if (base + t < n) out[base + t] = carry + sh[t] - v;
-    __syncthreads();
-    if (t == 1023) carry += sh[1023];
+    if (threadIdx.x == kScanB - 1) carry = base + incl;
     __syncthreads();
   }
-  if (t == 0) out[n] = carry;
+  if (threadIdx.x == 0) out[n] = carry;
+}
+
+__global__ __launch_bounds__(kScanB) void k_scan_add(long long n, long long* __restrict__ out) {
+  const long long b0 = (long long)blockIdx.x * kScanB, i = b0 + threadIdx.x;
+  if (threadIdx.x > 0 && i < n) out[i] += out[b0];
 }
 
 void scan_excl(hipStream_t s, long long n, const int* in, long long* out) {
-  hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, s, n, in, out);
+  if (n <= 0) {
+    PRBF_HIP(hipMemsetAsync(out, 0, sizeof(long long), s));
+    return;
+  }
+  const long long nb = (n + kScanB - 1) / kScanB;
+  hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(kScanB), 0, s, n, in, out);
+  hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(kScanB), 0, s, n, nb, out);
+  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(kScanB), 0, s, n, out);
 }
 
 // Unordered placement into cell slots, then per-cell ordering by original
