@@ -674,8 +674,9 @@ void learned_fallback(prbf_ctx* c, const std::vector<int>& newc) {
   for (int k : newc) failed |= h[k] != 0;
   c->n_net_fallback += (int)newc.size();
   c->h_mlp_list = list;
-  if (failed) rasm_setup_impl(c, PRBF_PC_LEARNED);  // an FP32 breakdown: the full setup adds FP64 blocks
   sync(c);
+  need(!failed, PRBF_ERR_FACTOR,
+       "the FP32 factorisation of a RASM block broke down (not positive definite in FP32); use PRBF_PC_RASM_FP64");
 }
 
 // Preconditioner application without timing (usable under graph capture).

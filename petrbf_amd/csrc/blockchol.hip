@@ -26,10 +26,11 @@
 //            L_II^{-1} tiles kept from phase 1, 16 right-hand sides at a
 //            time held in LDS;
 //   phase 3  write each 16-column slice of X to M_c in buffer order.
-// The left-looking variant (factor_block_ll) replaces phase 1: each panel is
-// brought up to date from the finished L rows left of it (those rows of the
-// panel in LDS, the rest streamed once), so L tiles are written once instead
-// of being re-read and re-written by every trailing update.
+// The left-looking variant (factor_block_ll) replaces phases 0 and 1: each
+// panel is brought up to date from the finished L rows left of it (those rows
+// of the panel in LDS, the rest streamed once), so L tiles are written once
+// instead of being re-read and re-written by every trailing update, and the
+// entries of A' are made where they enter instead of being stored.
 // T = double is the plain path (exact FP64 block solves); T = float runs the
 // same kernel in FP32 on the FP32 MFMA (mixed precision), with a per-block
 // status so a block whose FP32 factorisation breaks down is redone in FP64.
