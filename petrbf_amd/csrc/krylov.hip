@@ -13,17 +13,31 @@ namespace prbf {
 constexpr int kRT = 256;     // threads per reduction block
 constexpr int kMaxK = 64;    // max basis vectors per fused pass
 
-__device__ inline double block_sum(double v, double* sh) {
-  int t = threadIdx.x;
-  sh[t] = v;
-  __syncthreads();
-  for (int s = kRT / 2; s > 0; s >>= 1) {
-    if (t < s) sh[t] += sh[t + s];
-    __syncthreads();
+// Sums of v[0..NV) over the workgroup (kRT threads): a butterfly over each
+// wavefront, then the wavefronts' totals through LDS in wavefront order; the
+// results are valid in thread 0.  sh holds NV * kRT / 64 values.
+template <int NV>
+__device__ inline void block_sums(double (&v)[NV], double* sh) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) v[j] += __shfl_xor(v[j], o);
   }
-  double r = sh[0];
+  if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) sh[j * (kRT / 64) + wave] = v[j];
+  }
   __syncthreads();
-  return r;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      double s = sh[j * (kRT / 64)];
+      for (int w = 1; w < kRT / 64; ++w) s += sh[j * (kRT / 64) + w];
+      v[j] = s;
+    }
+  }
+  __syncthreads();
 }
 
 __device__ inline void slice(long long n, long long* a, long long* b) {
@@ -36,7 +50,7 @@ __device__ inline void slice(long long n, long long* a, long long* b) {
 __global__ __launch_bounds__(kRT) void k_mdot(long long n, const double* __restrict__ w,
                                               const double* __restrict__ V, long long ldv, int k,
                                               double* __restrict__ part) {
-  __shared__ double sh[kRT];
+  __shared__ double sh[8 * (kRT / 64)];
   long long a, b;
   slice(n, &a, &b);
   for (int j0 = 0; j0 < k; j0 += 8) {
@@ -48,10 +62,9 @@ __global__ __launch_bounds__(kRT) void k_mdot(long long n, const double* __restr
       for (int j = 0; j < 8; ++j)
         if (j < kk) acc[j] = fma(wi, V[(long long)(j0 + j) * ldv + i], acc[j]);
     }
-    for (int j = 0; j < kk; ++j) {
-      double s = block_sum(acc[j], sh);
-      if (threadIdx.x == 0) part[(long long)blockIdx.x * kMaxK + j0 + j] = s;
-    }
+    block_sums<8>(acc, sh);
+    if (threadIdx.x == 0)
+      for (int j = 0; j < kk; ++j) part[(long long)blockIdx.x * kMaxK + j0 + j] = acc[j];
   }
 }
 
@@ -74,7 +87,7 @@ __global__ __launch_bounds__(kRT) void k_maxpy_norm2(long long n, double* __rest
                                                      const double* __restrict__ V, long long ldv, int k,
                                                      const double* __restrict__ coef, double alpha,
                                                      double* __restrict__ part) {
-  __shared__ double sh[kRT];
+  __shared__ double sh[kRT / 64];
   __shared__ double cf[kMaxK];
   if (threadIdx.x < k) cf[threadIdx.x] = alpha * coef[threadIdx.x];
   __syncthreads();
@@ -87,8 +100,9 @@ __global__ __launch_bounds__(kRT) void k_maxpy_norm2(long long n, double* __rest
     w[i] = s;
     acc = fma(s, s, acc);
   }
-  double s = block_sum(acc, sh);
-  if (threadIdx.x == 0) part[(long long)blockIdx.x * kMaxK] = s;
+  double v[1] = {acc};
+  block_sums<1>(v, sh);
+  if (threadIdx.x == 0) part[(long long)blockIdx.x * kMaxK] = v[0];
 }
 
 void maxpy_norm2(hipStream_t st, long long n, double* w, const double* V, long long ldv, int k,
@@ -99,13 +113,14 @@ void maxpy_norm2(hipStream_t st, long long n, double* w, const double* V, long l
 
 __global__ __launch_bounds__(kRT) void k_norm2sq(long long n, const double* __restrict__ v,
                                                  double* __restrict__ part) {
-  __shared__ double sh[kRT];
+  __shared__ double sh[kRT / 64];
   long long a, b;
   slice(n, &a, &b);
   double acc = 0.0;
   for (long long i = a + threadIdx.x; i < b; i += kRT) acc = fma(v[i], v[i], acc);
-  double s = block_sum(acc, sh);
-  if (threadIdx.x == 0) part[(long long)blockIdx.x * kMaxK] = s;
+  double r[1] = {acc};
+  block_sums<1>(r, sh);
+  if (threadIdx.x == 0) part[(long long)blockIdx.x * kMaxK] = r[0];
 }
 
 void norm2sq(hipStream_t st, long long n, const double* v, double* part, double* out) {
