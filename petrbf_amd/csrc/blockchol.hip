@@ -33,7 +33,7 @@
 // entries of A' are made where they enter instead of being stored.
 // T = double is the plain path (exact FP64 block solves); T = float runs the
 // same kernel in FP32 on the FP32 MFMA (mixed precision), with a per-block
-// status so a block whose FP32 factorisation breaks down is redone in FP64.
+// status so that a block whose FP32 factorisation breaks down is reported.
 #include <mutex>
 #include <vector>
 
