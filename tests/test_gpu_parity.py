@@ -142,6 +142,42 @@ def test_rasm_apply_matches_oracle_lu(order):
     ctx.close()
 
 
+def test_large_blocks_both_orders():
+    """Blocks of 254 to 789 unknowns (a 12-spacing buffer; 52 of 100 above the
+    apply's 512-slot staging, up to 50 tile rows): the FP64 preconditioner of
+    both factorisation orders agrees with the oracle's LU, and FGMRES on the
+    FP32 left-looking one reaches rtol 1e-10."""
+    _announce("large blocks")
+    p = lattice(40, buffer_cells=12)
+    o = Oracle.from_problem(p)
+    o.rasm_setup()
+    rng = np.random.default_rng(13)
+    r = rng.standard_normal(p.n)
+    want = o.rasm_apply(r)
+    for order, nw, npc in (("right", 4, 1), ("left", 4, 1), ("left", 4, 4)):
+        ctx = Context.from_problem(p)
+        ctx.set_tuning(chol_waves=nw, chol_panel=npc, chol_order=order)
+        ctx.set_points(p.x, p.y)
+        ctx.rasm_setup("rasm")
+        info = ctx.info()
+        assert (info["chol_order"], info["chol_waves"], info["chol_panel"]) == (
+            1 if order == "right" else 2, nw, npc)
+        got = ctx.rasm_apply(r)
+        assert np.linalg.norm(got - want) <= 1e-8 * np.linalg.norm(want), (order, nw, npc)
+        ctx.close()
+    ctx = Context.from_problem(p)
+    ctx.set_tuning(chol_waves=4, chol_panel=2, chol_order="left")
+    ctx.set_points(p.x, p.y)
+    g, inf = ctx.solve(p.w[0], ksp="fgmres", pc="rasm32", rtol=1e-10, restart=30, max_it=1000)
+    assert ctx.info()["chol_order"] == 2
+    assert inf.reason == 2
+    pi, _ = o.perm()
+    w = np.empty(p.n)
+    w[pi] = o.matvec(g[pi])
+    assert np.linalg.norm(w - p.w[0]) <= 1e-9 * np.linalg.norm(p.w[0])
+    ctx.close()
+
+
 @pytest.mark.parametrize("n_side", [100, 160])
 def test_plain_solve_parity(n_side):
     """The solve workload (100x100 is the N = 10^4 CPU configuration)."""
