@@ -74,6 +74,12 @@ def cpu_baseline(p, its, pairs, stride=25, n_mv=3):
     ncell = o.ncell
     frac = (ncell // stride + 1) / ncell
     est = t_lists + t_fac / frac + its * (t_mv + t_ap / frac)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {
         "value": est, "unit": "s", "cores": num_threads(), "kind": "port",
         "sample": (f"N=4M lattice: lists+sort full ({t_lists:.2f}s), {n_mv} full matvecs ({t_mv:.2f}s each), "
@@ -81,6 +87,7 @@ def cpu_baseline(p, its, pairs, stride=25, n_mv=3):
                    f"scaled to all {ncell} clusters and {its} GMRES iterations"),
         "t_lists_s": t_lists, "t_matvec_s": t_mv,
         "pair_evals_per_s": pairs / t_mv,
+        "nproc": os.cpu_count(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "cpu_model": model,
     }
 
 
