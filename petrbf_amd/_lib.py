@@ -16,6 +16,9 @@ PRBF_PC_NONE, PRBF_PC_RASM_FP64, PRBF_PC_RASM_FP32, PRBF_PC_LEARNED = 0, 1, 2, 3
 PRBF_TUNE_CHOL_WAVES, PRBF_TUNE_CHOL_PANEL, PRBF_TUNE_CHOL_ORDER = 1, 2, 3
 PRBF_CHOL_RIGHT, PRBF_CHOL_LEFT = 1, 2
 
+(PRBF_OK, PRBF_ERR_ARG, PRBF_ERR_HIP, PRBF_ERR_NOMEM, PRBF_ERR_STATE, PRBF_ERR_FACTOR, PRBF_ERR_OUTSIDE,
+ PRBF_ERR_LIMIT) = range(8)
+
 STATUS = {0: "PRBF_OK", 1: "PRBF_ERR_ARG", 2: "PRBF_ERR_HIP", 3: "PRBF_ERR_NOMEM",
           4: "PRBF_ERR_STATE", 5: "PRBF_ERR_FACTOR", 6: "PRBF_ERR_OUTSIDE", 7: "PRBF_ERR_LIMIT"}
 

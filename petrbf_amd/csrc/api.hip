@@ -9,6 +9,8 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -186,6 +188,10 @@ prbf_status guarded(prbf_ctx* c, F&& f) {
     return fail(c, e.code, e.what);
   } catch (const HipError& e) {
     return fail(c, PRBF_ERR_HIP, e.what);
+  } catch (const std::bad_alloc&) {
+    return fail(c, PRBF_ERR_NOMEM, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(c, PRBF_ERR_ARG, e.what());
   }
 }
 
@@ -361,6 +367,7 @@ void set_points_impl(prbf_ctx* c, long long ni, const double* xi, const double* 
 void matvec_impl(prbf_ctx* c, int nrhs, const double* g, double* w) {
   need(c->has_points, PRBF_ERR_STATE, "set_points first");
   need(nrhs >= 1 && nrhs <= 3, PRBF_ERR_ARG, "nrhs must be 1..3");
+  need(g && w, PRBF_ERR_ARG, "null vector");
   matvec(c->st, c->prm.dim, nrhs, c->g.ncell, c->ista, c->iend, c->tptr, c->tidx, c->xi, c->yi, c->zi,
          c->xj_(), c->yj_(), c->zj_(), g, c->nj, w, c->ni, c->prm.sigma);
   PRBF_HIP(hipGetLastError());
@@ -608,6 +615,7 @@ void network_apply(prbf_ctx* c, const int* list, long long n, const double* r, d
 
 void rasm_apply_impl(prbf_ctx* c, const double* r, double* z) {
   need(c->rasm_ready, PRBF_ERR_STATE, "rasm_setup first");
+  need(r && z, PRBF_ERR_ARG, "null vector");
   if (c->rasm_pc == PRBF_PC_LEARNED) {
     need(c->learned_ready && c->net, PRBF_ERR_STATE, "prbf_learned_attach first");
     rasm_apply_fp32(c->st, c->g.ncell, c->bptr, c->bidx, c->ista, c->iend, c->mptr, c->M32, c->skip, r, z);
@@ -979,6 +987,13 @@ void solve_dev_impl(prbf_ctx* c, const prbf_solve_opts* opts, const double* w, d
   prbf_solve_opts_default(&o);
   if (opts) o = *opts;
   double t_setup = 0.0;
+  need(w && gamma, PRBF_ERR_ARG, "null vector");
+  need(o.ksp == PRBF_KSP_GMRES || o.ksp == PRBF_KSP_FGMRES, PRBF_ERR_ARG, "unknown Krylov method");
+  need(o.pc == PRBF_PC_NONE || o.pc == PRBF_PC_RASM_FP64 || o.pc == PRBF_PC_RASM_FP32 || o.pc == PRBF_PC_LEARNED,
+       PRBF_ERR_ARG, "unknown preconditioner");
+  need(o.max_it >= 0 && o.max_it <= (1 << 24), PRBF_ERR_ARG, "max_it must be in 0..2^24");
+  need(o.rtol >= 0 && o.abstol >= 0 && o.dtol > 0, PRBF_ERR_ARG, "tolerances must be non-negative (dtol positive)");
+  need(hist_cap >= 0, PRBF_ERR_ARG, "hist_cap must be non-negative");
   need(o.pc != PRBF_PC_LEARNED || c->learned_ready, PRBF_ERR_STATE, "prbf_learned_attach first");
   if (o.pc != PRBF_PC_NONE && (!c->rasm_ready || c->rasm_pc != o.pc)) {
     double t0 = now_s();
@@ -1086,9 +1101,11 @@ prbf_status prbf_set_points(prbf_ctx* c, int64_t ni, const double* xi, const dou
   return guarded(c, [&] {
     need(ni > 0, PRBF_ERR_ARG, "ni must be positive");
     const int dim = c->prm.dim;
+    need(xi && yi && (dim == 2 || zi), PRBF_ERR_ARG, "missing target coordinates");
     bool same = (xj == nullptr);
     long long nsrc = same ? 0 : nj;
     need(same || nj > 0, PRBF_ERR_ARG, "nj must be positive");
+    need(same || (yj && (dim == 2 || zj)), PRBF_ERR_ARG, "missing source coordinates");
     DBuf<double> buf;
     buf.ensure((size_t)3 * (ni + nsrc));
     double* p = buf;
@@ -1097,12 +1114,12 @@ prbf_status prbf_set_points(prbf_ctx* c, int64_t ni, const double* xi, const dou
     };
     up(p, xi, ni);
     up(p + ni, yi, ni);
-    if (dim == 3) { need(zi != nullptr, PRBF_ERR_ARG, "missing z"); up(p + 2 * ni, zi, ni); }
+    if (dim == 3) up(p + 2 * ni, zi, ni);
     double* q = p + 3 * ni;
     if (!same) {
       up(q, xj, nsrc);
       up(q + nsrc, yj, nsrc);
-      if (dim == 3) { need(zj != nullptr, PRBF_ERR_ARG, "missing z"); up(q + 2 * nsrc, zj, nsrc); }
+      if (dim == 3) up(q + 2 * nsrc, zj, nsrc);
     }
     set_points_impl(c, ni, p, p + ni, dim == 3 ? p + 2 * ni : nullptr, nj, same ? nullptr : q,
                     same ? nullptr : q + nsrc, (!same && dim == 3) ? q + 2 * nsrc : nullptr);
@@ -1188,6 +1205,7 @@ prbf_status prbf_matvec(prbf_ctx* c, int nrhs, const double* g, double* w) {
   return guarded(c, [&] {
     need(c->has_points, PRBF_ERR_STATE, "set_points first");
     need(nrhs >= 1 && nrhs <= 3, PRBF_ERR_ARG, "nrhs must be 1..3");
+    need(g && w, PRBF_ERR_ARG, "null vector");
     DBuf<double> dg, dw;
     dg.ensure((size_t)nrhs * c->nj);
     dw.ensure((size_t)nrhs * c->ni);
@@ -1201,6 +1219,7 @@ prbf_status prbf_matvec(prbf_ctx* c, int nrhs, const double* g, double* w) {
 static void evaluate_dev_impl(prbf_ctx* c, int nrhs, const double* gj, double* wi) {
   need(c->has_points, PRBF_ERR_STATE, "set_points first");
   need(nrhs >= 1 && nrhs <= 3, PRBF_ERR_ARG, "nrhs must be 1..3");
+  need(gj && wi, PRBF_ERR_ARG, "null vector");
   DBuf<double> gs, ws;
   gs.ensure((size_t)nrhs * c->nj);
   ws.ensure((size_t)nrhs * c->ni);
@@ -1218,6 +1237,7 @@ prbf_status prbf_evaluate(prbf_ctx* c, int nrhs, const double* gj, double* wi) {
   return guarded(c, [&] {
     need(c->has_points, PRBF_ERR_STATE, "set_points first");
     need(nrhs >= 1 && nrhs <= 3, PRBF_ERR_ARG, "nrhs must be 1..3");
+    need(gj && wi, PRBF_ERR_ARG, "null vector");
     DBuf<double> dg, dw;
     dg.ensure((size_t)nrhs * c->nj);
     dw.ensure((size_t)nrhs * c->ni);
@@ -1262,6 +1282,7 @@ prbf_status prbf_rasm_apply_dev(prbf_ctx* c, const double* r, double* z) {
 prbf_status prbf_rasm_apply(prbf_ctx* c, const double* r, double* z) {
   return guarded(c, [&] {
     need(c->rasm_ready, PRBF_ERR_STATE, "rasm_setup first");
+    need(r && z, PRBF_ERR_ARG, "null vector");
     DBuf<double> dr, dz;
     dr.ensure(c->ni);
     dz.ensure(c->ni);
@@ -1299,6 +1320,7 @@ prbf_status prbf_solve(prbf_ctx* c, const prbf_solve_opts* opts, const double* w
                        prbf_solve_info* info, double* hist, int hist_cap) {
   return guarded(c, [&] {
     need(c->has_points && c->same, PRBF_ERR_STATE, "set_points (one set) first");
+    need(w && gamma, PRBF_ERR_ARG, "null vector");
     DBuf<double> dw, dg;
     dw.ensure(c->ni);
     dg.ensure(c->ni);
@@ -1374,6 +1396,12 @@ prbf_status guarded_mlp(prbf_mlp* net, F&& f) {
   } catch (const HipError& e) {
     net->err = e.what;
     return PRBF_ERR_HIP;
+  } catch (const std::bad_alloc&) {
+    net->err = "host allocation failed";
+    return PRBF_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    net->err = e.what();
+    return PRBF_ERR_ARG;
   }
 }
 

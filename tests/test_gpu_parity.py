@@ -278,6 +278,34 @@ def test_errors_are_reported_not_crashed():
     ctx.set_points(np.array([0.0, 0.5]), np.array([0.0, 0.1]))
     with pytest.raises(RuntimeError):
         ctx.matvec(np.zeros((4, 2)))           # nrhs out of range
+    # null pointers and bad options through the C ABI: PRBF_ERR_ARG, nothing launched
+    import ctypes as C
+    from petrbf_amd import _lib as L
+    lib, h = ctx._L, ctx._h
+    x = np.array([0.0, 0.5])
+    px = x.ctypes.data
+    v = np.zeros(2)
+    pv = v.ctypes.data
+    assert lib.prbf_set_points(h, 2, px, None, None, 0, None, None, None) == L.PRBF_ERR_ARG
+    assert lib.prbf_set_points(h, 2, px, px, None, 2, px, None, None) == L.PRBF_ERR_ARG
+    assert lib.prbf_matvec(h, 1, None, pv) == L.PRBF_ERR_ARG
+    assert lib.prbf_matvec_dev(h, 1, None, None) == L.PRBF_ERR_ARG
+    assert lib.prbf_evaluate_dev(h, 1, None, None) == L.PRBF_ERR_ARG
+    opts = L.PrbfSolveOpts()
+    lib.prbf_solve_opts_default(C.byref(opts))
+    opts.pc = L.PRBF_PC_NONE
+    assert lib.prbf_solve_dev(h, C.byref(opts), None, None, None, None, 0) == L.PRBF_ERR_ARG
+    for field, bad in (("ksp", 7), ("pc", 9), ("max_it", -1), ("rtol", -1.0), ("dtol", 0.0)):
+        o = L.PrbfSolveOpts()
+        lib.prbf_solve_opts_default(C.byref(o))
+        o.pc = L.PRBF_PC_NONE
+        setattr(o, field, bad)
+        assert lib.prbf_solve(h, C.byref(o), pv, pv, None, None, 0) == L.PRBF_ERR_ARG, field
+    assert lib.prbf_rasm_setup(h) == L.PRBF_OK
+    assert lib.prbf_rasm_apply_dev(h, None, None) == L.PRBF_ERR_ARG
+    # the context still works after the rejected calls
+    w = ctx.matvec(np.array([1.0, 0.0]))
+    assert np.isfinite(w).all() and w[0] > 0
     ctx.close()
 
 
