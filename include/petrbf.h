@@ -7,7 +7,11 @@
  *
  * Conventions
  *   - Every function returns a prbf_status (0 = PRBF_OK).  prbf_last_error()
- *     gives the message of the last failure on that context.
+ *     gives the message of the last failure on that context.  Null vectors,
+ *     null coordinate arrays and out-of-range solver options are rejected
+ *     with PRBF_ERR_ARG before anything is copied or launched; a failed host
+ *     allocation inside a call returns PRBF_ERR_NOMEM.  No C++ exception
+ *     crosses the boundary.
  *   - "sorted" arrays are in cluster order (particles counting-sorted by cell,
  *     2d/get_cluster.h:77-138), the order of the reference's MatShell vectors;
  *     "original" arrays are in the caller's order.
