@@ -18,6 +18,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpetrbf.so")
+ARCHIVE = os.path.join(HERE, "libpetrbf.a")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 ARCH = "gfx950"
@@ -50,8 +51,16 @@ def build_library(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # the same objects as a static archive, for callers that link the library
+    # into their executable (the reference's 3D tree ships librbf.a,
+    # 3d/Makefile:12-14); link it with hipcc so the HIP runtime comes along
+    if os.path.exists(ARCHIVE):
+        os.remove(ARCHIVE)
+    r = subprocess.run(["ar", "rcs", ARCHIVE, *objs], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"ar failed:\n{r.stdout}\n{r.stderr}")
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {ARCHIVE}")
     return LIB
 
 
