@@ -72,7 +72,8 @@ def test_static_archive_exports_every_symbol_and_links_a_c_caller(tmp_path):
     subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), "-c", str(src),
                     "-o", str(obj)], check=True)
     exe = tmp_path / "caller"
-    r = subprocess.run(["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", str(obj), ar, "-o", str(exe)],
+    from petrbf_amd.build import ARCH, HIPCC
+    r = subprocess.run([HIPCC, f"--offload-arch={ARCH}", str(obj), ar, "-o", str(exe)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-2000:]
     assert exe.exists()
