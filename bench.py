@@ -4,7 +4,8 @@ One step = the whole solve path with the inputs already in HBM: cell list and
 sort, buffer / truncation lists, RASM setup (blocks, factorisation, core rows)
 and GMRES(30) with left RASM to rtol 1e-10, from particles and RHS to gamma.
 Workload: 2000 x 2000 jittered lattice on [-1,1]^2, sigma = h/0.8, 4x4-spacing
-cells, two-cell buffer, truncation 5 sigma beyond the cell, Lamb-Oseen RHS.
+cells, a buffer of 8 spacings (two cells) per side (sigma_buffer = 16 sigma,
+blocks of ~390 unknowns), truncation 5 sigma beyond the cell, Lamb-Oseen RHS.
 
 Prints one JSON line (see the keys below).  --gpus > 1 is refused: this build
 measures one GPU.
@@ -255,8 +256,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if head == "plain" else "f64 (Krylov, matvec) + f32 (RASM blocks)",
         "data": "synthetic",
-        "config": {"workload": f"2D jittered lattice {args.n_side}x{args.n_side} (N={p.n}), 4x4-spacing cells, "
-                               "2-cell buffer, 5 sigma truncation, Lamb-Oseen RHS, rtol 1e-10",
+        "config": {"workload": f"2D jittered lattice {args.n_side}x{args.n_side} (N={p.n}), sigma = h/0.8, "
+                               "4x4-spacing cells, buffer of 8 spacings (2 cells) per side (sigma_buffer = 16 sigma, "
+                               f"blocks of up to {info['max_buffer']} unknowns), 5 sigma truncation beyond the cell, "
+                               "Lamb-Oseen RHS, rtol 1e-10",
                    "mode": head, "n": p.n, "ncell": info["ncell"]},
         "modes": results,
         "speedup_mixed_vs_plain": (results["plain"]["ms_per_step"] / results["mixed"]["ms_per_step"]
@@ -277,7 +280,7 @@ def main():
     }
     # the dominant kernel of the headline (mixed) step: the FP32 block
     # factorisation, one persistent launch per setup, timed with HIP events on
-    # the library's stream; FLOPs = sum over blocks of n^3/3 + 2 n^2 T
+    # the library's stream; FLOPs = sum over blocks of n^3/3 + n^2 T + T^3
     # (template arguments: precision, dimension, wavefronts, panel columns,
     # left-looking)
     for mode, key, peak, name in (("mixed", "roofline", PEAK_FP32_MFMA_TFLOPS, "k_block_chol<float,2,%d,%d,%s>"),

@@ -60,8 +60,9 @@ typedef struct {
 
 typedef enum { PRBF_KSP_GMRES = 0, PRBF_KSP_FGMRES = 1 } prbf_ksp;
 /* RASM_FP64: exact FP64 block solves (the reference's -sub_pc_type lu).
- * RASM_FP32: the same blocks factorised and solved in FP32 on the matrix
- * cores, applied with FP64 accumulation; meant for FGMRES. */
+ * RASM_FP32: blocks of 32 or more unknowns factorised and solved in FP32 on
+ * the matrix cores, smaller ones in FP64, all applied with FP64
+ * accumulation; meant for FGMRES. */
 /* LEARNED: the network of prbf_learned_attach() on the clusters it accepted,
  * exact FP32 blocks on the others; FGMRES only. */
 typedef enum { PRBF_PC_NONE = 0, PRBF_PC_RASM_FP64 = 1, PRBF_PC_RASM_FP32 = 2, PRBF_PC_LEARNED = 3 } prbf_pc;
@@ -129,9 +130,10 @@ prbf_status prbf_set_points_dev(prbf_ctx* ctx, int64_t ni, const double* xi, con
  * and order (prbf_chol_order) the last block factorisation used (0 before
  * any).
  * geom[0..7]: xmin, ymin, zmin, box, buffer half-width, trunc half-width, the
- * device time (s) of the last block-factorisation launch and its algorithmic
- * FLOPs, sum over blocks of n^3/3 + 2 n^2 T (Cholesky and the two triangular
- * solves with the T core right-hand sides). */
+ * device time (s) of the last setup's block-factorisation launches and their
+ * algorithmic FLOPs, sum over blocks of n^3/3 + n^2 T + T^3 (Cholesky, the
+ * backward substitution of the T core right-hand sides through all n rows and
+ * their forward substitution, which starts at the core, ordered last). */
 prbf_status prbf_get_info(prbf_ctx* ctx, int64_t* info, double* geom);
 /* Sorted position -> original index (the reference's particle->ii / jj). */
 prbf_status prbf_get_permutation(prbf_ctx* ctx, int32_t* perm_i, int32_t* perm_j);
@@ -179,6 +181,14 @@ prbf_status prbf_set_tuning(prbf_ctx* ctx, int key, int value);
 /* PCApply_ASM (restrict): z = sum_c P_core A_c^{-1} R_buffer r, sorted order. */
 prbf_status prbf_rasm_apply(prbf_ctx* ctx, const double* r_sorted, double* z_sorted);
 prbf_status prbf_rasm_apply_dev(prbf_ctx* ctx, const double* r_sorted, double* z_sorted);
+/* The dense untruncated RASM block of cluster c, A[i*n + j] = G(x_i - x_j)
+ * over its buffer particles (FP64, row-major, n x n), and the particles'
+ * sorted indices idx[0..n) in the order of the PCASM subdomain `is` (core
+ * first, then the neighbour cells'): what the MatShell GET_SUBMATRICES
+ * callback mysubmat builds per cluster (2d/matmult.cxx:53-104, subdomains
+ * from 2d/rbf_interpolation.cxx:214-229).  A or idx may be NULL; n receives
+ * the block size.  Needs prbf_set_points with one particle set. */
+prbf_status prbf_rasm_get_block(prbf_ctx* ctx, int64_t c, double* A, int32_t* idx, int32_t* n);
 /* Core rows of A_c^{-1} for cluster c: M[t*n + b], t < T core members,
  * b < n buffer slots; n and T returned.  Pass M == NULL to query the sizes. */
 prbf_status prbf_rasm_get_core_rows(prbf_ctx* ctx, int64_t c, double* M, int32_t* n, int32_t* T);

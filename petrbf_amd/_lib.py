@@ -29,6 +29,7 @@ EXPORTS = [
     "prbf_get_permutation", "prbf_get_cells", "prbf_get_buffer_csr", "prbf_get_trunc_csr",
     "prbf_get_sorted_points", "prbf_matvec", "prbf_matvec_dev", "prbf_evaluate", "prbf_evaluate_dev",
     "prbf_rasm_setup", "prbf_rasm_setup_pc", "prbf_set_tuning", "prbf_rasm_apply", "prbf_rasm_apply_dev", "prbf_rasm_get_core_rows",
+    "prbf_rasm_get_block",
     "prbf_solve", "prbf_solve_dev", "prbf_rbf_interpolation_f32",
     "prbf_mlp_num_params", "prbf_mlp_create", "prbf_mlp_destroy", "prbf_mlp_last_error", "prbf_mlp_set_params",
     "prbf_mlp_get_params", "prbf_mlp_forward_dev", "prbf_mlp_loss_grad_dev", "prbf_mlp_train_dev",
@@ -105,6 +106,7 @@ def load():
     for f in ("prbf_rasm_apply", "prbf_rasm_apply_dev"):
         getattr(L, f).argtypes = [ctx, dp, dp]
     L.prbf_rasm_get_core_rows.argtypes = [ctx, i64, dp, C.POINTER(i32), C.POINTER(i32)]
+    L.prbf_rasm_get_block.argtypes = [ctx, i64, dp, dp, C.POINTER(i32)]
     for f in ("prbf_solve", "prbf_solve_dev"):
         getattr(L, f).argtypes = [ctx, C.POINTER(PrbfSolveOpts), dp, dp, C.POINTER(PrbfSolveInfo), dp, i32]
     L.prbf_rbf_interpolation_f32.argtypes = [C.POINTER(PrbfParams), i64, dp, dp, dp, dp, i64, dp, dp, dp, dp,

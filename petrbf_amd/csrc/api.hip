@@ -111,11 +111,18 @@ struct prbf_ctx {
   int max_buffer = 0, max_core = 0, max_trunc = 0;
   std::vector<long long> h_bptr;
   std::vector<int> h_ista, h_iend;
-  // RASM
+  // RASM: core rows of the FP32 blocks (M32, offsets mptr) and of the FP64
+  // blocks (M, offsets mptr64); per cluster the precision of its block
+  // (h_prec: 0 none, 1 FP64, 2 FP32) and, per precision, the list of the
+  // clusters the apply serves (null list: every cluster)
   bool rasm_ready = false;
-  DBuf<long long> mptr;
+  DBuf<long long> mptr, mptr64;
   DBuf<double> M;
-  std::vector<long long> h_mptr;
+  std::vector<long long> h_mptr, h_mptr64;
+  std::vector<unsigned char> h_prec;
+  DBuf<int> l32, l64;
+  long long n32 = 0, n64 = 0;
+  bool l32_all = false, l64_all = false;
   // scratch
   DBuf<int> i_scr1, i_scr2, i_scr3, d_err;
   DBuf<long long> l_scr;
@@ -140,11 +147,11 @@ struct prbf_ctx {
   // executable graph per step index, and the launch arguments they captured
   int last_restart = 0;  // basis size of the last Krylov solve
   // learned preconditioner: patches of the eligible clusters, the attached
-  // network, the clusters it serves (skip[c] = 1) and inference buffers
+  // network, the clusters it serves (h_skip[c] = 1) and inference buffers
   bool patches_ready = false, learned_ready = false;
   DBuf<int> pidx, cidx, elig, mlp_list;
   DBuf<double> pctr;
-  DBuf<unsigned char> pok, skip;
+  DBuf<unsigned char> pok;
   long long n_elig = 0, n_mlp = 0;
   int n_net_fallback = 0;
   std::vector<unsigned char> h_skip;
@@ -192,6 +199,8 @@ prbf_status guarded(prbf_ctx* c, F&& f) {
     return fail(c, PRBF_ERR_NOMEM, "host allocation failed");
   } catch (const std::exception& e) {
     return fail(c, PRBF_ERR_ARG, e.what());
+  } catch (...) {
+    return fail(c, PRBF_ERR_HIP, "unexpected error");
   }
 }
 
@@ -401,6 +410,20 @@ std::vector<std::pair<long long, CholCfg>>& tuned_cache() {
   return v;
 }
 
+// Algorithmic FLOPs of the block setups of the clusters in `order`: per block
+// of n unknowns with T core members, the Cholesky factorisation (n^3/3), the
+// backward substitution of the T core right-hand sides through all n rows
+// (n^2 T) and their forward substitution, which starts at the core because
+// the core is ordered last (T^3: T right-hand sides over T rows).
+double block_flops(const prbf_ctx* c, const std::vector<int>& order) {
+  double fl = 0;
+  for (int k : order) {
+    const double nb = (double)(c->h_bptr[k + 1] - c->h_bptr[k]), T = c->h_iend[k] - c->h_ista[k] + 1;
+    fl += nb * nb * nb / 3.0 + nb * nb * T + T * T * T;
+  }
+  return fl;
+}
+
 template <typename Real>
 void launch_blocks(prbf_ctx* c, bool fp32, const CholCfg& cfg, long long nwork, const int* order, int* next, int nwg,
                    long long ws_slot, int nmax, int tmax, Real* W, const long long* mptr, Real* Mout, int* status) {
@@ -511,11 +534,111 @@ void factor_blocks(prbf_ctx* c, bool fp32, const std::vector<int>& order, int nm
   float ms = 0;
   PRBF_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->t_chol_s = ms * 1e-3;
-  double fl = 0;
-  for (int k : order) {
-    const double nb = (double)(c->h_bptr[k + 1] - c->h_bptr[k]), T = c->h_iend[k] - c->h_ista[k] + 1;
-    fl += nb * nb * nb / 3.0 + 2.0 * nb * nb * T;
+  c->chol_flops = block_flops(c, order);
+}
+
+// Blocks of fewer unknowns than this are factorised and applied in FP64 by
+// the mixed-precision preconditioners; larger ones in FP32 on the matrix cores.
+constexpr int kFp32MinUnknowns = 32;
+
+// Largest block first (stable counting sort by block size): the persistent
+// workgroups of the factorisation then finish together.
+void largest_first(const prbf_ctx* c, std::vector<int>& order) {
+  const int nb = c->max_buffer + 1;
+  auto sz = [&](int k) { return (int)(c->h_bptr[k + 1] - c->h_bptr[k]); };
+  std::vector<int> cnt(nb + 1, 0), sorted(order.size());
+  for (int k : order) ++cnt[nb - 1 - sz(k) + 1];
+  for (int i = 0; i < nb; ++i) cnt[i + 1] += cnt[i];
+  for (int k : order) sorted[cnt[nb - 1 - sz(k)]++] = k;
+  order.swap(sorted);
+}
+
+// The apply's cluster lists from h_prec and the network membership h_skip:
+// the clusters with a block of each precision that the network does not
+// serve, in cluster order; a list that holds every nonempty cluster is
+// replaced by a launch over all clusters (empty ones return at once).
+void build_apply_lists(prbf_ctx* c, bool learned) {
+  const long long nc = c->g.ncell;
+  std::vector<int> a32, a64;
+  long long nonempty = 0;
+  for (long long k = 0; k < nc; ++k) {
+    if (!c->h_prec[k]) continue;
+    ++nonempty;
+    if (learned && c->h_skip[k]) continue;
+    (c->h_prec[k] == 2 ? a32 : a64).push_back((int)k);
   }
+  c->l32_all = (long long)a32.size() == nonempty;
+  c->l64_all = (long long)a64.size() == nonempty;
+  c->n32 = c->l32_all ? nc : (long long)a32.size();
+  c->n64 = c->l64_all ? nc : (long long)a64.size();
+  c->l32.ensure(std::max<size_t>(1, a32.size()));
+  c->l64.ensure(std::max<size_t>(1, a64.size()));
+  if (!c->l32_all && !a32.empty())
+    PRBF_HIP(hipMemcpyAsync(c->l32, a32.data(), sizeof(int) * a32.size(), hipMemcpyHostToDevice, c->st));
+  if (!c->l64_all && !a64.empty())
+    PRBF_HIP(hipMemcpyAsync(c->l64, a64.data(), sizeof(int) * a64.size(), hipMemcpyHostToDevice, c->st));
+  sync(c);  // the host vectors are released on return
+}
+
+// Factorises the blocks of `order` (any precisions, per h_prec) and writes
+// their core rows; throws PRBF_ERR_FACTOR when an FP32 block breaks down or
+// an FP64 block is not numerically positive definite.
+void factor_clusters(prbf_ctx* c, const std::vector<int>& order) {
+  std::vector<int> o32, o64;
+  int nmax32 = 0, tmax32 = 0, nmax64 = 0, tmax64 = 0;
+  for (int k : order) {
+    const int n = (int)(c->h_bptr[k + 1] - c->h_bptr[k]), T = c->h_iend[k] - c->h_ista[k] + 1;
+    if (c->h_prec[k] == 2) {
+      o32.push_back(k);
+      nmax32 = std::max(nmax32, n);
+      tmax32 = std::max(tmax32, T);
+    } else if (c->h_prec[k] == 1) {
+      o64.push_back(k);
+      nmax64 = std::max(nmax64, n);
+      tmax64 = std::max(tmax64, T);
+    }
+  }
+  largest_first(c, o32);
+  largest_first(c, o64);
+  double t_k = 0, fl = 0;
+  int used[3] = {0, 0, 0};
+  if (!o64.empty()) {
+    PRBF_HIP(hipMemsetAsync(c->d_err, 0, sizeof(int), c->st));
+    factor_blocks<double>(c, false, o64, nmax64, tmax64, c->mptr64, c->M.p, nullptr, true);
+    int herr = 0;
+    PRBF_HIP(hipMemcpyAsync(&herr, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    sync(c);
+    need(herr == 0, PRBF_ERR_FACTOR, "a RASM block is not numerically positive definite");
+    t_k += c->t_chol_s;
+    fl += c->chol_flops;
+    used[0] = c->used_nw, used[1] = c->used_npc, used[2] = c->used_order;
+  }
+  if (!o32.empty()) {
+    const long long nc = c->g.ncell;
+    DBuf<int>& status = c->fb_status;
+    status.ensure(nc);
+    PRBF_HIP(hipMemsetAsync(status, 0, sizeof(int) * nc, c->st));
+    factor_blocks<float>(c, true, o32, nmax32, tmax32, c->mptr, c->M32.p, status, true);
+    // the blocks whose FP32 factorisation broke down, counted on the device
+    DBuf<int>& cnt = c->fb_next;  // the counter of the finished launch
+    c->fb_failed.ensure(o32.size());
+    PRBF_HIP(hipMemsetAsync(cnt, 0, sizeof(int), c->st));
+    failed_blocks(c->st, (long long)o32.size(), c->fb_order, status, c->fb_failed, cnt);
+    int nf = 0;
+    PRBF_HIP(hipMemcpyAsync(&nf, cnt, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    sync(c);
+    // a block whose FP32 factorisation broke down is reported, not redone
+    if (nf > 0) {
+      const std::string msg = "the FP32 factorisation of " + std::to_string(nf) +
+                              " RASM blocks broke down (not positive definite in FP32); use PRBF_PC_RASM_FP64";
+      need(false, PRBF_ERR_FACTOR, msg.c_str());
+    }
+    t_k += c->t_chol_s;
+    fl += c->chol_flops;
+  } else if (used[0]) {
+    c->used_nw = used[0], c->used_npc = used[1], c->used_order = used[2];
+  }
+  c->t_chol_s = t_k;
   c->chol_flops = fl;
 }
 
@@ -524,76 +647,41 @@ void rasm_setup_impl(prbf_ctx* c, int pc) {
   need(c->same, PRBF_ERR_STATE, "RASM needs the source set equal to the target set");
   need(pc == PRBF_PC_RASM_FP64 || pc == PRBF_PC_RASM_FP32 || pc == PRBF_PC_LEARNED, PRBF_ERR_ARG,
        "unknown preconditioner");
-  const bool fp32 = (pc != PRBF_PC_RASM_FP64);
+  const bool mixed = (pc != PRBF_PC_RASM_FP64);
   // with the learned preconditioner only the clusters off the network get blocks
   const bool learned = (pc == PRBF_PC_LEARNED);
   need(!learned || c->h_skip.size() == (size_t)c->g.ncell, PRBF_ERR_STATE, "prbf_learned_attach first");
   const Grid& g = c->g;
   const long long nc = g.ncell;
+  c->rasm_ready = false;
+  // block precision per cluster, and storage for every block of each
+  // precision (network clusters too, so a later fallback can add theirs)
+  c->h_prec.assign(nc, 0);
   c->h_mptr.assign(nc + 1, 0);
+  c->h_mptr64.assign(nc + 1, 0);
   std::vector<int> order;
   order.reserve(nc);
   for (long long k = 0; k < nc; ++k) {
-    long long n = c->h_bptr[k + 1] - c->h_bptr[k];
-    long long T = n ? (c->h_iend[k] - c->h_ista[k] + 1) : 0;
-    // (space is kept for network clusters too, so a later fallback can add their blocks)
-    c->h_mptr[k + 1] = c->h_mptr[k] + apply_elems((int)n, (int)T);
+    const long long n = c->h_bptr[k + 1] - c->h_bptr[k];
+    const long long T = n ? (c->h_iend[k] - c->h_ista[k] + 1) : 0;
+    const long long e = apply_elems((int)n, (int)T);
+    c->h_prec[k] = n == 0 ? 0 : (mixed && n >= kFp32MinUnknowns) ? 2 : 1;
+    c->h_mptr[k + 1] = c->h_mptr[k] + (c->h_prec[k] == 2 ? e : 0);
+    c->h_mptr64[k + 1] = c->h_mptr64[k] + (c->h_prec[k] == 1 ? e : 0);
     if (n && !(learned && c->h_skip[k])) order.push_back((int)k);
   }
-  // largest blocks first (stable counting sort by size): the persistent
-  // workgroups then finish together
-  {
-    const int nb = c->max_buffer + 1;
-    std::vector<int> cnt(nb + 1, 0), sorted(order.size());
-    for (int k : order) ++cnt[nb - 1 - (int)(c->h_bptr[k + 1] - c->h_bptr[k]) + 1];
-    for (int i = 0; i < nb; ++i) cnt[i + 1] += cnt[i];
-    for (int k : order) sorted[cnt[nb - 1 - (int)(c->h_bptr[k + 1] - c->h_bptr[k])]++] = k;
-    order.swap(sorted);
-  }
-  need(block_lds_bytes(fp32, c->max_buffer, c->max_core, 4, 1, false) <= kMaxLdsBytes, PRBF_ERR_LIMIT,
+  need(block_lds_bytes(mixed, c->max_buffer, c->max_core, 4, 1, false) <= kMaxLdsBytes, PRBF_ERR_LIMIT,
        "RASM block too large for the LDS panel of the block factorisation");
-  c->rasm_ready = false;
   c->mptr.ensure(nc + 1);
+  c->mptr64.ensure(nc + 1);
   PRBF_HIP(hipMemcpyAsync(c->mptr, c->h_mptr.data(), sizeof(long long) * (nc + 1), hipMemcpyHostToDevice, c->st));
-  PRBF_HIP(hipMemsetAsync(c->d_err, 0, sizeof(int), c->st));
+  PRBF_HIP(hipMemcpyAsync(c->mptr64, c->h_mptr64.data(), sizeof(long long) * (nc + 1), hipMemcpyHostToDevice,
+                          c->st));
+  c->M.ensure(c->h_mptr64[nc]);
+  c->M32.ensure(c->h_mptr[nc]);
   c->n_fallback = 0;
-  if (!fp32) {
-    c->M.ensure(c->h_mptr[nc]);
-    factor_blocks<double>(c, false, order, c->max_buffer, c->max_core, c->mptr, c->M.p, nullptr, true);
-  } else {
-    c->M32.ensure(c->h_mptr[nc]);
-    DBuf<int>& status = c->fb_status;
-    status.ensure(nc);
-    PRBF_HIP(hipMemsetAsync(status, 0, sizeof(int) * nc, c->st));
-    factor_blocks<float>(c, true, order, c->max_buffer, c->max_core, c->mptr, c->M32.p, status, true);
-    // the blocks whose FP32 factorisation broke down, listed on the device
-    std::vector<int> failed;
-    {
-      DBuf<int>& cnt = c->fb_next;  // the counter of the finished launch
-      c->fb_failed.ensure(std::max<size_t>(1, order.size()));
-      PRBF_HIP(hipMemsetAsync(cnt, 0, sizeof(int), c->st));
-      failed_blocks(c->st, (long long)order.size(), c->fb_order, status, c->fb_failed, cnt);
-      int nf = 0;
-      PRBF_HIP(hipMemcpyAsync(&nf, cnt, sizeof(int), hipMemcpyDeviceToHost, c->st));
-      sync(c);
-      if (nf > 0) {
-        failed.resize(nf);
-        PRBF_HIP(hipMemcpy(failed.data(), c->fb_failed, sizeof(int) * nf, hipMemcpyDeviceToHost));
-        std::sort(failed.begin(), failed.end());
-      }
-    }
-    // a block whose FP32 factorisation broke down is reported, not redone in
-    // FP64 (that path is switched off)
-    if (!failed.empty()) {
-      const std::string msg = "the FP32 factorisation of " + std::to_string(failed.size()) +
-                              " RASM blocks broke down (not positive definite in FP32); use PRBF_PC_RASM_FP64";
-      need(false, PRBF_ERR_FACTOR, msg.c_str());
-    }
-  }
-  int herr = 0;
-  PRBF_HIP(hipMemcpyAsync(&herr, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->st));
-  sync(c);
-  need(herr == 0, PRBF_ERR_FACTOR, "a RASM block is not numerically positive definite");
+  build_apply_lists(c, learned);
+  factor_clusters(c, order);
   c->rasm_ready = true;
   c->rasm_pc = pc;
 }
@@ -616,14 +704,12 @@ void network_apply(prbf_ctx* c, const int* list, long long n, const double* r, d
 void rasm_apply_impl(prbf_ctx* c, const double* r, double* z) {
   need(c->rasm_ready, PRBF_ERR_STATE, "rasm_setup first");
   need(r && z, PRBF_ERR_ARG, "null vector");
-  if (c->rasm_pc == PRBF_PC_LEARNED) {
-    need(c->learned_ready && c->net, PRBF_ERR_STATE, "prbf_learned_attach first");
-    rasm_apply_fp32(c->st, c->g.ncell, c->bptr, c->bidx, c->ista, c->iend, c->mptr, c->M32, c->skip, r, z);
-    network_apply(c, c->mlp_list, c->n_mlp, r, z);
-  } else if (c->rasm_pc == PRBF_PC_RASM_FP32)
-    rasm_apply_fp32(c->st, c->g.ncell, c->bptr, c->bidx, c->ista, c->iend, c->mptr, c->M32, nullptr, r, z);
-  else
-    rasm_apply_fp64(c->st, c->g.ncell, c->bptr, c->bidx, c->ista, c->iend, c->mptr, c->M, r, z, c->max_core);
+  if (c->rasm_pc == PRBF_PC_LEARNED) need(c->learned_ready && c->net, PRBF_ERR_STATE, "prbf_learned_attach first");
+  rasm_apply_fp32(c->st, c->n32, c->l32_all ? nullptr : c->l32.p, c->bptr, c->bidx, c->ista, c->iend, c->mptr, c->M32,
+                  r, z);
+  rasm_apply_fp64(c->st, c->n64, c->l64_all ? nullptr : c->l64.p, c->bptr, c->bidx, c->ista, c->iend, c->mptr64, c->M,
+                  r, z);
+  if (c->rasm_pc == PRBF_PC_LEARNED) network_apply(c, c->mlp_list, c->n_mlp, r, z);
   PRBF_HIP(hipGetLastError());
 }
 // ---------------------------------------------------------------- Krylov
@@ -656,10 +742,13 @@ struct Timed {
   }
 };
 
-// Moves clusters from the network to exact FP32 blocks (the learned
-// preconditioner's per-cluster fallback) and factorises their blocks.
+// Moves clusters from the network to exact blocks (the learned
+// preconditioner's per-cluster fallback) and factorises their blocks.  The
+// context is marked not set up until the blocks are in place, so a failed
+// factorisation leaves a state that the next solve sets up again.
 void learned_fallback(prbf_ctx* c, const std::vector<int>& newc) {
   if (newc.empty()) return;
+  c->rasm_ready = false;
   for (int k : newc) c->h_skip[k] = 0;
   std::vector<int> list;
   for (long long k = 0; k < c->g.ncell; ++k)
@@ -667,24 +756,11 @@ void learned_fallback(prbf_ctx* c, const std::vector<int>& newc) {
   c->n_mlp = (long long)list.size();
   if (!list.empty())
     PRBF_HIP(hipMemcpyAsync(c->mlp_list, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice, c->st));
-  PRBF_HIP(hipMemcpyAsync(c->skip, c->h_skip.data(), c->g.ncell, hipMemcpyHostToDevice, c->st));
-  std::vector<int> order(newc);
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-    return c->h_bptr[a + 1] - c->h_bptr[a] > c->h_bptr[b + 1] - c->h_bptr[b];
-  });
-  DBuf<int> status;
-  status.ensure(c->g.ncell);
-  PRBF_HIP(hipMemsetAsync(status, 0, sizeof(int) * c->g.ncell, c->st));
-  factor_blocks<float>(c, true, order, c->max_buffer, c->max_core, c->mptr, c->M32.p, status, true);
-  std::vector<int> h(c->g.ncell);
-  PRBF_HIP(hipMemcpy(h.data(), status, sizeof(int) * c->g.ncell, hipMemcpyDeviceToHost));
-  bool failed = false;
-  for (int k : newc) failed |= h[k] != 0;
-  c->n_net_fallback += (int)newc.size();
   c->h_mlp_list = list;
-  sync(c);
-  need(!failed, PRBF_ERR_FACTOR,
-       "the FP32 factorisation of a RASM block broke down (not positive definite in FP32); use PRBF_PC_RASM_FP64");
+  c->n_net_fallback += (int)newc.size();
+  build_apply_lists(c, true);
+  factor_clusters(c, newc);
+  c->rasm_ready = true;
 }
 
 // Preconditioner application without timing (usable under graph capture).
@@ -740,7 +816,9 @@ void ensure_graphs(prbf_ctx* c, int pc, bool flex, long long n, int m) {
       (unsigned long long)c->nj, (unsigned long long)c->prm.dim, sig, u(c->V.p), u(c->Z.p), u(c->v1.p),
       u(c->part.p), u(c->scal.p), u(c->kst.p), u(c->pin), u(c->ista.p), u(c->iend.p), u(c->tptr.p), u(c->tidx.p),
       u(c->xi.p), u(c->yi.p), u(c->zi.p), u(c->xj_()), u(c->yj_()), u(c->zj_()), u(c->bptr.p), u(c->bidx.p),
-      u(c->mptr.p), u(c->M.p), u(c->M32.p), u(c->skip.p), u(c->mlp_list.p),
+      u(c->mptr.p), u(c->M.p), u(c->M32.p), u(c->mptr64.p), u(c->l32.p), u(c->l64.p),
+      (unsigned long long)c->n32, (unsigned long long)c->n64, (unsigned long long)c->l32_all,
+      (unsigned long long)c->l64_all, u(c->mlp_list.p),
       (unsigned long long)c->n_mlp, u(c->pidx.p), u(c->cidx.p), u(c->pctr.p), u(c->lX.p), u(c->ls.p), u(c->net)};
   if (c->net) {
     unsigned long long os;
@@ -1048,7 +1126,8 @@ prbf_status prbf_create(prbf_ctx** out, const prbf_params* p) {
   if ((p->dim != 2 && p->dim != 3) || !(p->sigma > 0) || !(p->nsigma_box > 0) || !(p->sigma_buffer > 0) ||
       !(p->sigma_trunc > 0))
     return PRBF_ERR_ARG;
-  prbf_ctx* c = new prbf_ctx;
+  prbf_ctx* c = new (std::nothrow) prbf_ctx;
+  if (!c) return PRBF_ERR_NOMEM;
   c->prm = *p;
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
       hipHostMalloc((void**)&c->pin, 512 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
@@ -1303,11 +1382,31 @@ prbf_status prbf_rasm_get_core_rows(prbf_ctx* c, int64_t k, double* M, int32_t* 
     if (T) *T = tt;
     if (M && nn) {
       std::vector<double> st((size_t)apply_elems(nn, tt));
-      d2h(c, st.data(), c->M.p + c->h_mptr[k], sizeof(double) * st.size());
+      d2h(c, st.data(), c->M.p + c->h_mptr64[k], sizeof(double) * st.size());
       sync(c);
       for (int t = 0; t < tt; ++t)
         for (int b = 0; b < nn; ++b) M[(size_t)t * nn + b] = st[apply_offset<2>(nn, tt, t, b)];
     }
+  });
+}
+
+prbf_status prbf_rasm_get_block(prbf_ctx* c, int64_t k, double* A, int32_t* idx, int32_t* n) {
+  return guarded(c, [&] {
+    need(c->has_points && c->same, PRBF_ERR_STATE, "set_points (one set) first");
+    need(k >= 0 && k < c->g.ncell, PRBF_ERR_ARG, "cluster index out of range");
+    const long long b0 = c->h_bptr[k];
+    const int nn = (int)(c->h_bptr[k + 1] - b0);
+    if (n) *n = nn;
+    if (idx && nn) d2h(c, idx, c->bidx.p + b0, sizeof(int) * nn);
+    if (A && nn) {
+      DBuf<double> dA;
+      dA.ensure((size_t)nn * nn);
+      dense_block(c->st, c->prm.dim, b0, nn, c->bidx, c->xi, c->yi, c->zi, c->prm.sigma, dA);
+      PRBF_HIP(hipGetLastError());
+      d2h(c, A, dA.p, sizeof(double) * nn * nn);
+      sync(c);  // before the device copy is released
+    }
+    sync(c);
   });
 }
 
@@ -1331,18 +1430,10 @@ prbf_status prbf_solve(prbf_ctx* c, const prbf_solve_opts* opts, const double* w
   });
 }
 
-// RBFInterpolation (3d/wrapper.cxx:20-105): FP32 grid values in, grid
-// strengths out.  When the grid is not the particle set (first coordinates
-// differ by more than 1e-6, 3d/wrapper.cxx:93) the field of the particles
-// (strengths gp) is first evaluated at the grid; then the grid strengths
-// solving A~ g = w are written back over wg.
-prbf_status prbf_rbf_interpolation_f32(const prbf_params* p, int64_t ng, const float* xg, const float* yg,
-                                       const float* zg, float* wg, int64_t np_, const float* xp, const float* yp,
-                                       const float* zp, const float* gp, const prbf_solve_opts* opts, int* its) {
-  if (!p || ng <= 0 || !xg || !yg || !wg || (p->dim == 3 && !zg)) {
-    g_free_err = "invalid arguments";
-    return PRBF_ERR_ARG;
-  }
+static prbf_status rbf_interpolation_f32_impl(const prbf_params* p, int64_t ng, const float* xg, const float* yg,
+                                              const float* zg, float* wg, int64_t np_, const float* xp,
+                                              const float* yp, const float* zp, const float* gp,
+                                              const prbf_solve_opts* opts, int* its) {
   const int dim = p->dim;
   auto up = [](const float* a, int64_t n) {
     std::vector<double> v(a ? n : 0);
@@ -1350,12 +1441,17 @@ prbf_status prbf_rbf_interpolation_f32(const prbf_params* p, int64_t ng, const f
     return v;
   };
   std::vector<double> gx = up(xg, ng), gy = up(yg, ng), gz = up(dim == 3 ? zg : nullptr, ng), w = up(wg, ng);
+  std::vector<double> g(ng);
   prbf_ctx* c = nullptr;
   prbf_status st = prbf_create(&c, p);
   if (st) {
     g_free_err = "prbf_create failed";
     return st;
   }
+  struct Owner {  // destroys the context on every way out, exceptions included
+    prbf_ctx* c;
+    ~Owner() { prbf_destroy(c); }
+  } owner{c};
   const bool remesh = np_ > 0 && xp && std::fabs((double)xg[0] - (double)xp[0]) > kEpsf;
   if (remesh) {
     std::vector<double> px = up(xp, np_), py = up(yp, np_), pz = up(dim == 3 ? zp : nullptr, np_), pg = up(gp, np_);
@@ -1363,7 +1459,6 @@ prbf_status prbf_rbf_interpolation_f32(const prbf_params* p, int64_t ng, const f
                          dim == 3 ? pz.data() : nullptr);
     if (!st) st = prbf_evaluate(c, 1, pg.data(), w.data());
   }
-  std::vector<double> g(ng);
   prbf_solve_info info{};
   if (!st) st = prbf_set_points(c, ng, gx.data(), gy.data(), dim == 3 ? gz.data() : nullptr, 0, nullptr, nullptr, nullptr);
   if (!st) st = prbf_solve(c, opts, w.data(), g.data(), &info, nullptr, 0);
@@ -1373,8 +1468,35 @@ prbf_status prbf_rbf_interpolation_f32(const prbf_params* p, int64_t ng, const f
     for (int64_t i = 0; i < ng; ++i) wg[i] = (float)g[i];
     if (its) *its = info.its;
   }
-  prbf_destroy(c);
   return st;
+}
+
+// RBFInterpolation (3d/wrapper.cxx:20-105): FP32 grid values in, grid
+// strengths out.  When the grid is not the particle set (first coordinates
+// differ by more than 1e-6, 3d/wrapper.cxx:93) the field of the particles
+// (strengths gp) is first evaluated at the grid; then the grid strengths
+// solving A~ g = w are written back over wg.  Host allocation failures return
+// PRBF_ERR_NOMEM; no exception leaves the call.
+prbf_status prbf_rbf_interpolation_f32(const prbf_params* p, int64_t ng, const float* xg, const float* yg,
+                                       const float* zg, float* wg, int64_t np_, const float* xp, const float* yp,
+                                       const float* zp, const float* gp, const prbf_solve_opts* opts, int* its) {
+  if (!p || ng <= 0 || !xg || !yg || !wg || (p->dim == 3 && !zg) || (np_ > 0 && xp && (!yp || !gp)) ||
+      (np_ > 0 && xp && p->dim == 3 && !zp)) {
+    g_free_err = "invalid arguments";
+    return PRBF_ERR_ARG;
+  }
+  try {
+    return rbf_interpolation_f32_impl(p, ng, xg, yg, zg, wg, np_, xp, yp, zp, gp, opts, its);
+  } catch (const std::bad_alloc&) {
+    g_free_err = "host allocation failed";
+    return PRBF_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    g_free_err = e.what();
+    return PRBF_ERR_ARG;
+  } catch (...) {
+    g_free_err = "unexpected error";
+    return PRBF_ERR_HIP;
+  }
 }
 
 // ------------------------------------------------------------ learned preconditioner
@@ -1675,10 +1797,8 @@ prbf_status prbf_learned_attach(prbf_ctx* c, prbf_mlp* net, double tau, int64_t*
     c->n_net_fallback = 0;
     c->lnorm.ensure(std::max<size_t>(1, list.size()));
     c->mlp_list.ensure(std::max<size_t>(1, list.size()));
-    c->skip.ensure(nc);
     if (!list.empty())
       PRBF_HIP(hipMemcpyAsync(c->mlp_list, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice, c->st));
-    PRBF_HIP(hipMemcpyAsync(c->skip, c->h_skip.data(), nc, hipMemcpyHostToDevice, c->st));
     sync(c);
     rasm_setup_impl(c, PRBF_PC_LEARNED);
     c->learned_ready = true;

@@ -1094,6 +1094,31 @@ void failed_blocks(hipStream_t st, long long nwork, const int* order, const int*
                      list, count);
 }
 
+// Dense untruncated block of one cluster, A[i][j] = G(x_bi - x_bj) over its
+// buffer in list order (2d/matmult.cxx:83-93), with the entry formula the FP64
+// factorisation uses; row-major, one entry per thread.
+template <int DIM>
+__global__ void k_dense_block(long long b0, int n, const int* __restrict__ bidx, const double* __restrict__ xs,
+                              const double* __restrict__ ys, const double* __restrict__ zs, double inv_two_s2,
+                              double norm, double* __restrict__ A) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)n * n) return;
+  const int i = (int)(e / n), j = (int)(e % n);
+  const int pi = bidx[b0 + i], pj = bidx[b0 + j];
+  A[e] = gauss_entry<double>(xs[pi] - xs[pj], ys[pi] - ys[pj], DIM == 3 ? zs[pi] - zs[pj] : 0.0, inv_two_s2, norm);
+}
+
+void dense_block(hipStream_t st, int dim, long long b0, int n, const int* bidx, const double* xs, const double* ys,
+                 const double* zs, double sigma, double* A) {
+  if (n <= 0) return;
+  const Kern k = make_kern(sigma);
+  const dim3 grid((unsigned)(((long long)n * n + 255) / 256));
+  if (dim == 3)
+    hipLaunchKernelGGL(k_dense_block<3>, grid, dim3(256), 0, st, b0, n, bidx, xs, ys, zs, k.inv_two_s2, k.norm, A);
+  else
+    hipLaunchKernelGGL(k_dense_block<2>, grid, dim3(256), 0, st, b0, n, bidx, xs, ys, zs, k.inv_two_s2, k.norm, A);
+}
+
 long long block_workspace(int n, int T) { return blk_ws(n, T); }
 
 bool block_chol_has_config(bool fp32, int nw, int npc, bool ll) {

@@ -60,6 +60,10 @@ __host__ __device__ inline long long apply_offset(int n, int T, int t, int b) {
   return apply_chunk_base(n, kc) + (long long)R * 64 * g + (long long)((tt / V) * ng + l) * V + tt % V;
 }
 
+// Dense untruncated block of the cluster whose buffer list starts at b0 (n
+// entries), row-major into A (n x n, device)
+void dense_block(hipStream_t st, int dim, long long b0, int n, const int* bidx, const double* xs, const double* ys,
+                 const double* zs, double sigma, double* A);
 // Workspace (elements) and LDS bytes of one block factorisation (ll: the
 // left-looking kernels).
 long long block_workspace(int n, int T);
@@ -82,12 +86,14 @@ void block_chol_fp32(hipStream_t st, int dim, int nw, int npc, bool ll, long lon
                      int nwg, long long ws_slot, int nmax, int tmax, const long long* bptr, const int* bidx,
                      const int* ista, const int* iend, const double* xs, const double* ys, const double* zs, float* W,
                      const long long* mptr, float* M, double sigma, int* blk_status);
-void rasm_apply_fp64(hipStream_t st, long long ncell, const long long* bptr, const int* bidx,
-                     const int* ista, const int* iend, const long long* mptr, const double* M,
-                     const double* r, double* z, int tmax);
-void rasm_apply_fp32(hipStream_t st, long long ncell, const long long* bptr, const int* bidx,
-                     const int* ista, const int* iend, const long long* mptr, const float* M,
-                     const unsigned char* skip, const double* r, double* z);
+// z[core(c)] = M_c r[buffer(c)] for the clusters list[0..nlist) (all clusters
+// 0..nlist-1 when list is null)
+void rasm_apply_fp64(hipStream_t st, long long nlist, const int* list, const long long* bptr, const int* bidx,
+                     const int* ista, const int* iend, const long long* mptr, const double* M, const double* r,
+                     double* z);
+void rasm_apply_fp32(hipStream_t st, long long nlist, const int* list, const long long* bptr, const int* bidx,
+                     const int* ista, const int* iend, const long long* mptr, const float* M, const double* r,
+                     double* z);
 
 // krylov.hip ------------------------------------------------------------------
 constexpr int kRedBlocks = 1024;  // fixed partition => run-to-run identical sums

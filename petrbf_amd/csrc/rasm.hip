@@ -25,8 +25,9 @@ namespace prbf {
 // partials are combined by a butterfly over the wavefront (reduce-scatter by
 // halves, then two full exchanges): a fixed order, so the result is identical
 // run to run.  MT = double (plain) or float (mixed; products and sums still
-// in FP64).  Clusters with skip[c] set are left to another part of the
-// preconditioner.  Lanes past the end of
+// in FP64).  Each launch covers a list of clusters, so that the mixed
+// preconditioner can serve its small blocks in FP64 and the learned one can
+// leave its network clusters out.  Lanes past the end of
 // the last group read a valid slot of the same group with r taken as 0, so
 // the loads need no branches.
 constexpr int kApplyWaves = 4;  // clusters per workgroup, one per wavefront
@@ -198,22 +199,21 @@ __device__ __forceinline__ void apply_cluster(const ET* __restrict__ Mc, int n, 
   }
 }
 
-// One launch over all clusters; clusters with skip[c] set are left out.  One
-// slot group in flight per pass keeps the kernels at 96 (FP32, five
-// wavefronts per SIMD) or 128 VGPRs (FP64, four): the apply streams each core
-// row once, and more resident clusters hide the gathers' latency better than
-// more loads in flight per cluster.
+// One launch over the clusters of `list` (list[0..nlist); all ncell clusters
+// in order when list is null).  One slot group in flight per pass keeps the
+// kernels at 96 (FP32, five wavefronts per SIMD) or 128 VGPRs (FP64, four):
+// the apply streams each core row once, and more resident clusters hide the
+// gathers' latency better than more loads in flight per cluster.
 template <typename MT>
 __global__ __launch_bounds__(64 * kApplyWaves, sizeof(MT) == 4 ? 5 : 4) void k_rasm_apply(
-    long long ncell, const long long* __restrict__ bptr, const int* __restrict__ bidx,
+    long long nlist, const int* __restrict__ list, const long long* __restrict__ bptr, const int* __restrict__ bidx,
     const int* __restrict__ ista, const int* __restrict__ iend, const long long* __restrict__ mptr,
-    const MT* __restrict__ M, const unsigned char* __restrict__ skip, const double* __restrict__ r,
-    double* __restrict__ z) {
+    const MT* __restrict__ M, const double* __restrict__ r, double* __restrict__ z) {
   __shared__ double rs[kApplyWaves][kMaxGroups * 64];
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  const long long c = (long long)xcd_block(blockIdx.x, gridDim.x) * kApplyWaves + wave;
-  if (c >= ncell) return;
-  if (skip && skip[c]) return;  // core written by another preconditioner part
+  const long long i = (long long)xcd_block(blockIdx.x, gridDim.x) * kApplyWaves + wave;
+  if (i >= nlist) return;
+  const long long c = list ? (long long)list[i] : i;
   const long long b0 = bptr[c];
   const int n = (int)(bptr[c + 1] - b0);
   if (n == 0) return;
@@ -222,21 +222,22 @@ __global__ __launch_bounds__(64 * kApplyWaves, sizeof(MT) == 4 ? 5 : 4) void k_r
   apply_cluster<MT, 1>(M + mptr[c], n, T, lane, bidx + b0, r, rs[wave], z + s);
 }
 
-static inline dim3 apply_grid(long long ncell) { return dim3((unsigned)((ncell + kApplyWaves - 1) / kApplyWaves)); }
+static inline dim3 apply_grid(long long n) { return dim3((unsigned)((n + kApplyWaves - 1) / kApplyWaves)); }
 
-void rasm_apply_fp64(hipStream_t st, long long ncell, const long long* bptr, const int* bidx,
-                     const int* ista, const int* iend, const long long* mptr, const double* M,
-                     const double* r, double* z, int tmax) {
-  (void)tmax;
-  hipLaunchKernelGGL((k_rasm_apply<double>), apply_grid(ncell), dim3(64 * kApplyWaves), 0, st, ncell, bptr, bidx,
-                     ista, iend, mptr, M, nullptr, r, z);
+void rasm_apply_fp64(hipStream_t st, long long nlist, const int* list, const long long* bptr, const int* bidx,
+                     const int* ista, const int* iend, const long long* mptr, const double* M, const double* r,
+                     double* z) {
+  if (nlist <= 0) return;
+  hipLaunchKernelGGL((k_rasm_apply<double>), apply_grid(nlist), dim3(64 * kApplyWaves), 0, st, nlist, list, bptr,
+                     bidx, ista, iend, mptr, M, r, z);
 }
 
-void rasm_apply_fp32(hipStream_t st, long long ncell, const long long* bptr, const int* bidx,
-                     const int* ista, const int* iend, const long long* mptr, const float* M,
-                     const unsigned char* skip, const double* r, double* z) {
-  hipLaunchKernelGGL((k_rasm_apply<float>), apply_grid(ncell), dim3(64 * kApplyWaves), 0, st, ncell, bptr, bidx,
-                     ista, iend, mptr, M, skip, r, z);
+void rasm_apply_fp32(hipStream_t st, long long nlist, const int* list, const long long* bptr, const int* bidx,
+                     const int* ista, const int* iend, const long long* mptr, const float* M, const double* r,
+                     double* z) {
+  if (nlist <= 0) return;
+  hipLaunchKernelGGL((k_rasm_apply<float>), apply_grid(nlist), dim3(64 * kApplyWaves), 0, st, nlist, list, bptr,
+                     bidx, ista, iend, mptr, M, r, z);
 }
 
 }  // namespace prbf

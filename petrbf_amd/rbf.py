@@ -251,6 +251,17 @@ class Context:
             self._check(self._L.prbf_rasm_get_core_rows(self._h, int(c), _p(M), C.byref(n), C.byref(t)))
         return M  # (T, n): rows of A_c^{-1} for the core members
 
+    def block(self, c):
+        """The dense untruncated RASM block of cluster c and its buffer
+        indices in subdomain order (the mysubmat block, 2d/matmult.cxx:53-104)."""
+        n = C.c_int()
+        self._check(self._L.prbf_rasm_get_block(self._h, int(c), None, None, C.byref(n)))
+        A = np.zeros((n.value, n.value))
+        idx = np.zeros(n.value, np.int32)
+        if n.value:
+            self._check(self._L.prbf_rasm_get_block(self._h, int(c), _p(A), _p(idx), C.byref(n)))
+        return A, idx
+
     # ---- solve
     def _opts(self, ksp, pc, rtol, restart, max_it, graph=True):
         o = L.PrbfSolveOpts()

@@ -14,6 +14,7 @@ from oracle.oracle import Oracle
 from petrbf_amd import problems
 from petrbf_amd.rbf import Context, rbf_interpolation, vorticity_evaluation
 from tests import _brute as B
+from tests._fp32ref import fp32_apply
 from tests.cases import all_cases, lattice
 
 pytestmark = pytest.mark.gpu
@@ -182,9 +183,8 @@ def test_large_blocks_both_orders():
 def test_3d_blocks_both_orders():
     """3D blocks of 124 to 730 unknowns (14^3 lattice, the 3D workload's cell
     and buffer widths): the FP64 preconditioner of both factorisation orders
-    agrees with the oracle's LU; the FP32 ones of both orders agree with each
-    other and stay close to FP64 (the 3D method does not reach rtol 1e-10 at
-    these widths, so no solve is checked)."""
+    agrees with the oracle's LU; the FP32 ones of both orders are as close to
+    FP64 as LAPACK's FP32 factorisation of the same blocks, within 10x."""
     _announce("3d blocks")
     p = problems.lattice_3d_problem(14, seed=31)
     o = Oracle.from_problem(p)
@@ -211,11 +211,17 @@ def test_3d_blocks_both_orders():
     z32["right"] = ctx.rasm_apply(r)
     ctx.close()
     rel = np.linalg.norm(z32["left"] - z32["right"]) / np.linalg.norm(z32["right"])
-    rel64 = np.linalg.norm(z32["left"] - want) / np.linalg.norm(want)
-    _announce(f"3d fp32 left vs right {rel:.2e}, vs fp64 {rel64:.2e}")
-    # measured: 7.6e-3 between the orders, 1.2e-2 from FP64 (FP32 rounding of
-    # blocks this ill-conditioned); a broken kernel is off by O(1)
-    assert rel < 5e-2 and rel64 < 0.5
+    _announce(f"3d fp32 left vs right {rel:.2e}")
+    x, y, z = o.sorted_coords()
+    bptr, bidx = o.buffer_csr()
+    ista, iend, _, _ = o.cells()
+    zref, _ = fp32_apply([x, y, z], bptr, bidx, ista, iend, p.sigma, r)
+    e_ref = np.linalg.norm(zref - want)
+    for order in ("left", "right"):
+        e = np.linalg.norm(z32[order] - want)
+        _announce(f"3d fp32 {order}: error {e / np.linalg.norm(want):.2e}, "
+                  f"LAPACK FP32 reference {e_ref / np.linalg.norm(want):.2e}")
+        assert e <= 10 * e_ref, (order, e, e_ref)
 
 
 @pytest.mark.parametrize("n_side", [100, 160])
@@ -330,62 +336,26 @@ def test_mixed_fp32_rasm_fgmres_reaches_fp64_residual():
     ctx.close()
 
 
-def test_mixed_core_rows_close_to_fp64():
-    _announce("fp32 apply vs fp64 apply")
-    p = lattice(40)
-    ctx = Context.from_problem(p)
-    ctx.set_points(p.x, p.y)
-    rng = np.random.default_rng(11)
-    r = rng.standard_normal(p.n)
-    ctx.rasm_setup("rasm")
-    z64 = ctx.rasm_apply(r)
-    ctx.rasm_setup("rasm32")
-    z32 = ctx.rasm_apply(r)
-    rel = np.linalg.norm(z32 - z64) / np.linalg.norm(z64)
-    _announce(f"fp32 vs fp64 RASM apply rel diff {rel:.2e}")
-    assert rel < 0.5
-    ctx.close()
-
-
-@pytest.mark.skip(reason="the only run of this case on the GPU printed nothing for 180 s and was stopped; "
-                         "no hang was found by reading the code, so the case stays off the GPU and the FP64 "
-                         "redo it exercises is switched off (an FP32 breakdown now fails the setup)")
-def test_fp32_breakdowns_are_redone_in_fp64():
-    """Blocks whose FP32 Cholesky breaks down are refactorised in FP64 and
-    applied by a launch over their list.  With sigma = h/0.5 numpy's FP32
-    Cholesky fails on all 36 blocks of this lattice and its FP64 one on none,
-    so the mixed preconditioner must equal the FP64 one (same launch
-    configuration forced for both, so the same bits); with sigma = h/0.55 it
-    fails on 12 of 36, and FGMRES on the mixed preconditioner still reaches
-    rtol 1e-10."""
-    _announce("fp32 breakdown fallback")
-    rng = np.random.default_rng(21)
+@pytest.mark.skip(reason="FP32 breakdown case: its only GPU run (with the FP64 redo that has since been removed) "
+                         "printed nothing for 180 s and was stopped; the cause has not been shown, so the case "
+                         "stays off the GPU")
+def test_fp32_breakdowns_fail_the_mixed_setup():
+    """With sigma = h/0.5 numpy's FP32 Cholesky fails on all 36 blocks of this
+    lattice: the mixed setup reports PRBF_ERR_FACTOR, no block is redone
+    (n_fallback stays 0), and the context is left not set up, so a later
+    FP64 setup still works."""
+    from petrbf_amd import _lib as L
+    _announce("fp32 breakdown reported")
     p = lattice(24, overlap=0.5)
     ctx = Context.from_problem(p)
     ctx.set_tuning(chol_waves=4, chol_panel=1, chol_order="left")
     ctx.set_points(p.x, p.y)
-    r = rng.standard_normal(p.n)
-    _, inf = ctx.solve(p.w[0], ksp="fgmres", pc="rasm32", rtol=1e-6, restart=30, max_it=5)
-    nblk = int((np.diff(ctx.buffer_csr()[0]) > 0).sum())
-    _announce(f"all-breakdown case: {inf.n_fallback} of {nblk} blocks redone in FP64")
-    assert inf.n_fallback == nblk
-    z32 = ctx.rasm_apply(r)
+    with pytest.raises(L.PrbfError) as e:
+        ctx.solve(p.w[0], ksp="fgmres", pc="rasm32", rtol=1e-6, restart=30, max_it=5)
+    assert e.value.code == L.PRBF_ERR_FACTOR
     ctx.rasm_setup("rasm")
-    z64 = ctx.rasm_apply(r)
-    np.testing.assert_array_equal(z32, z64)
-    ctx.close()
-    p = lattice(24, overlap=0.55)
-    ctx = Context.from_problem(p)
-    ctx.set_points(p.x, p.y)
-    g, inf = ctx.solve(p.w[0], ksp="fgmres", pc="rasm32", rtol=1e-10, restart=30, max_it=1000)
-    _announce(f"mixed case: {inf.n_fallback} blocks redone in FP64, its {inf.its}")
-    assert 0 < inf.n_fallback < nblk
-    assert inf.reason == 2
-    o = Oracle.from_problem(p)
-    pi, _ = o.perm()
-    w = np.empty(p.n)
-    w[pi] = o.matvec(g[pi])
-    assert np.linalg.norm(w - p.w[0]) <= 1e-9 * np.linalg.norm(p.w[0])
+    _, inf = ctx.solve(p.w[0], ksp="gmres", pc="rasm", rtol=1e-6, restart=30, max_it=5)
+    assert inf.n_fallback == 0
     ctx.close()
 
 
