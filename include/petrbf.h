@@ -203,6 +203,22 @@ prbf_status prbf_solve(prbf_ctx* ctx, const prbf_solve_opts* opts, const double*
 prbf_status prbf_solve_dev(prbf_ctx* ctx, const prbf_solve_opts* opts, const double* w, double* gamma,
                            prbf_solve_info* info, double* hist, int hist_cap);
 
+/* The same solve for nrhs = 1..3 right-hand sides at once (w and gamma hold
+ * them back to back, ni values each, original order; info receives nrhs
+ * entries; hist, optional, nrhs rows of hist_cap).  The reference solves the
+ * three components of the 3D vorticity field (3d/main.cxx:104) with one
+ * RBFInterpolation call each (3d/wrapper.cxx:93-101); here each right-hand
+ * side keeps its own Krylov basis, Givens rotations and convergence test (the
+ * results equal nrhs single solves run kernel by kernel, bit for bit), while
+ * every iteration runs one matvec over all of them (one exponential per
+ * pair) and one preconditioner apply that streams each block's core rows
+ * once.  Kernel by kernel whatever opts->graph says; not with
+ * PRBF_PC_LEARNED. */
+prbf_status prbf_solve_multi(prbf_ctx* ctx, const prbf_solve_opts* opts, int nrhs, const double* w, double* gamma,
+                             prbf_solve_info* info, double* hist, int hist_cap);
+prbf_status prbf_solve_multi_dev(prbf_ctx* ctx, const prbf_solve_opts* opts, int nrhs, const double* w,
+                                 double* gamma, prbf_solve_info* info, double* hist, int hist_cap);
+
 /* ---- learned preconditioner (2D; no counterpart in the reference) --------
  * A small MLP (192 -> 256 -> 256 -> 256 -> 16, GELU, BF16 weights and
  * activations on the BF16 matrix cores, FP32 accumulation) maps the residual

@@ -30,7 +30,7 @@ EXPORTS = [
     "prbf_get_sorted_points", "prbf_matvec", "prbf_matvec_dev", "prbf_evaluate", "prbf_evaluate_dev",
     "prbf_rasm_setup", "prbf_rasm_setup_pc", "prbf_set_tuning", "prbf_rasm_apply", "prbf_rasm_apply_dev", "prbf_rasm_get_core_rows",
     "prbf_rasm_get_block",
-    "prbf_solve", "prbf_solve_dev", "prbf_rbf_interpolation_f32",
+    "prbf_solve", "prbf_solve_dev", "prbf_solve_multi", "prbf_solve_multi_dev", "prbf_rbf_interpolation_f32",
     "prbf_mlp_num_params", "prbf_mlp_create", "prbf_mlp_destroy", "prbf_mlp_last_error", "prbf_mlp_set_params",
     "prbf_mlp_get_params", "prbf_mlp_forward_dev", "prbf_mlp_loss_grad_dev", "prbf_mlp_train_dev",
     "prbf_learned_eligible", "prbf_learned_pairs_dev", "prbf_learned_rho_dev", "prbf_learned_attach",
@@ -109,6 +109,8 @@ def load():
     L.prbf_rasm_get_block.argtypes = [ctx, i64, dp, dp, C.POINTER(i32)]
     for f in ("prbf_solve", "prbf_solve_dev"):
         getattr(L, f).argtypes = [ctx, C.POINTER(PrbfSolveOpts), dp, dp, C.POINTER(PrbfSolveInfo), dp, i32]
+    for f in ("prbf_solve_multi", "prbf_solve_multi_dev"):
+        getattr(L, f).argtypes = [ctx, C.POINTER(PrbfSolveOpts), i32, dp, dp, C.POINTER(PrbfSolveInfo), dp, i32]
     L.prbf_rbf_interpolation_f32.argtypes = [C.POINTER(PrbfParams), i64, dp, dp, dp, dp, i64, dp, dp, dp, dp,
                                              C.POINTER(PrbfSolveOpts), C.POINTER(i32)]
     net = C.c_void_p

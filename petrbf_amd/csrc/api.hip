@@ -701,14 +701,18 @@ void network_apply(prbf_ctx* c, const int* list, long long n, const double* r, d
   core_scatter(c->st, n, list, c->cidx, net->Y, c->ls, net->out_scale, z);
 }
 
-void rasm_apply_impl(prbf_ctx* c, const double* r, double* z) {
+// z = B r for nrhs vectors stored back to back (length ni each); every block's
+// core rows are streamed once for all of them.
+void rasm_apply_impl(prbf_ctx* c, const double* r, double* z, int nrhs = 1) {
   need(c->rasm_ready, PRBF_ERR_STATE, "rasm_setup first");
   need(r && z, PRBF_ERR_ARG, "null vector");
+  need(nrhs >= 1 && nrhs <= 3, PRBF_ERR_ARG, "nrhs must be 1..3");
+  need(nrhs == 1 || c->rasm_pc != PRBF_PC_LEARNED, PRBF_ERR_ARG, "the learned preconditioner takes one vector");
   if (c->rasm_pc == PRBF_PC_LEARNED) need(c->learned_ready && c->net, PRBF_ERR_STATE, "prbf_learned_attach first");
-  rasm_apply_fp32(c->st, c->n32, c->l32_all ? nullptr : c->l32.p, c->bptr, c->bidx, c->ista, c->iend, c->mptr, c->M32,
-                  r, z);
-  rasm_apply_fp64(c->st, c->n64, c->l64_all ? nullptr : c->l64.p, c->bptr, c->bidx, c->ista, c->iend, c->mptr64, c->M,
-                  r, z);
+  rasm_apply_fp32(c->st, nrhs, c->n32, c->l32_all ? nullptr : c->l32.p, c->bptr, c->bidx, c->ista, c->iend, c->mptr,
+                  c->M32, r, c->ni, z);
+  rasm_apply_fp64(c->st, nrhs, c->n64, c->l64_all ? nullptr : c->l64.p, c->bptr, c->bidx, c->ista, c->iend,
+                  c->mptr64, c->M, r, c->ni, z);
   if (c->rasm_pc == PRBF_PC_LEARNED) network_apply(c, c->mlp_list, c->n_mlp, r, z);
   PRBF_HIP(hipGetLastError());
 }
@@ -1058,14 +1062,228 @@ void krylov_impl(prbf_ctx* c, const prbf_solve_opts& o, const double* b, double*
   if (info) *info = inf;
 }
 
-void solve_dev_impl(prbf_ctx* c, const prbf_solve_opts* opts, const double* w, double* gamma,
-                    prbf_solve_info* info, double* hist, int hist_cap) {
+// Restarted GMRES (left PC) or FGMRES (right PC) on k = 1..3 right-hand
+// sides at once, sorted order, b and x holding them back to back.  Each
+// right-hand side keeps its own Krylov basis, Hessenberg matrix, Givens
+// rotations and convergence test, exactly as krylov_impl's kernel-by-kernel
+// path runs them (same kernels, same reduction order, host Givens), while
+// every step's matvec is one launch over the k vectors (one exponential per
+// pair for all of them) and every preconditioner apply streams each block's
+// core rows once.  The bases interleave the right-hand sides, V[j][r][n], so
+// the k vectors of one basis index are contiguous.  A right-hand side that
+// has converged stops taking part in the Gram-Schmidt steps; the cycles of all
+// of them restart together, as each would on its own.
+void krylov_batch_impl(prbf_ctx* c, const prbf_solve_opts& o, int k, const double* b, double* x,
+                       prbf_solve_info* info, double* const* hist, int hist_cap) {
+  const long long n = c->ni, kn = (long long)k * n;
+  const int m = o.restart;
+  need(k >= 1 && k <= 3, PRBF_ERR_ARG, "nrhs must be 1..3");
+  need(m >= 1 && m < 63, PRBF_ERR_ARG, "restart must be in 1..62");
+  need(o.pc == PRBF_PC_NONE || c->rasm_ready, PRBF_ERR_STATE, "rasm_setup first");
+  need(o.pc != PRBF_PC_LEARNED, PRBF_ERR_ARG, "the learned preconditioner takes one right-hand side");
+  const bool flex = (o.ksp == PRBF_KSP_FGMRES);
+  c->last_restart = 0;  // the interleaved bases are not single-vector bases
+  c->V.ensure((size_t)(m + 1) * kn);
+  if (flex) c->Z.ensure((size_t)m * kn);
+  c->v1.ensure(kn);
+  c->v2.ensure(kn);
+  c->part.ensure(kRedBlocks * 64);
+  c->scal.ensure(256 * 3);
+  double* V = c->V;
+  auto Vr = [&](int j, int r) { return V + (long long)j * kn + (long long)r * n; };
+  double* tmp = c->v1;
+  double* tmp2 = c->v2;
+  double* pin = c->pin;  // [r 65 + j] dots and norms of right-hand side r; [256 + 64 r + j] its y
+  auto dsc = [&](int r) { return c->scal.p + 256 * r; };
+  prbf_solve_info inf[3] = {};
+  // (n_matvec and n_pc count, per right-hand side, the operations it used)
+  auto A = [&](const double* v, double* y) { matvec_impl(c, k, v, y); };
+  auto B = [&](const double* v, double* y) {
+    if (o.pc == PRBF_PC_NONE)
+      PRBF_HIP(hipMemcpyAsync(y, v, sizeof(double) * kn, hipMemcpyDeviceToDevice, c->st));
+    else
+      rasm_apply_impl(c, v, y, k);
+  };
+  struct State {
+    std::vector<double> hh, cc, ss, rs, nrs;
+    Conv conv{0, 0, 0};
+    int its = 0, itcount = 0, reason = 0, nh = 0, loc = 0;
+    bool hapend = false, guess_zero = true;
+    double last_res = 0.0;
+  };
+  std::vector<State> S(k);
+  for (State& q : S) {
+    q.hh.assign((size_t)(m + 1) * (m + 1), 0.0);
+    q.cc.assign(m + 1, 0.0);
+    q.ss.assign(m + 1, 0.0);
+    q.rs.assign(m + 2, 0.0);
+    q.nrs.assign(m + 1, 0.0);
+    q.conv = Conv{o.rtol, o.abstol, o.dtol};
+  }
+  auto HH = [&](State& q, int i, int j) -> double& { return q.hh[(size_t)j * (m + 1) + i]; };
+  auto log_res = [&](int r, double v) {
+    State& q = S[r];
+    if (hist && hist[r] && q.nh < hist_cap) hist[r][q.nh] = v;
+    ++q.nh;
+    q.last_res = v;
+  };
+  const double t0 = now_s();
+  fill(c->st, kn, x, 0.0);
+  auto any_active = [&]() {
+    for (const State& q : S)
+      if (!q.reason) return true;
+    return false;
+  };
+  while (any_active()) {
+    // initial residuals of the cycle (the matvec covers all k; only the
+    // unfinished right-hand sides use it)
+    bool need_ax = false;
+    for (const State& q : S) need_ax |= !q.reason && !q.guess_zero;
+    if (need_ax) A(x, tmp2);
+    for (int r = 0; r < k; ++r) {
+      const State& q = S[r];
+      if (q.reason) continue;
+      if (!q.guess_zero) ++inf[r].n_matvec;
+      if (!flex && o.pc != PRBF_PC_NONE) ++inf[r].n_pc;
+      if (q.guess_zero)
+        PRBF_HIP(hipMemcpyAsync(tmp + r * n, b + r * n, sizeof(double) * n, hipMemcpyDeviceToDevice, c->st));
+      else
+        xpay_sub(c->st, n, b + r * n, tmp2 + r * n, tmp + r * n);
+    }
+    if (flex)
+      PRBF_HIP(hipMemcpyAsync(Vr(0, 0), tmp, sizeof(double) * kn, hipMemcpyDeviceToDevice, c->st));
+    else
+      B(tmp, Vr(0, 0));
+    for (int r = 0; r < k; ++r)
+      if (!S[r].reason) norm2sq(c->st, n, Vr(0, r), c->part, dsc(r) + 64);
+    for (int r = 0; r < k; ++r)
+      if (!S[r].reason) PRBF_HIP(hipMemcpyAsync(pin + 65 * r + 64, dsc(r) + 64, sizeof(double), hipMemcpyDeviceToHost,
+                                                c->st));
+    sync(c);
+    std::vector<bool> run(k, false);
+    for (int r = 0; r < k; ++r) {
+      State& q = S[r];
+      if (q.reason) continue;
+      const double res = std::sqrt(pin[65 * r + 64]);
+      if (res != 0.0) scale(c->st, n, Vr(0, r), 1.0 / res);
+      q.rs[0] = res;
+      log_res(r, res);
+      if (res == 0.0) {
+        q.reason = 3;
+        continue;
+      }
+      q.reason = q.conv(q.its, res);
+      q.loc = 0;
+      q.hapend = false;
+      run[r] = !q.reason && q.its < o.max_it;
+    }
+    // the cycle: steps while any right-hand side still runs it
+    for (int loc = 0; loc < m; ++loc) {
+      bool any = false;
+      for (int r = 0; r < k; ++r) any |= run[r];
+      if (!any) break;
+      if (flex) {
+        B(Vr(loc, 0), c->Z.p + (long long)loc * kn);
+        A(c->Z.p + (long long)loc * kn, Vr(loc + 1, 0));
+      } else {
+        A(Vr(loc, 0), tmp);
+        B(tmp, Vr(loc + 1, 0));
+      }
+      for (int r = 0; r < k; ++r) {
+        if (!run[r]) continue;
+        ++inf[r].n_matvec;
+        if (o.pc != PRBF_PC_NONE) ++inf[r].n_pc;
+        double* w = Vr(loc + 1, r);
+        mdot(c->st, n, w, Vr(0, r), kn, loc + 1, c->part, dsc(r));
+        maxpy_norm2(c->st, n, w, Vr(0, r), kn, loc + 1, dsc(r), -1.0, c->part, dsc(r) + 64);
+        PRBF_HIP(hipMemcpyAsync(pin + 65 * r, dsc(r), sizeof(double) * 65, hipMemcpyDeviceToHost, c->st));
+      }
+      sync(c);
+      for (int r = 0; r < k; ++r) {
+        if (!run[r]) continue;
+        State& q = S[r];
+        double* w = Vr(loc + 1, r);
+        const double* pr = pin + 65 * r;
+        for (int j = 0; j <= loc; ++j) HH(q, j, loc) = pr[j];
+        const double tt = std::sqrt(pr[64]);
+        if (tt != 0.0) scale(c->st, n, w, 1.0 / tt);
+        HH(q, loc + 1, loc) = tt;
+        const double hapbnd = std::min(std::fabs(tt / q.rs[loc]), 1e-30);
+        if (tt < hapbnd) q.hapend = true;
+        for (int j = 1; j <= loc; ++j) {
+          const double a0 = HH(q, j - 1, loc);
+          HH(q, j - 1, loc) = q.cc[j - 1] * a0 + q.ss[j - 1] * HH(q, j, loc);
+          HH(q, j, loc) = q.cc[j - 1] * HH(q, j, loc) - q.ss[j - 1] * a0;
+        }
+        double res;
+        if (!q.hapend) {
+          const double a0 = HH(q, loc, loc), a1 = HH(q, loc + 1, loc);
+          const double t = std::sqrt(a0 * a0 + a1 * a1);
+          if (t == 0.0) {
+            q.reason = -2;
+            run[r] = false;
+            continue;
+          }
+          q.cc[loc] = a0 / t;
+          q.ss[loc] = a1 / t;
+          q.rs[loc + 1] = -(q.ss[loc] * q.rs[loc]);
+          q.rs[loc] = q.cc[loc] * q.rs[loc];
+          HH(q, loc, loc) = q.cc[loc] * a0 + q.ss[loc] * a1;
+          res = std::fabs(q.rs[loc + 1]);
+        } else {
+          res = 0.0;
+        }
+        q.loc = loc + 1;
+        ++q.its;
+        log_res(r, res);
+        q.reason = q.conv(q.its, res);
+        if (q.hapend && !q.reason) q.reason = 8;
+        run[r] = !q.reason && q.its < o.max_it;
+      }
+    }
+    // solution updates, each with its own step count
+    for (int r = 0; r < k; ++r) {
+      State& q = S[r];
+      const int it = q.loc - 1;  // steps this cycle (finished right-hand sides: none)
+      if (it >= 0) {
+        q.nrs[it] = q.rs[it] / HH(q, it, it);
+        for (int ii = 1; ii <= it; ++ii) {
+          const int kk = it - ii;
+          double t = q.rs[kk];
+          for (int j = kk + 1; j <= it; ++j) t = t - HH(q, kk, j) * q.nrs[j];
+          q.nrs[kk] = t / HH(q, kk, kk);
+        }
+        for (int j = 0; j <= it; ++j) pin[256 + 64 * r + j] = q.nrs[j];
+        PRBF_HIP(hipMemcpyAsync(dsc(r) + 128, pin + 256 + 64 * r, sizeof(double) * (it + 1), hipMemcpyHostToDevice,
+                                c->st));
+        update_x(c->st, n, x + r * n, flex ? c->Z.p + r * n : Vr(0, r), kn, it + 1, dsc(r) + 128);
+      }
+      q.itcount += q.loc;
+      q.loc = 0;
+      if (!q.reason && q.itcount >= o.max_it) q.reason = -3;
+      q.guess_zero = false;
+    }
+    sync(c);  // pinned y consumed before the next cycle reuses it
+  }
+  sync(c);
+  for (int r = 0; r < k; ++r) {
+    inf[r].its = S[r].its;
+    inf[r].reason = S[r].reason;
+    inf[r].rnorm0 = S[r].conv.rnorm0;
+    inf[r].rnorm = S[r].last_res;
+    inf[r].t_solve_s = now_s() - t0;
+    if (info) info[r] = inf[r];
+  }
+}
+
+// The solver options (defaults when opts is null), validated; the
+// preconditioner set up when it is not (its wall time in *t_setup).
+prbf_solve_opts checked_opts(prbf_ctx* c, const prbf_solve_opts* opts, int hist_cap, double* t_setup) {
   need(c->has_points && c->same, PRBF_ERR_STATE, "set_points (one set) first");
   prbf_solve_opts o;
   prbf_solve_opts_default(&o);
   if (opts) o = *opts;
-  double t_setup = 0.0;
-  need(w && gamma, PRBF_ERR_ARG, "null vector");
+  *t_setup = 0.0;
   need(o.ksp == PRBF_KSP_GMRES || o.ksp == PRBF_KSP_FGMRES, PRBF_ERR_ARG, "unknown Krylov method");
   need(o.pc == PRBF_PC_NONE || o.pc == PRBF_PC_RASM_FP64 || o.pc == PRBF_PC_RASM_FP32 || o.pc == PRBF_PC_LEARNED,
        PRBF_ERR_ARG, "unknown preconditioner");
@@ -1076,8 +1294,40 @@ void solve_dev_impl(prbf_ctx* c, const prbf_solve_opts* opts, const double* w, d
   if (o.pc != PRBF_PC_NONE && (!c->rasm_ready || c->rasm_pc != o.pc)) {
     double t0 = now_s();
     rasm_setup_impl(c, o.pc);
-    t_setup = now_s() - t0;
+    *t_setup = now_s() - t0;
   }
+  return o;
+}
+
+void solve_multi_dev_impl(prbf_ctx* c, const prbf_solve_opts* opts, int k, const double* w, double* gamma,
+                          prbf_solve_info* info, double* hist, int hist_cap) {
+  need(w && gamma, PRBF_ERR_ARG, "null vector");
+  need(k >= 1 && k <= 3, PRBF_ERR_ARG, "nrhs must be 1..3");
+  double t_setup = 0.0;
+  const prbf_solve_opts o = checked_opts(c, opts, hist_cap, &t_setup);
+  need(o.pc != PRBF_PC_LEARNED, PRBF_ERR_ARG, "the learned preconditioner takes one right-hand side");
+  const long long n = c->ni;
+  c->v3.ensure((size_t)2 * k * n);
+  double* bs = c->v3;
+  double* xs = c->v3.p + (long long)k * n;
+  for (int r = 0; r < k; ++r) gather(c->st, n, c->perm_i, w + r * n, bs + r * n);
+  double* hp[3] = {nullptr, nullptr, nullptr};
+  for (int r = 0; r < k && hist; ++r) hp[r] = hist + (long long)r * hist_cap;
+  prbf_solve_info local[3] = {};
+  krylov_batch_impl(c, o, k, bs, xs, local, hp, hist_cap);
+  for (int r = 0; r < k; ++r) scatter(c->st, n, c->perm_i, xs + r * n, gamma + r * n);
+  for (int r = 0; r < k; ++r) {
+    local[r].t_setup_s = t_setup;
+    local[r].n_fallback = c->n_fallback;
+    if (info) info[r] = local[r];
+  }
+}
+
+void solve_dev_impl(prbf_ctx* c, const prbf_solve_opts* opts, const double* w, double* gamma,
+                    prbf_solve_info* info, double* hist, int hist_cap) {
+  need(w && gamma, PRBF_ERR_ARG, "null vector");
+  double t_setup = 0.0;
+  const prbf_solve_opts o = checked_opts(c, opts, hist_cap, &t_setup);
   const long long n = c->ni;
   c->v3.ensure(2 * n);
   double* bs = c->v3;
@@ -1413,6 +1663,27 @@ prbf_status prbf_rasm_get_block(prbf_ctx* c, int64_t k, double* A, int32_t* idx,
 prbf_status prbf_solve_dev(prbf_ctx* c, const prbf_solve_opts* opts, const double* w, double* gamma,
                            prbf_solve_info* info, double* hist, int hist_cap) {
   return guarded(c, [&] { solve_dev_impl(c, opts, w, gamma, info, hist, hist_cap); });
+}
+
+prbf_status prbf_solve_multi_dev(prbf_ctx* c, const prbf_solve_opts* opts, int nrhs, const double* w, double* gamma,
+                                 prbf_solve_info* info, double* hist, int hist_cap) {
+  return guarded(c, [&] { solve_multi_dev_impl(c, opts, nrhs, w, gamma, info, hist, hist_cap); });
+}
+
+prbf_status prbf_solve_multi(prbf_ctx* c, const prbf_solve_opts* opts, int nrhs, const double* w, double* gamma,
+                             prbf_solve_info* info, double* hist, int hist_cap) {
+  return guarded(c, [&] {
+    need(c->has_points && c->same, PRBF_ERR_STATE, "set_points (one set) first");
+    need(w && gamma, PRBF_ERR_ARG, "null vector");
+    need(nrhs >= 1 && nrhs <= 3, PRBF_ERR_ARG, "nrhs must be 1..3");
+    DBuf<double> dw, dg;
+    dw.ensure((size_t)nrhs * c->ni);
+    dg.ensure((size_t)nrhs * c->ni);
+    PRBF_HIP(hipMemcpyAsync(dw, w, sizeof(double) * nrhs * c->ni, hipMemcpyHostToDevice, c->st));
+    solve_multi_dev_impl(c, opts, nrhs, dw, dg, info, hist, hist_cap);
+    PRBF_HIP(hipMemcpyAsync(gamma, dg, sizeof(double) * nrhs * c->ni, hipMemcpyDeviceToHost, c->st));
+    sync(c);
+  });
 }
 
 prbf_status prbf_solve(prbf_ctx* c, const prbf_solve_opts* opts, const double* w, double* gamma,

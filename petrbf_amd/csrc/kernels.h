@@ -87,13 +87,13 @@ void block_chol_fp32(hipStream_t st, int dim, int nw, int npc, bool ll, long lon
                      const int* ista, const int* iend, const double* xs, const double* ys, const double* zs, float* W,
                      const long long* mptr, float* M, double sigma, int* blk_status);
 // z[core(c)] = M_c r[buffer(c)] for the clusters list[0..nlist) (all clusters
-// 0..nlist-1 when list is null)
-void rasm_apply_fp64(hipStream_t st, long long nlist, const int* list, const long long* bptr, const int* bidx,
-                     const int* ista, const int* iend, const long long* mptr, const double* M, const double* r,
-                     double* z);
-void rasm_apply_fp32(hipStream_t st, long long nlist, const int* list, const long long* bptr, const int* bidx,
-                     const int* ista, const int* iend, const long long* mptr, const float* M, const double* r,
-                     double* z);
+// 0..nlist-1 when list is null), for nrhs (1..3) vectors stored at stride ld
+void rasm_apply_fp64(hipStream_t st, int nrhs, long long nlist, const int* list, const long long* bptr,
+                     const int* bidx, const int* ista, const int* iend, const long long* mptr, const double* M,
+                     const double* r, long long ld, double* z);
+void rasm_apply_fp32(hipStream_t st, int nrhs, long long nlist, const int* list, const long long* bptr,
+                     const int* bidx, const int* ista, const int* iend, const long long* mptr, const float* M,
+                     const double* r, long long ld, double* z);
 
 // krylov.hip ------------------------------------------------------------------
 constexpr int kRedBlocks = 1024;  // fixed partition => run-to-run identical sums

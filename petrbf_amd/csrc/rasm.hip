@@ -44,12 +44,14 @@ struct Vec16<double> {
 };
 
 // Residuals at this lane's buffer slots (one per slot group, up to kMaxGroups
-// groups, i.e. n <= 64 kMaxGroups) into the wavefront's LDS slice rs[g][lane]:
-// gathered once per cluster, before the core rows stream in, so the index ->
-// residual dependency is paid once.
+// groups, i.e. n <= 64 kMaxGroups) into the wavefront's LDS slice
+// rs[(v kMaxGroups + g) 64 + lane] for each of the NR vectors v (stride ldr
+// in r): gathered once per cluster, before the core rows stream in, so the
+// index -> residual dependency is paid once.
 constexpr int kMaxGroups = 8;
+template <int NR>
 __device__ __forceinline__ void gather_slots(int n, int lane, const int* __restrict__ bi, const double* __restrict__ r,
-                                             double* __restrict__ rs) {
+                                             long long ldr, double* __restrict__ rs) {
   const int ngrp = (n + 63) / 64;
   int idx[kMaxGroups];
 #pragma unroll
@@ -59,22 +61,26 @@ __device__ __forceinline__ void gather_slots(int n, int lane, const int* __restr
     idx[g] = bi[64 * gg + min(lane, ng - 1)];
   }
 #pragma unroll
-  for (int g = 0; g < kMaxGroups; ++g) {
-    const int gg = g < ngrp ? g : ngrp - 1;
-    const int ng = min(64, n - 64 * gg);
-    const double v = r[idx[g]];
-    rs[g * 64 + lane] = (g < ngrp && lane < ng) ? v : 0.0;
+  for (int v = 0; v < NR; ++v) {
+#pragma unroll
+    for (int g = 0; g < kMaxGroups; ++g) {
+      const int gg = g < ngrp ? g : ngrp - 1;
+      const int ng = min(64, n - 64 * gg);
+      const double x = r[v * ldr + idx[g]];
+      rs[(v * kMaxGroups + g) * 64 + lane] = (g < ngrp && lane < ng) ? x : 0.0;
+    }
   }
 }
 
-// acc[t] += sum over this lane's slots of M[t][b] r[b], the R stored rows of
-// one chunk (rows R..15 of acc stay zero); KG slot groups per pass (all loads
-// of a pass are issued before the FMAs), residuals from the LDS slice.  The
-// core rows are read once per apply: non-temporal loads keep them from
-// displacing the residuals that neighbouring clusters gather.
-template <typename ET, int KG>
+// acc[v][t] += sum over this lane's slots of M[t][b] r_v[b], the R stored rows
+// of one chunk (rows R..15 of acc stay zero); KG slot groups per pass (all
+// loads of a pass are issued before the FMAs), residuals from the LDS slice.
+// The core rows are read once per apply, for all NR vectors: non-temporal
+// loads keep them from displacing the residuals that neighbouring clusters
+// gather.
+template <typename ET, int KG, int NR>
 __device__ __forceinline__ void chunk_times(const ET* __restrict__ Mk, int n, int R, int lane,
-                                            const double* __restrict__ rs, double (&acc)[kRowChunk]) {
+                                            const double* __restrict__ rs, double (&acc)[NR][kRowChunk]) {
   constexpr int V = 16 / (int)sizeof(ET), QN = kRowChunk / V;
   using VT = typename Vec16<ET>::type;
   const int ngrp = (n + 63) / 64, qn = R / V;
@@ -90,11 +96,14 @@ __device__ __forceinline__ void chunk_times(const ET* __restrict__ Mk, int n, in
     }
 #pragma unroll
     for (int k = 0; k < KG; ++k) {
-      const double rb = g0 + k < ngrp ? rs[(g0 + k) * 64 + lane] : 0.0;
 #pragma unroll
-      for (int q = 0; q < QN; ++q) {
+      for (int w = 0; w < NR; ++w) {
+        const double rb = g0 + k < ngrp ? rs[(w * kMaxGroups + g0 + k) * 64 + lane] : 0.0;
 #pragma unroll
-        for (int v = 0; v < V; ++v) acc[q * V + v] = fma((double)m[k][q][v], rb, acc[q * V + v]);
+        for (int q = 0; q < QN; ++q) {
+#pragma unroll
+          for (int v = 0; v < V; ++v) acc[w][q * V + v] = fma((double)m[k][q][v], rb, acc[w][q * V + v]);
+        }
       }
     }
   }
@@ -102,10 +111,10 @@ __device__ __forceinline__ void chunk_times(const ET* __restrict__ Mk, int n, in
 
 // The same for blocks of more than 64 kMaxGroups slots: indices and residuals
 // gathered pass by pass, KG slot groups per pass
-template <typename ET, int KG>
+template <typename ET, int KG, int NR>
 __device__ __forceinline__ void chunk_times_gather(const ET* __restrict__ Mk, int n, int R, int lane,
                                                    const int* __restrict__ bi, const double* __restrict__ r,
-                                                   double (&acc)[kRowChunk]) {
+                                                   long long ldr, double (&acc)[NR][kRowChunk]) {
   constexpr int V = 16 / (int)sizeof(ET), QN = kRowChunk / V;
   using VT = typename Vec16<ET>::type;
   const int ngrp = (n + 63) / 64, qn = R / V;
@@ -124,19 +133,25 @@ __device__ __forceinline__ void chunk_times_gather(const ET* __restrict__ Mk, in
 #pragma unroll
       for (int q = 0; q < QN; ++q) m[k][q] = q < qn ? p[q * ng] : VT{};
     }
-    double rb[KG];
+    double rb[KG][NR];
 #pragma unroll
     for (int k = 0; k < KG; ++k) {
-      const double v = r[idx[k]];
-      rb[k] = ok[k] ? v : 0.0;
+#pragma unroll
+      for (int w = 0; w < NR; ++w) {
+        const double x = r[w * ldr + idx[k]];
+        rb[k][w] = ok[k] ? x : 0.0;
+      }
     }
 #pragma unroll
     for (int k = 0; k < KG; ++k) {
 #pragma unroll
-      for (int q = 0; q < QN; ++q) {
-        const ET* e = reinterpret_cast<const ET*>(&m[k][q]);
+      for (int w = 0; w < NR; ++w) {
 #pragma unroll
-        for (int v = 0; v < V; ++v) acc[q * V + v] = fma((double)e[v], rb[k], acc[q * V + v]);
+        for (int q = 0; q < QN; ++q) {
+          const ET* e = reinterpret_cast<const ET*>(&m[k][q]);
+#pragma unroll
+          for (int v = 0; v < V; ++v) acc[w][q * V + v] = fma((double)e[v], rb[k][w], acc[w][q * V + v]);
+        }
       }
     }
   }
@@ -177,39 +192,47 @@ __device__ inline double wave_rows_sum(const double (&acc)[kRowChunk], int lane)
   return v;
 }
 
-template <typename ET, int KG>
+template <typename ET, int KG, int NR>
 __device__ __forceinline__ void apply_cluster(const ET* __restrict__ Mc, int n, int T, int lane, const int* __restrict__ bi,
-                                              const double* __restrict__ r, double* __restrict__ rs,
+                                              const double* __restrict__ r, long long ld, double* __restrict__ rs,
                                               double* __restrict__ zc) {
   const int trow = 8 * ((lane >> 5) & 1) + 4 * ((lane >> 4) & 1) + 2 * ((lane >> 3) & 1) + ((lane >> 2) & 1);
   const bool small = n <= 64 * kMaxGroups;
-  if (small) gather_slots(n, lane, bi, r, rs);
+  if (small) gather_slots<NR>(n, lane, bi, r, ld, rs);
   for (int t0 = 0; t0 < T; t0 += kRowChunk) {
-    double acc[kRowChunk];
+    double acc[NR][kRowChunk];
 #pragma unroll
-    for (int t = 0; t < kRowChunk; ++t) acc[t] = 0.0;
+    for (int w = 0; w < NR; ++w)
+#pragma unroll
+      for (int t = 0; t < kRowChunk; ++t) acc[w][t] = 0.0;
     const int kc = t0 / kRowChunk, R = apply_chunk_rows(T, kc);
     const ET* Mk = Mc + apply_chunk_base(n, kc);
     if (small)
-      chunk_times<ET, KG>(Mk, n, R, lane, rs, acc);
+      chunk_times<ET, KG, NR>(Mk, n, R, lane, rs, acc);
     else
-      chunk_times_gather<ET, KG>(Mk, n, R, lane, bi, r, acc);
-    const double s = wave_rows_sum(acc, lane);
-    if ((lane & 3) == 0 && t0 + trow < T) zc[t0 + trow] = s;
+      chunk_times_gather<ET, KG, NR>(Mk, n, R, lane, bi, r, ld, acc);
+#pragma unroll
+    for (int w = 0; w < NR; ++w) {
+      const double s = wave_rows_sum(acc[w], lane);
+      if ((lane & 3) == 0 && t0 + trow < T) zc[w * ld + t0 + trow] = s;
+    }
   }
 }
 
 // One launch over the clusters of `list` (list[0..nlist); all ncell clusters
-// in order when list is null).  One slot group in flight per pass keeps the
+// in order when list is null), for NR vectors at once (r and z hold them at
+// stride ld).  One slot group in flight per pass keeps the single-vector
 // kernels at 96 (FP32, five wavefronts per SIMD) or 128 VGPRs (FP64, four):
 // the apply streams each core row once, and more resident clusters hide the
-// gathers' latency better than more loads in flight per cluster.
-template <typename MT>
-__global__ __launch_bounds__(64 * kApplyWaves, sizeof(MT) == 4 ? 5 : 4) void k_rasm_apply(
+// gathers' latency better than more loads in flight per cluster.  Each
+// vector's sums run in the same order whatever NR is, so a vector's result
+// does not depend on the vectors applied beside it.
+template <typename MT, int NR>
+__global__ __launch_bounds__(64 * kApplyWaves, NR > 1 ? 2 : (sizeof(MT) == 4 ? 5 : 4)) void k_rasm_apply(
     long long nlist, const int* __restrict__ list, const long long* __restrict__ bptr, const int* __restrict__ bidx,
     const int* __restrict__ ista, const int* __restrict__ iend, const long long* __restrict__ mptr,
-    const MT* __restrict__ M, const double* __restrict__ r, double* __restrict__ z) {
-  __shared__ double rs[kApplyWaves][kMaxGroups * 64];
+    const MT* __restrict__ M, const double* __restrict__ r, long long ld, double* __restrict__ z) {
+  __shared__ double rs[kApplyWaves][NR * kMaxGroups * 64];
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const long long i = (long long)xcd_block(blockIdx.x, gridDim.x) * kApplyWaves + wave;
   if (i >= nlist) return;
@@ -219,25 +242,35 @@ __global__ __launch_bounds__(64 * kApplyWaves, sizeof(MT) == 4 ? 5 : 4) void k_r
   if (n == 0) return;
   const int s = ista[c];
   const int T = iend[c] - s + 1;
-  apply_cluster<MT, 1>(M + mptr[c], n, T, lane, bidx + b0, r, rs[wave], z + s);
+  apply_cluster<MT, 1, NR>(M + mptr[c], n, T, lane, bidx + b0, r, ld, rs[wave], z + s);
 }
 
 static inline dim3 apply_grid(long long n) { return dim3((unsigned)((n + kApplyWaves - 1) / kApplyWaves)); }
 
-void rasm_apply_fp64(hipStream_t st, long long nlist, const int* list, const long long* bptr, const int* bidx,
-                     const int* ista, const int* iend, const long long* mptr, const double* M, const double* r,
-                     double* z) {
+template <typename MT>
+static void launch_apply(hipStream_t st, int nrhs, long long nlist, const int* list, const long long* bptr,
+                         const int* bidx, const int* ista, const int* iend, const long long* mptr, const MT* M,
+                         const double* r, long long ld, double* z) {
   if (nlist <= 0) return;
-  hipLaunchKernelGGL((k_rasm_apply<double>), apply_grid(nlist), dim3(64 * kApplyWaves), 0, st, nlist, list, bptr,
-                     bidx, ista, iend, mptr, M, r, z);
+#define PRBF_AP(NR)                                                                                              \
+  hipLaunchKernelGGL((k_rasm_apply<MT, NR>), apply_grid(nlist), dim3(64 * kApplyWaves), 0, st, nlist, list, bptr, \
+                     bidx, ista, iend, mptr, M, r, ld, z)
+  if (nrhs == 1) PRBF_AP(1);
+  else if (nrhs == 2) PRBF_AP(2);
+  else PRBF_AP(3);
+#undef PRBF_AP
 }
 
-void rasm_apply_fp32(hipStream_t st, long long nlist, const int* list, const long long* bptr, const int* bidx,
-                     const int* ista, const int* iend, const long long* mptr, const float* M, const double* r,
-                     double* z) {
-  if (nlist <= 0) return;
-  hipLaunchKernelGGL((k_rasm_apply<float>), apply_grid(nlist), dim3(64 * kApplyWaves), 0, st, nlist, list, bptr,
-                     bidx, ista, iend, mptr, M, r, z);
+void rasm_apply_fp64(hipStream_t st, int nrhs, long long nlist, const int* list, const long long* bptr,
+                     const int* bidx, const int* ista, const int* iend, const long long* mptr, const double* M,
+                     const double* r, long long ld, double* z) {
+  launch_apply<double>(st, nrhs, nlist, list, bptr, bidx, ista, iend, mptr, M, r, ld, z);
+}
+
+void rasm_apply_fp32(hipStream_t st, int nrhs, long long nlist, const int* list, const long long* bptr,
+                     const int* bidx, const int* ista, const int* iend, const long long* mptr, const float* M,
+                     const double* r, long long ld, double* z) {
+  launch_apply<float>(st, nrhs, nlist, list, bptr, bidx, ista, iend, mptr, M, r, ld, z);
 }
 
 }  // namespace prbf

@@ -302,6 +302,30 @@ class Context:
         info.hist = info.hist[: self._hist_len(info)]
         return info
 
+    def solve_multi(self, w, ksp="fgmres", pc="rasm32", rtol=1e-10, restart=30, max_it=1000):
+        """Strengths for k = 1..3 right-hand sides w (k, n) in one batched
+        Krylov solve; returns gamma (k, n) and one SolveInfo per right-hand side."""
+        o = self._opts(ksp, pc, rtol, restart, max_it, graph=False)
+        w = np.ascontiguousarray(np.atleast_2d(w), dtype=np.float64)
+        k = w.shape[0]
+        g = np.zeros_like(w)
+        cap = max_it + 64
+        hist = np.zeros((k, cap))
+        inf = (L.PrbfSolveInfo * k)()
+        self._check(self._L.prbf_solve_multi(self._h, C.byref(o), k, _p(w), _p(g), inf, _p(hist), cap))
+        infos = []
+        for r in range(k):
+            info = self._info(inf[r], hist[r])
+            info.hist = info.hist[: self._hist_len(info)]
+            infos.append(info)
+        return g, infos
+
+    def solve_multi_dev(self, w, gamma, nrhs, ksp="fgmres", pc="rasm32", rtol=1e-10, restart=30, max_it=1000):
+        o = self._opts(ksp, pc, rtol, restart, max_it, graph=False)
+        inf = (L.PrbfSolveInfo * nrhs)()
+        self._check(self._L.prbf_solve_multi_dev(self._h, C.byref(o), int(nrhs), _p(w), _p(gamma), inf, None, 0))
+        return [self._info(inf[r], np.zeros(0)) for r in range(nrhs)]
+
     @staticmethod
     def _hist_len(info):
         # one entry per cycle start plus one per iteration; trailing zeros are unused slots
