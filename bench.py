@@ -105,9 +105,12 @@ def main():
     ap.add_argument("--chol-panel", type=int, default=0, help="1, 2 or 4 tile columns per panel")
     ap.add_argument("--chol-order", choices=["right", "left"], default=None,
                     help="right- or left-looking block factorisation")
-    ap.add_argument("--workload", choices=["c2", "c5"], default="c2",
-                    help="c2: the 2D N=4M solve (the headline); c5: the 3D 160^3 three-RHS workload "
-                         "(lists, batched matvec, FP32 setup, FGMRES progress), its own JSON line")
+    ap.add_argument("--workload", choices=["c2", "c3", "c5"], default="c2",
+                    help="c2: the 2D N=4M solve (the headline); c3: the 2D N=4M non-uniform mixture on the "
+                         "adaptive cluster structure; c5: the 3D 160^3 three-RHS workload; c3 and c5 print "
+                         "their own JSON line")
+    ap.add_argument("--c3-its", type=int, default=60, help="FGMRES/GMRES iteration budget (c3)")
+    ap.add_argument("--c3-cap", type=int, default=12, help="particles per leaf before a cell splits (c3)")
     ap.add_argument("--c5-its", type=int, default=150, help="FGMRES iteration budget per component (c5)")
     ap.add_argument("--setup-only", choices=["rasm", "rasm32"], default=None,
                     help="only build the workload's lists and run one RASM setup (for counter runs)")
@@ -134,6 +137,8 @@ def main():
     dev = torch.device("cuda:0")
     if args.workload == "c5":
         return bench_c5(args, torch, problems, Context, dev)
+    if args.workload == "c3":
+        return bench_c3(args, torch, problems, Context, dev)
     log(f"generating lattice {args.n_side}^2")
     p = problems.lattice_2d_problem(args.n_side, seed=0x5EED0001)
     x = torch.from_numpy(p.x).to(dev)
@@ -176,7 +181,7 @@ def main():
         for i in range(args.warmup):
             inf = step()
             log(f"{mode} warmup {i}: its {inf.its} reason {inf.reason} setup {inf.t_setup_s:.3f}s "
-                f"solve {inf.t_solve_s:.3f}s fallback {inf.n_fallback}")
+                f"solve {inf.t_solve_s:.3f}s regularised blocks {inf.n_regularised}")
         torch.cuda.synchronize()
         infos = []
         t0 = time.perf_counter()
@@ -200,7 +205,7 @@ def main():
                              chol_order="left" if ci["chol_order"] == 2 else "right",
                              t_setup_s=inf.t_setup_s, t_solve_s=inf.t_solve_s, t_matvec_s=inf.t_matvec_s,
                              n_matvec=inf.n_matvec, t_pc_s=inf.t_pc_s, n_pc=inf.n_pc,
-                             n_fallback=inf.n_fallback, true_rel_residual=true_rel)
+                             n_regularised=inf.n_regularised, true_rel_residual=true_rel)
 
     # ---- per-kernel timing with HIP events on the library's stream (FP64 RASM set up last if plain ran)
     ctx.set_points_dev(x, y)
@@ -299,6 +304,77 @@ def main():
         res["cpu_baseline"] = cpu_baseline(p, inf_its, pairs)
         log(f"cpu baseline est {res['cpu_baseline']['value']:.1f}s on {res['cpu_baseline']['cores']} threads")
     print(json.dumps(res), flush=True)
+
+
+def bench_c3(args, torch, problems, Context, dev):
+    """2D non-uniform mixture, N = 4*10^6 (70% uniform on [-1,1]^2, 30% in
+    two Gaussians at (+-0.4, 0), std 0.15), sigma from the mean spacing, 5
+    sigma cutoff, on the adaptive cluster structure (grid cells of more than
+    cap particles split down to depth 8; buffer 8 mean spacings per side at
+    depth 0, halved per level).  One step = adaptive lists + RASM setup +
+    Krylov solve within an iteration budget, mixed (FP32 blocks of >= 32
+    unknowns, FGMRES) and plain (FP64 blocks, GMRES).  The points cluster
+    (nearest neighbours down to ~1% of the mean spacing), so blocks and the
+    whole system are numerically singular in FP64; the line reports how far
+    each solve gets, with the true residual."""
+    n = 4_000_000
+    log(f"generating mixture N={n}")
+    p = problems.mixture_2d_problem(n)
+    x, y, w = (torch.from_numpy(a).to(dev) for a in (p.x, p.y, p.w[0]))
+    gamma = torch.empty_like(w)
+    wv = torch.empty_like(w)
+    stream = torch.cuda.Stream(device=dev)
+    ctx = Context.from_problem(p)
+    ctx.set_stream(stream.cuda_stream)
+    cap, depth = args.c3_cap, 8
+    ctx.set_points_adaptive_dev(x, y, cap=cap, max_depth=depth)
+    torch.cuda.synchronize()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.set_points_adaptive_dev(x, y, cap=cap, max_depth=depth)
+    torch.cuda.synchronize()
+    t_lists = (time.perf_counter() - t0) / reps
+    info = ctx.info()
+    lv = ctx.leaves()
+    nb = np.diff(ctx.buffer_ptr())
+    ista, iend, _, _ = ctx.cells()
+    T = iend - ista + 1
+    ne = nb > 0
+    stats = {"leaves": info["ncell"], "nonempty_leaves": int(ne.sum()),
+             "depth_histogram": np.bincount(lv["depth"]).tolist(),
+             "block_unknowns_min_mean_max": [int(nb[ne].min()), float(nb[ne].mean()), int(nb.max())],
+             "blocks_under_32": int(((nb > 0) & (nb < 32)).sum()), "core_max": int(T.max()),
+             "trunc_mean": float(np.diff(ctx.trunc_ptr())[ne].mean())}
+    log(f"adaptive structure {t_lists*1e3:.1f} ms: {stats}")
+    out = {}
+    for mode, ksp, pc in (("mixed", "fgmres", "rasm32"), ("plain", "gmres", "rasm")):
+        def step():
+            ctx.set_points_adaptive_dev(x, y, cap=cap, max_depth=depth)
+            return ctx.solve_dev(w, gamma, ksp=ksp, pc=pc, rtol=1e-10, restart=30, max_it=args.c3_its)
+        inf = step()  # warm-up (tuner, graphs)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        inf = step()
+        torch.cuda.synchronize()
+        t_step = time.perf_counter() - t0
+        ctx.evaluate_dev(gamma, wv)
+        torch.cuda.synchronize()
+        true_rel = float(torch.linalg.norm(wv - w) / torch.linalg.norm(w))
+        ci = ctx.info()
+        out[mode] = {"step_s": t_step, "its": inf.its, "reason": inf.reason, "t_setup_s": inf.t_setup_s,
+                     "t_solve_s": inf.t_solve_s, "residual_estimate": inf.rnorm / inf.rnorm0,
+                     "true_rel_residual": true_rel, "regularised_blocks": inf.n_regularised,
+                     "chol_kernel_s": ci["chol_kernel_s"],
+                     "chol_tflops": ci["chol_flops"] / ci["chol_kernel_s"] / 1e12 if ci["chol_kernel_s"] else None}
+        log(f"{mode}: step {t_step:.3f}s (setup {inf.t_setup_s:.3f}s, solve {inf.t_solve_s:.3f}s), its {inf.its} "
+            f"reason {inf.reason}, residual estimate {inf.rnorm/inf.rnorm0:.3e}, true {true_rel:.3e}, "
+            f"regularised blocks {inf.n_regularised}")
+    print(json.dumps({
+        "metric": "2D N=4M non-uniform mixture on the adaptive cluster structure: step time and residual reached",
+        "n": n, "adaptive_cap": cap, "adaptive_max_depth": depth, "t_adaptive_lists_s": t_lists,
+        "structure": stats, "iteration_budget": args.c3_its, "modes": out,
+    }), flush=True)
 
 
 def bench_c5(args, torch, problems, Context, dev):

@@ -39,7 +39,8 @@ typedef enum {
   PRBF_ERR_HIP = 2,      /* HIP runtime error (device fault, launch failure) */
   PRBF_ERR_NOMEM = 3,    /* device allocation failed */
   PRBF_ERR_STATE = 4,    /* call out of order (e.g. solve before set_points) */
-  PRBF_ERR_FACTOR = 5,   /* a RASM block was not positive definite */
+  PRBF_ERR_FACTOR = 5,   /* reserved: RASM blocks are factorised with regularised pivots
+                            (prbf_solve_info.n_regularised), not rejected */
   PRBF_ERR_OUTSIDE = 6,  /* a point fell outside the cell grid */
   PRBF_ERR_LIMIT = 7     /* a block exceeds the kernel's supported size */
 } prbf_status;
@@ -99,7 +100,10 @@ typedef struct {
   double t_pc_s;     /* summed device time of the preconditioner applies (0 with graph = 1) */
   int n_matvec;
   int n_pc;
-  int n_fallback;    /* always 0: an FP32 block breakdown fails the setup with PRBF_ERR_FACTOR */
+  int n_regularised; /* RASM blocks whose Cholesky met a pivot at or below tau times the
+                        diagonal 1/(2 pi sigma^2) and used that floor instead (tau = 2^-20
+                        for FP32 blocks, 2^-46 for FP64): numerically singular blocks of
+                        clustered points; 0 on well-separated points */
   int n_network;     /* LEARNED: clusters served by the network at the end of the solve */
   int n_net_fallback; /* LEARNED: clusters the solve moved from the network to exact blocks */
 } prbf_solve_info;
@@ -125,7 +129,34 @@ prbf_status prbf_set_points(prbf_ctx* ctx, int64_t ni, const double* xi, const d
 prbf_status prbf_set_points_dev(prbf_ctx* ctx, int64_t ni, const double* xi, const double* yi, const double* zi,
                                 int64_t nj, const double* xj, const double* yj, const double* zj);
 
-/* info[0..15]: ncell, nx, ny, nz, nb_buffer, nb_trunc, buffer_nnz, trunc_nnz,
+/* Adaptive cluster structure over one point set (the solve), for strongly
+ * non-uniform particle sets.  No reference counterpart: it generalises
+ * Get_cluster::get_cluster's uniform grid (2d/get_cluster.h:20-166) and the
+ * list builders (2d/get_buffer.h:9-90, 2d/get_trunc.h:9-87).  The grid of
+ * prbf_set_points is level 0; a cell holding more than `cap` particles splits
+ * into 2^dim children of half its side (child 2 qx + qy, 3D 4 qz + 2 qx + qy,
+ * q = coordinate >= the cell centre), recursively down to depth max_depth
+ * (0..20); the leaves, in depth-first order with empty ones kept, are the
+ * clusters, and the particles are sorted by leaf, then original index.  A
+ * leaf at depth d has buffer half-width buffer_len 2^-d (blocks keep their
+ * shape relative to the cluster) and truncation half-width
+ * (trunc_len - box/2) + box 2^-(d+1) (the same cutoff beyond the cluster);
+ * its lists are its own particles, then those of the level-0 cells within the
+ * reference's reach of its cell, in the reference's scan order, except its
+ * own, that pass the per-axis box test around the leaf centre.  With no split
+ * everything equals prbf_set_points.  Every later call (matvec, RASM, solve,
+ * evaluation) works on the leaves; prbf_get_cells and the CSR getters return
+ * per-leaf arrays and info[0] the number of leaves. */
+prbf_status prbf_set_points_adaptive(prbf_ctx* ctx, int64_t n, const double* x, const double* y, const double* z,
+                                     int cap, int max_depth);
+prbf_status prbf_set_points_adaptive_dev(prbf_ctx* ctx, int64_t n, const double* x, const double* y,
+                                         const double* z, int cap, int max_depth);
+/* Per leaf of the adaptive structure: centre, depth, level-0 cell, buffer and
+ * truncation half-widths (any pointer may be NULL; zc ignored in 2D). */
+prbf_status prbf_get_leaves(prbf_ctx* ctx, double* xc, double* yc, double* zc, int32_t* depth, int32_t* cell0,
+                            double* blen, double* tlen);
+
+/* info[0..15]: ncell (clusters: grid cells, or leaves), nx, ny, nz, nb_buffer, nb_trunc, buffer_nnz, trunc_nnz,
  * max_buffer, max_core, max_trunc, ni, nj, and the wavefronts, panel width
  * and order (prbf_chol_order) the last block factorisation used (0 before
  * any).

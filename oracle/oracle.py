@@ -40,6 +40,9 @@ def lib():
     L.orc_num_threads.restype = i32
     L.orc_set_points.restype = i32
     L.orc_set_points.argtypes = [vp, ll, vp, vp, vp, ll, vp, vp, vp]
+    L.orc_set_points_tree.restype = i32
+    L.orc_set_points_tree.argtypes = [vp, ll, vp, vp, vp, i32, i32]
+    L.orc_get_leaves.argtypes = [vp, _dp, _dp, _dp, _ip, _lp, _dp, _dp]
     for f in ("orc_ncell", "orc_buffer_nnz", "orc_trunc_nnz"):
         getattr(L, f).restype = ll
         getattr(L, f).argtypes = [vp]
@@ -90,7 +93,9 @@ class Oracle:
     source set), with the reference's lists, matvec, RASM and GMRES."""
 
     def __init__(self, dim, sigma, nsigma_box, sigma_buffer, sigma_trunc,
-                 xi, yi, zi=None, xj=None, yj=None, zj=None):
+                 xi, yi, zi=None, xj=None, yj=None, zj=None, tree=None):
+        """tree=(cap, dmax): the adaptive structure (one point set; cells of
+        more than cap particles split down to depth dmax)."""
         L = lib()
         self.dim = dim
         self.h = L.orc_create(dim, sigma, nsigma_box, sigma_buffer, sigma_trunc)
@@ -100,15 +105,32 @@ class Oracle:
         xi, yi, zi, xj, yj, zj = self._keep
         self.ni = xi.shape[0]
         self.nj = self.ni if xj is None else xj.shape[0]
-        st = L.orc_set_points(self.h, self.ni, _ptr(xi), _ptr(yi), _ptr(zi),
-                              self.nj, _ptr(xj), _ptr(yj), _ptr(zj))
+        self.tree = tree
+        if tree is not None:
+            if xj is not None:
+                raise ValueError("the adaptive structure takes one point set")
+            st = L.orc_set_points_tree(self.h, self.ni, _ptr(xi), _ptr(yi), _ptr(zi), int(tree[0]), int(tree[1]))
+        else:
+            st = L.orc_set_points(self.h, self.ni, _ptr(xi), _ptr(yi), _ptr(zi),
+                                  self.nj, _ptr(xj), _ptr(yj), _ptr(zj))
         if st != 0:
             raise ValueError(f"orc_set_points failed ({st})")
         self.ncell = L.orc_ncell(self.h)
 
     @classmethod
-    def from_problem(cls, p):
-        return cls(p.dim, p.sigma, p.nsigma_box, p.sigma_buffer, p.sigma_trunc, p.x, p.y, p.z)
+    def from_problem(cls, p, tree=None):
+        return cls(p.dim, p.sigma, p.nsigma_box, p.sigma_buffer, p.sigma_trunc, p.x, p.y, p.z, tree=tree)
+
+    def leaves(self):
+        """Adaptive structure: per leaf its centre (x, y, z), depth, level-0
+        cell, buffer and truncation half-widths."""
+        n = self.ncell
+        xc, yc, zc = np.zeros(n), np.zeros(n), np.zeros(n)
+        depth = np.zeros(n, np.int32)
+        cell0 = np.zeros(n, np.int64)
+        blen, tlen = np.zeros(n), np.zeros(n)
+        lib().orc_get_leaves(self.h, xc, yc, zc, depth, cell0, blen, tlen)
+        return dict(xc=xc, yc=yc, zc=zc, depth=depth, cell0=cell0, blen=blen, tlen=tlen)
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -60,6 +60,15 @@ struct Oracle {
   std::vector<int> piv;          // offset bptr[c]
   bool rasm_ready = false;
   long long rasm_stride = 1;
+  // adaptive structure: clusters are the leaves of an occupancy-capped
+  // subdivision of the grid (ncell then counts leaves); per leaf its centre,
+  // depth, grid cell, and per grid cell its particle range in sorted order
+  bool tree = false;
+  long long ncell0 = 0;
+  std::vector<double> lctr[3];
+  std::vector<int> ldepth;
+  std::vector<long long> lcell0;
+  std::vector<int> c0sta, c0end;
 };
 
 inline long long cell_id(const Oracle& o, int ix, int iy, int iz) {
@@ -192,6 +201,137 @@ void trunc_of(const Oracle& o, long long c, Emit&& emit) {
     for (int j = o.jsta[jc]; j <= o.jend[jc]; ++j)
       if (in_box(o, c, o.xj, j, len)) emit(j);
   });
+}
+
+// ---- adaptive cluster structure ---------------------------------------------
+// The reference clusters particles on a uniform grid (2d/get_cluster.h:20-166)
+// and builds each cluster's lists from the neighbour cells
+// (2d/get_buffer.h:11-81, 2d/get_trunc.h:14-78).  For strongly non-uniform
+// particle sets the adaptive structure keeps that grid as level 0 and splits
+// any cell holding more than `cap` particles into 2^dim children of half its
+// side, recursively, down to depth `dmax`; the leaves are the clusters.
+//   * child k of a node with centre c at depth d: k = 2 qx + qy (3D: 4 qz +
+//     2 qx + qy, the grid's own index order), q = (coordinate >= c) per axis,
+//     centre c +- box 2^-(d+2) per axis; level-0 centres are the reference's;
+//   * leaves in depth-first order, children in index order, empty ones kept;
+//     particles sorted by leaf, then by original index;
+//   * a leaf at depth d has buffer half-width buffer_len 2^-d (the block keeps
+//     its shape relative to the cluster) and truncation half-width
+//     (trunc_len - box/2) + box 2^-(d+1) (the same cutoff beyond the cluster;
+//     trunc_len itself at depth 0);
+//   * a leaf's list is its own particles, then those of the level-0 cells
+//     within the reference's reach of its level-0 cell, in the reference's
+//     scan order and sorted order, except its own, that pass the per-axis box
+//     test around the leaf centre.
+// With no cell split the structure and its lists are the reference's.
+struct TreeLeaf {
+  double c[3];
+  int depth;
+  long long cell0;
+};
+
+void tree_split(const Oracle& o, const std::vector<int>& pts, const double* const* src, const double c[3],
+                int depth, long long cell0, int cap, int dmax, std::vector<TreeLeaf>& leaves,
+                std::vector<std::vector<int>>& members) {
+  if ((int)pts.size() <= cap || depth >= dmax) {
+    leaves.push_back({{c[0], c[1], c[2]}, depth, cell0});
+    members.push_back(pts);
+    return;
+  }
+  const int nch = 1 << o.dim;
+  std::vector<std::vector<int>> ch(nch);
+  for (int p : pts) {
+    const int qx = src[0][p] >= c[0], qy = src[1][p] >= c[1], qz = o.dim == 3 && src[2][p] >= c[2];
+    ch[o.dim == 3 ? 4 * qz + 2 * qx + qy : 2 * qx + qy].push_back(p);
+  }
+  const double q = ldexp(o.box, -(depth + 2));
+  for (int k = 0; k < nch; ++k) {
+    const int qx = (k >> 1) & 1, qy = k & 1, qz = (k >> 2) & 1;
+    const double cc[3] = {c[0] + (qx ? q : -q), c[1] + (qy ? q : -q), o.dim == 3 ? c[2] + (qz ? q : -q) : 0.0};
+    tree_split(o, ch[k], src, cc, depth + 1, cell0, cap, dmax, leaves, members);
+  }
+}
+
+// Leaves, sorted order and level-0 ranges of one point set (the i-set = the
+// j-set).  Returns false when a point falls outside the grid.
+bool tree_build(Oracle& o, long long n, const double* const* src, int cap, int dmax) {
+  std::vector<std::vector<int>> in_cell(o.ncell);
+  for (long long p = 0; p < n; ++p) {
+    double q[3] = {src[0][p], src[1][p], o.dim == 3 ? src[2][p] : 0.0};
+    long long ic;
+    if (!point_cell(o, q, &ic)) return false;
+    in_cell[ic].push_back((int)p);
+  }
+  std::vector<TreeLeaf> leaves;
+  std::vector<std::vector<int>> members;
+  std::vector<long long> first_leaf(o.ncell + 1, 0);
+  for (long long c = 0; c < o.ncell; ++c) {
+    first_leaf[c] = (long long)leaves.size();
+    const double c0[3] = {o.ctr[0][c], o.ctr[1][c], o.dim == 3 ? o.ctr[2][c] : 0.0};
+    tree_split(o, in_cell[c], src, c0, 0, c, cap, dmax, leaves, members);
+  }
+  first_leaf[o.ncell] = (long long)leaves.size();
+  const long long nl = (long long)leaves.size();
+  o.ncell0 = o.ncell;
+  o.ista.assign(nl, 0);
+  o.iend.assign(nl, -1);
+  o.perm_i.assign(n, 0);
+  for (int d = 0; d < o.dim; ++d) o.xi[d].assign(n, 0.0);
+  for (int d = 0; d < 3; ++d) o.lctr[d].assign(nl, 0.0);
+  o.ldepth.assign(nl, 0);
+  o.lcell0.assign(nl, 0);
+  long long s = 0;
+  for (long long l = 0; l < nl; ++l) {
+    o.ista[l] = (int)s;
+    for (int p : members[l]) {
+      o.perm_i[s] = p;
+      for (int d = 0; d < o.dim; ++d) o.xi[d][s] = src[d][p];
+      ++s;
+    }
+    o.iend[l] = (int)s - 1;
+    for (int d = 0; d < 3; ++d) o.lctr[d][l] = leaves[l].c[d];
+    o.ldepth[l] = leaves[l].depth;
+    o.lcell0[l] = leaves[l].cell0;
+  }
+  o.c0sta.assign(o.ncell0, 0);
+  o.c0end.assign(o.ncell0, -1);
+  for (long long c = 0; c < o.ncell0; ++c) {
+    o.c0sta[c] = o.ista[first_leaf[c]];
+    o.c0end[c] = first_leaf[c + 1] < nl ? o.ista[first_leaf[c + 1]] - 1 : (int)n - 1;
+  }
+  o.ncell = nl;
+  return true;
+}
+
+// Half-widths of a leaf's buffer and truncation boxes at depth d.
+inline double tree_blen(const Oracle& o, int d) { return ldexp(o.sigma_buffer * o.sigma / 2 + kEpsf, -d); }
+inline double tree_tlen(const Oracle& o, int d) {
+  const double t = o.sigma_trunc * o.sigma / 2 + kEpsf;
+  return d == 0 ? t : (t - o.box * 0.5) + ldexp(o.box, -(d + 1));
+}
+
+template <class Emit>
+void tree_list_of(const Oracle& o, long long l, int reach, double len, Emit&& emit) {
+  if (o.ista[l] > o.iend[l]) return;
+  for (int j = o.ista[l]; j <= o.iend[l]; ++j) emit(j);
+  const long long c0 = o.lcell0[l];
+  const int k[3] = {o.cix[0][c0], o.cix[1][c0], o.cix[2][c0]};
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = d < o.dim ? std::max(0, k[d] - reach) : 0;
+    hi[d] = d < o.dim ? std::min(o.nc[d] - 1, k[d] + reach) : 0;
+  }
+  for (int jx = lo[0]; jx <= hi[0]; ++jx)
+    for (int jy = lo[1]; jy <= hi[1]; ++jy)
+      for (int jz = lo[2]; jz <= hi[2]; ++jz) {
+        const long long jc = cell_id(o, jx, jy, jz);
+        for (int j = o.c0sta[jc]; j <= o.c0end[jc]; ++j) {
+          if (j >= o.ista[l] && j <= o.iend[l]) continue;
+          bool in = true;
+          for (int d = 0; d < o.dim; ++d) in = in && fabs(o.xi[d][j] - o.lctr[d][l]) < len;
+          if (in) emit(j);
+        }
+      }
 }
 
 template <class ListFn>
@@ -642,6 +782,7 @@ int orc_set_points(void* h, long long ni, const double* xi, const double* yi, co
                    long long nj, const double* xj, const double* yj, const double* zj) {
   Oracle& o = *static_cast<Oracle*>(h);
   o.same_sets = (xj == nullptr);
+  o.tree = false;
   if (o.same_sets) { nj = ni; xj = xi; yj = yi; zj = zi; }
   if (ni <= 0 || nj <= 0) return 1;
   const double* si[3] = {xi, yi, zi};
@@ -663,6 +804,52 @@ int orc_set_points(void* h, long long ni, const double* xi, const double* yi, co
   return 0;
 }
 
+// Adaptive structure over one point set (the solve): the grid of
+// orc_set_points, cells of more than `cap` particles split recursively down to
+// depth `dmax`, the leaves as clusters (see tree_build).
+int orc_set_points_tree(void* h, long long n, const double* x, const double* y, const double* z, int cap,
+                        int dmax) {
+  Oracle& o = *static_cast<Oracle*>(h);
+  if (n <= 0 || cap < 1 || dmax < 0 || dmax > 20) return 1;
+  const double* si[3] = {x, y, z};
+  double lo[3], hi[3];
+  for (int d = 0; d < o.dim; ++d) {
+    lo[d] = si[d][0]; hi[d] = si[d][0];
+    for (long long p = 0; p < n; ++p) { lo[d] = std::min(lo[d], si[d][p]); hi[d] = std::max(hi[d], si[d][p]); }
+  }
+  make_grid(o, lo, hi);
+  o.same_sets = true;
+  o.tree = true;
+  o.ni = o.nj = n;
+  if (!tree_build(o, n, si, cap, dmax)) return 1;
+  o.perm_j = o.perm_i;
+  o.jsta = o.ista;
+  o.jend = o.iend;
+  for (int d = 0; d < 3; ++d) o.xj[d] = o.xi[d];
+  build_csr(o, [](const Oracle& oo, long long l, auto&& e) {
+    tree_list_of(oo, l, oo.nb_buffer, tree_blen(oo, oo.ldepth[l]), e); }, o.bptr, o.bidx);
+  build_csr(o, [](const Oracle& oo, long long l, auto&& e) {
+    tree_list_of(oo, l, oo.nb_trunc, tree_tlen(oo, oo.ldepth[l]), e); }, o.tptr, o.tidx);
+  o.rasm_ready = false;
+  return 0;
+}
+
+// Leaf geometry of the adaptive structure: centres, depths, level-0 cells,
+// buffer and truncation half-widths.
+void orc_get_leaves(void* h, double* xc, double* yc, double* zc, int* depth, long long* cell0, double* blen,
+                    double* tlen) {
+  Oracle& o = *static_cast<Oracle*>(h);
+  for (long long l = 0; l < o.ncell; ++l) {
+    xc[l] = o.lctr[0][l];
+    yc[l] = o.lctr[1][l];
+    if (zc) zc[l] = o.lctr[2][l];
+    depth[l] = o.ldepth[l];
+    cell0[l] = o.lcell0[l];
+    blen[l] = tree_blen(o, o.ldepth[l]);
+    tlen[l] = tree_tlen(o, o.ldepth[l]);
+  }
+}
+
 // ints[0..7] = nx, ny, nz, nb_buffer, nb_trunc, (ncell lo 31 bits) ...; use orc_ncell for counts.
 long long orc_ncell(void* h) { return static_cast<Oracle*>(h)->ncell; }
 long long orc_buffer_nnz(void* h) { auto& o = *static_cast<Oracle*>(h); return o.bptr.empty() ? 0 : o.bptr.back(); }
@@ -678,9 +865,10 @@ void orc_get_grid(void* h, int* ints, double* reals) {
 
 void orc_get_centres(void* h, double* xc, double* yc, double* zc) {
   Oracle& o = *static_cast<Oracle*>(h);
-  std::memcpy(xc, o.ctr[0].data(), sizeof(double) * o.ncell);
-  std::memcpy(yc, o.ctr[1].data(), sizeof(double) * o.ncell);
-  if (zc) std::memcpy(zc, o.ctr[2].data(), sizeof(double) * o.ncell);
+  const std::vector<double>* c = o.tree ? o.lctr : o.ctr;  // cluster centres: leaves or cells
+  std::memcpy(xc, c[0].data(), sizeof(double) * o.ncell);
+  std::memcpy(yc, c[1].data(), sizeof(double) * o.ncell);
+  if (zc) std::memcpy(zc, c[2].data(), sizeof(double) * o.ncell);
 }
 
 void orc_get_perm(void* h, int* perm_i, int* perm_j) {

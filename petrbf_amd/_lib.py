@@ -25,7 +25,8 @@ STATUS = {0: "PRBF_OK", 1: "PRBF_ERR_ARG", 2: "PRBF_ERR_HIP", 3: "PRBF_ERR_NOMEM
 # every symbol the header declares
 EXPORTS = [
     "prbf_solve_opts_default", "prbf_create", "prbf_destroy", "prbf_last_error", "prbf_set_stream",
-    "prbf_synchronize", "prbf_set_points", "prbf_set_points_dev", "prbf_get_info",
+    "prbf_synchronize", "prbf_set_points", "prbf_set_points_dev", "prbf_set_points_adaptive",
+    "prbf_set_points_adaptive_dev", "prbf_get_leaves", "prbf_get_info",
     "prbf_get_permutation", "prbf_get_cells", "prbf_get_buffer_csr", "prbf_get_trunc_csr",
     "prbf_get_sorted_points", "prbf_matvec", "prbf_matvec_dev", "prbf_evaluate", "prbf_evaluate_dev",
     "prbf_rasm_setup", "prbf_rasm_setup_pc", "prbf_set_tuning", "prbf_rasm_apply", "prbf_rasm_apply_dev", "prbf_rasm_get_core_rows",
@@ -51,7 +52,7 @@ class PrbfSolveOpts(C.Structure):
 class PrbfSolveInfo(C.Structure):
     _fields_ = [("its", C.c_int), ("reason", C.c_int), ("rnorm0", C.c_double), ("rnorm", C.c_double),
                 ("t_setup_s", C.c_double), ("t_solve_s", C.c_double), ("t_matvec_s", C.c_double),
-                ("t_pc_s", C.c_double), ("n_matvec", C.c_int), ("n_pc", C.c_int), ("n_fallback", C.c_int),
+                ("t_pc_s", C.c_double), ("n_matvec", C.c_int), ("n_pc", C.c_int), ("n_regularised", C.c_int),
                 ("n_network", C.c_int), ("n_net_fallback", C.c_int)]
 
 
@@ -92,6 +93,9 @@ def load():
     L.prbf_synchronize.argtypes = [ctx]
     for f in ("prbf_set_points", "prbf_set_points_dev"):
         getattr(L, f).argtypes = [ctx, i64, dp, dp, dp, i64, dp, dp, dp]
+    for f in ("prbf_set_points_adaptive", "prbf_set_points_adaptive_dev"):
+        getattr(L, f).argtypes = [ctx, i64, dp, dp, dp, i32, i32]
+    L.prbf_get_leaves.argtypes = [ctx, dp, dp, dp, dp, dp, dp, dp]
     L.prbf_get_info.argtypes = [ctx, dp, dp]
     L.prbf_get_permutation.argtypes = [ctx, dp, dp]
     L.prbf_get_cells.argtypes = [ctx, dp, dp, dp, dp]

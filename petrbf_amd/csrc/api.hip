@@ -102,6 +102,14 @@ struct prbf_ctx {
   std::string err;
   bool has_points = false, same = true;
   long long ni = 0, nj = 0;
+  // clusters: the grid's cells, or the leaves of the adaptive structure
+  // (tree), with the leaves' geometry and the grid cells' particle ranges
+  long long nclus = 0;
+  bool tree = false;
+  int tree_cap = 0, tree_dmax = 0;
+  DBuf<int> lcell, c0sta, c0end;
+  DBuf<double> lcx, lcy, lcz, lbl, ltl;
+  std::vector<int> h_ldepth;
   // sorted particles and cell structure
   DBuf<double> xi, yi, zi, xj, yj, zj;
   DBuf<int> perm_i, perm_j, ista, iend, jsta, jend;
@@ -124,7 +132,7 @@ struct prbf_ctx {
   long long n32 = 0, n64 = 0;
   bool l32_all = false, l64_all = false;
   // scratch
-  DBuf<int> i_scr1, i_scr2, i_scr3, d_err;
+  DBuf<int> i_scr1, i_scr2, i_scr3, i_scr4, d_err;
   DBuf<long long> l_scr;
   DBuf<double> part, scal, v1, v2, v3;
   double* pin = nullptr;  // pinned host scalars
@@ -137,7 +145,7 @@ struct prbf_ctx {
   int used_nw = 0, used_npc = 0, used_order = 0;  // configuration of the last factorisation
   double t_chol_s = 0, chol_flops = 0;  // its kernel time and algorithmic FLOPs
   int rasm_pc = 0;
-  int n_fallback = 0;
+  int n_regularised = 0;
   DBuf<float> M32;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // block factorisation: work list, counter and workspace slots
@@ -295,6 +303,27 @@ void build_lists_same(prbf_ctx* c) {
   c->trunc_nnz = tot[1];
 }
 
+// Host copies of the cluster structure (list pointers, ranges) for chunking
+// and size limits.
+void host_cluster_copies(prbf_ctx* c) {
+  const long long nc = c->nclus;
+  c->h_bptr.resize(nc + 1);
+  c->h_ista.resize(nc);
+  c->h_iend.resize(nc);
+  std::vector<long long> h_tptr(nc + 1);
+  PRBF_HIP(hipMemcpyAsync(c->h_bptr.data(), c->bptr, sizeof(long long) * (nc + 1), hipMemcpyDeviceToHost, c->st));
+  PRBF_HIP(hipMemcpyAsync(h_tptr.data(), c->tptr, sizeof(long long) * (nc + 1), hipMemcpyDeviceToHost, c->st));
+  PRBF_HIP(hipMemcpyAsync(c->h_ista.data(), c->ista, sizeof(int) * nc, hipMemcpyDeviceToHost, c->st));
+  PRBF_HIP(hipMemcpyAsync(c->h_iend.data(), c->iend, sizeof(int) * nc, hipMemcpyDeviceToHost, c->st));
+  sync(c);
+  c->max_buffer = c->max_core = c->max_trunc = 0;
+  for (long long k = 0; k < nc; ++k) {
+    c->max_buffer = std::max<int>(c->max_buffer, (int)(c->h_bptr[k + 1] - c->h_bptr[k]));
+    c->max_trunc = std::max<int>(c->max_trunc, (int)(h_tptr[k + 1] - h_tptr[k]));
+    c->max_core = std::max(c->max_core, c->h_iend[k] - c->h_ista[k] + 1);
+  }
+}
+
 void set_points_impl(prbf_ctx* c, long long ni, const double* xi, const double* yi, const double* zi,
                      long long nj, const double* xj, const double* yj, const double* zj) {
   const int dim = c->prm.dim;
@@ -312,6 +341,7 @@ void set_points_impl(prbf_ctx* c, long long ni, const double* xi, const double* 
   c->rasm_ready = false;
   c->patches_ready = false;
   c->learned_ready = false;
+  c->tree = false;
   double lo[3], hi[3], lo2[3], hi2[3];
   bbox_of(c, ni, xi, yi, zi, lo, hi);
   if (!same) {
@@ -329,6 +359,7 @@ void set_points_impl(prbf_ctx* c, long long ni, const double* xi, const double* 
   PRBF_HIP(hipMemsetAsync(c->d_err, 0, sizeof(int), c->st));
   c->l_scr.ensure(g.ncell + 1);
   c->i_scr1.ensure(std::max<long long>(ni, nj));
+  c->i_scr4.ensure(std::max<long long>(ni, nj));
   c->i_scr2.ensure(g.ncell);
   c->i_scr3.ensure(g.ncell);
   auto sort_set = [&](long long n, const double* x, const double* y, const double* z, DBuf<double>& xs,
@@ -339,8 +370,8 @@ void set_points_impl(prbf_ctx* c, long long ni, const double* xi, const double* 
     perm.ensure(n);
     sta.ensure(g.ncell);
     end.ensure(g.ncell);
-    sort_points(c->st, g, n, x, y, dim == 3 ? z : nullptr, c->i_scr1, c->i_scr2, c->l_scr, c->i_scr3, perm, sta,
-                end, xs, ys, dim == 3 ? zs.p : nullptr, c->d_err);
+    sort_points(c->st, g, n, x, y, dim == 3 ? z : nullptr, c->i_scr1, c->i_scr2, c->l_scr, c->i_scr3, c->i_scr4,
+                perm, sta, end, xs, ys, dim == 3 ? zs.p : nullptr, c->d_err);
   };
   sort_set(ni, xi, yi, zi, c->xi, c->yi, c->zi, c->perm_i, c->ista, c->iend);
   if (!same) sort_set(nj, xj, yj, zj, c->xj, c->yj, c->zj, c->perm_j, c->jsta, c->jend);
@@ -354,22 +385,223 @@ void set_points_impl(prbf_ctx* c, long long ni, const double* xi, const double* 
     build_lists(c, true);
     build_lists(c, false);
   }
-  // host copies for chunking and size limits
-  c->h_bptr.resize(g.ncell + 1);
-  c->h_ista.resize(g.ncell);
-  c->h_iend.resize(g.ncell);
-  std::vector<long long> h_tptr(g.ncell + 1);
-  PRBF_HIP(hipMemcpyAsync(c->h_bptr.data(), c->bptr, sizeof(long long) * (g.ncell + 1), hipMemcpyDeviceToHost, c->st));
-  PRBF_HIP(hipMemcpyAsync(h_tptr.data(), c->tptr, sizeof(long long) * (g.ncell + 1), hipMemcpyDeviceToHost, c->st));
-  PRBF_HIP(hipMemcpyAsync(c->h_ista.data(), c->ista, sizeof(int) * g.ncell, hipMemcpyDeviceToHost, c->st));
-  PRBF_HIP(hipMemcpyAsync(c->h_iend.data(), c->iend, sizeof(int) * g.ncell, hipMemcpyDeviceToHost, c->st));
-  sync(c);
-  c->max_buffer = c->max_core = c->max_trunc = 0;
-  for (long long k = 0; k < g.ncell; ++k) {
-    c->max_buffer = std::max<int>(c->max_buffer, (int)(c->h_bptr[k + 1] - c->h_bptr[k]));
-    c->max_trunc = std::max<int>(c->max_trunc, (int)(h_tptr[k + 1] - h_tptr[k]));
-    c->max_core = std::max(c->max_core, c->h_iend[k] - c->h_ista[k] + 1);
+  c->nclus = g.ncell;
+  host_cluster_copies(c);
+  c->has_points = true;
+}
+
+// Half-widths of an adaptive leaf's buffer and truncation boxes at depth d:
+// the buffer keeps its shape relative to the cluster (buffer_len 2^-d), the
+// truncation keeps the cutoff beyond the cluster ((trunc_len - box/2) +
+// box 2^-(d+1); trunc_len itself at depth 0).
+double tree_blen(const Grid& g, int d) { return ldexp(g.buffer_len, -d); }
+double tree_tlen(const Grid& g, int d) {
+#pragma clang fp contract(off)
+  return d == 0 ? g.trunc_len : (g.trunc_len - g.box * 0.5) + ldexp(g.box, -(d + 1));
+}
+
+// The adaptive structure over one point set (the solve): the reference's grid
+// as level 0, cells of more than cap particles split into 2^dim children down
+// to depth dmax, the leaves as clusters (rules in tree.hip).  The frontier's
+// node counts come to the host level by level; the host keeps the node
+// records, decides the splits, numbers the children and gives their centres.
+void set_points_tree_impl(prbf_ctx* c, long long n, const double* x, const double* y, const double* z, int cap,
+                          int dmax) {
+  const int dim = c->prm.dim;
+  need(n > 0, PRBF_ERR_ARG, "n must be positive");
+  need(x && y && (dim == 2 || z), PRBF_ERR_ARG, "missing coordinates");
+  need(n < (1LL << 31), PRBF_ERR_ARG, "n exceeds int32 indexing");
+  need(cap >= 1 && dmax >= 0 && dmax <= 20, PRBF_ERR_ARG, "cap must be positive and max_depth in 0..20");
+  c->has_points = false;
+  c->rasm_ready = false;
+  c->patches_ready = false;
+  c->learned_ready = false;
+  c->tree = false;
+  double lo[3], hi[3];
+  bbox_of(c, n, x, y, z, lo, hi);
+  for (int d = 0; d < dim; ++d) need(std::isfinite(lo[d]) && std::isfinite(hi[d]), PRBF_ERR_ARG, "non-finite coordinate");
+  c->g = make_grid(c->prm, lo, hi);
+  const Grid& g = c->g;
+  need(g.ncell > 0 && g.ncell < (1LL << 31), PRBF_ERR_LIMIT, "cell grid too large");
+  c->same = true;
+  c->ni = c->nj = n;
+  const long long nc0 = g.ncell;
+  // level 0: the grid cells (node v < nc0 is cell v)
+  c->d_err.ensure(1);
+  PRBF_HIP(hipMemsetAsync(c->d_err, 0, sizeof(int), c->st));
+  DBuf<int> node, cnt;
+  node.ensure(n);
+  cnt.ensure(nc0);
+  cell_assign(c->st, g, n, x, y, z, node, cnt, c->d_err);
+  struct Node {
+    double c[3];
+    int depth;
+    int child;  // first child's node id, -1 for a leaf
+    long long cell0;
+  };
+  std::vector<Node> nodes(nc0);
+  for (long long k = 0; k < nc0; ++k) {
+    int ix, iy, iz;
+    cell_coords(g, k, &ix, &iy, &iz);
+    nodes[k] = {{cell_centre(g.xmin, ix, g.box), cell_centre(g.ymin, iy, g.box),
+                 dim == 3 ? cell_centre(g.zmin, iz, g.box) : 0.0},
+                0, -1, k};
   }
+  std::vector<int> hcnt(nc0);
+  PRBF_HIP(hipMemcpyAsync(hcnt.data(), cnt, sizeof(int) * nc0, hipMemcpyDeviceToHost, c->st));
+  int herr = 0;
+  PRBF_HIP(hipMemcpyAsync(&herr, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->st));
+  sync(c);
+  need(herr == 0, PRBF_ERR_OUTSIDE, "a point fell outside the cell grid");
+  const int nch = 1 << dim;
+  long long lo_f = 0, hi_f = nc0;  // the frontier: nodes [lo_f, hi_f)
+  DBuf<int> cbase;
+  DBuf<double> fcx, fcy, fcz;
+  for (int d = 0; d < dmax; ++d) {
+    const long long nf = hi_f - lo_f;
+    std::vector<int> hb(nf, -1);
+    std::vector<double> hx(nf), hy(nf), hz(nf);
+    bool any = false;
+    for (long long f = 0; f < nf; ++f) {
+      const Node v = nodes[lo_f + f];  // (a copy: the children's push_back may move the records)
+      hx[f] = v.c[0];
+      hy[f] = v.c[1];
+      hz[f] = v.c[2];
+      if (hcnt[f] <= cap) continue;
+      need((long long)nodes.size() + nch < (1LL << 31), PRBF_ERR_LIMIT, "adaptive structure too large");
+      nodes[lo_f + f].child = (int)nodes.size();
+      hb[f] = (int)nodes.size();
+      any = true;
+      const double q = ldexp(g.box, -(d + 2));
+      for (int k = 0; k < nch; ++k) {
+        const int qx = (k >> 1) & 1, qy = k & 1, qz = (k >> 2) & 1;
+        nodes.push_back({{v.c[0] + (qx ? q : -q), v.c[1] + (qy ? q : -q), dim == 3 ? v.c[2] + (qz ? q : -q) : 0.0},
+                         d + 1, -1, v.cell0});
+      }
+    }
+    if (!any) break;
+    cbase.ensure(nf);
+    fcx.ensure(nf);
+    fcy.ensure(nf);
+    fcz.ensure(nf);
+    PRBF_HIP(hipMemcpyAsync(cbase, hb.data(), sizeof(int) * nf, hipMemcpyHostToDevice, c->st));
+    PRBF_HIP(hipMemcpyAsync(fcx, hx.data(), sizeof(double) * nf, hipMemcpyHostToDevice, c->st));
+    PRBF_HIP(hipMemcpyAsync(fcy, hy.data(), sizeof(double) * nf, hipMemcpyHostToDevice, c->st));
+    PRBF_HIP(hipMemcpyAsync(fcz, hz.data(), sizeof(double) * nf, hipMemcpyHostToDevice, c->st));
+    tree_descend(c->st, dim, n, x, y, z, node, (int)lo_f, (int)hi_f, cbase, fcx, fcy, fcz);
+    lo_f = hi_f;
+    hi_f = (long long)nodes.size();
+    cnt.ensure(hi_f - lo_f);
+    tree_count(c->st, n, node, (int)lo_f, (int)hi_f, cnt);
+    PRBF_HIP(hipGetLastError());
+    hcnt.assign(hi_f - lo_f, 0);
+    PRBF_HIP(hipMemcpyAsync(hcnt.data(), cnt, sizeof(int) * (hi_f - lo_f), hipMemcpyDeviceToHost, c->st));
+    sync(c);  // the host vectors of this level are released on return
+  }
+  // leaves in depth-first order, children in index order
+  std::vector<int> leaf_of(nodes.size(), -1), first_leaf(nc0 + 1, 0);
+  std::vector<long long> order;
+  order.reserve(nodes.size());
+  {
+    std::vector<long long> stack;
+    for (long long k = 0; k < nc0; ++k) {
+      first_leaf[k] = (int)order.size();
+      stack.push_back(k);
+      while (!stack.empty()) {
+        const long long v = stack.back();
+        stack.pop_back();
+        if (nodes[v].child < 0) {
+          leaf_of[v] = (int)order.size();
+          order.push_back(v);
+        } else {
+          for (int j = nch - 1; j >= 0; --j) stack.push_back(nodes[v].child + j);
+        }
+      }
+    }
+    first_leaf[nc0] = (int)order.size();
+  }
+  const long long nl = (long long)order.size();
+  c->nclus = nl;
+  // particles sorted by leaf (stable in the original index)
+  DBuf<int> lof;
+  lof.ensure(nodes.size());
+  PRBF_HIP(hipMemcpyAsync(lof, leaf_of.data(), sizeof(int) * nodes.size(), hipMemcpyHostToDevice, c->st));
+  c->i_scr1.ensure(n);
+  c->i_scr2.ensure(nl);
+  c->i_scr3.ensure(nl);
+  c->i_scr4.ensure(n);
+  c->l_scr.ensure(nl + 1);
+  tree_leaf_keys(c->st, n, nl, node, lof, c->i_scr1, c->i_scr2);
+  c->xi.ensure(n);
+  c->yi.ensure(n);
+  if (dim == 3) c->zi.ensure(n);
+  c->perm_i.ensure(n);
+  c->ista.ensure(nl);
+  c->iend.ensure(nl);
+  sort_by_key(c->st, nl, n, c->i_scr1, c->i_scr2, c->l_scr, c->i_scr3, c->i_scr4, c->perm_i, c->ista, c->iend, x, y,
+              dim == 3 ? z : nullptr, c->xi, c->yi, dim == 3 ? c->zi.p : nullptr);
+  // leaf geometry and the grid cells' particle ranges
+  std::vector<long long> start(nl + 1);
+  PRBF_HIP(hipMemcpyAsync(start.data(), c->l_scr, sizeof(long long) * (nl + 1), hipMemcpyDeviceToHost, c->st));
+  sync(c);
+  std::vector<int> hl0(nl), s0(nc0), e0(nc0);
+  std::vector<double> hx(nl), hy(nl), hz(nl), hbl(nl), htl(nl);
+  c->h_ldepth.resize(nl);
+  for (long long l = 0; l < nl; ++l) {
+    const Node& v = nodes[order[l]];
+    hl0[l] = (int)v.cell0;
+    hx[l] = v.c[0];
+    hy[l] = v.c[1];
+    hz[l] = v.c[2];
+    hbl[l] = tree_blen(g, v.depth);
+    htl[l] = tree_tlen(g, v.depth);
+    c->h_ldepth[l] = v.depth;
+  }
+  for (long long k = 0; k < nc0; ++k) {
+    s0[k] = (int)start[first_leaf[k]];
+    e0[k] = (int)start[first_leaf[k + 1]] - 1;
+  }
+  c->lcell.ensure(nl);
+  c->lcx.ensure(nl);
+  c->lcy.ensure(nl);
+  c->lcz.ensure(nl);
+  c->lbl.ensure(nl);
+  c->ltl.ensure(nl);
+  c->c0sta.ensure(nc0);
+  c->c0end.ensure(nc0);
+  PRBF_HIP(hipMemcpyAsync(c->lcell, hl0.data(), sizeof(int) * nl, hipMemcpyHostToDevice, c->st));
+  PRBF_HIP(hipMemcpyAsync(c->lcx, hx.data(), sizeof(double) * nl, hipMemcpyHostToDevice, c->st));
+  PRBF_HIP(hipMemcpyAsync(c->lcy, hy.data(), sizeof(double) * nl, hipMemcpyHostToDevice, c->st));
+  PRBF_HIP(hipMemcpyAsync(c->lcz, hz.data(), sizeof(double) * nl, hipMemcpyHostToDevice, c->st));
+  PRBF_HIP(hipMemcpyAsync(c->lbl, hbl.data(), sizeof(double) * nl, hipMemcpyHostToDevice, c->st));
+  PRBF_HIP(hipMemcpyAsync(c->ltl, htl.data(), sizeof(double) * nl, hipMemcpyHostToDevice, c->st));
+  PRBF_HIP(hipMemcpyAsync(c->c0sta, s0.data(), sizeof(int) * nc0, hipMemcpyHostToDevice, c->st));
+  PRBF_HIP(hipMemcpyAsync(c->c0end, e0.data(), sizeof(int) * nc0, hipMemcpyHostToDevice, c->st));
+  // buffer and truncation lists of every leaf
+  c->bptr.ensure(nl + 1);
+  c->tptr.ensure(nl + 1);
+  DBuf<int> cb, ct;
+  cb.ensure(nl);
+  ct.ensure(nl);
+  tree_lists(c->st, g, false, nl, c->ista, c->iend, c->c0sta, c->c0end, c->lcell, c->lcx, c->lcy, c->lcz, c->lbl,
+             c->ltl, c->xi, c->yi, c->zi, cb, ct, nullptr, nullptr, nullptr, nullptr);
+  scan_excl(c->st, nl, cb, c->bptr);
+  scan_excl(c->st, nl, ct, c->tptr);
+  long long tot[2] = {0, 0};
+  PRBF_HIP(hipMemcpyAsync(&tot[0], c->bptr.p + nl, sizeof(long long), hipMemcpyDeviceToHost, c->st));
+  PRBF_HIP(hipMemcpyAsync(&tot[1], c->tptr.p + nl, sizeof(long long), hipMemcpyDeviceToHost, c->st));
+  sync(c);
+  c->bidx.ensure(tot[0]);
+  c->tidx.ensure(tot[1]);
+  tree_lists(c->st, g, true, nl, c->ista, c->iend, c->c0sta, c->c0end, c->lcell, c->lcx, c->lcy, c->lcz, c->lbl,
+             c->ltl, c->xi, c->yi, c->zi, nullptr, nullptr, c->bptr, c->bidx, c->tptr, c->tidx);
+  PRBF_HIP(hipGetLastError());
+  c->buffer_nnz = tot[0];
+  c->trunc_nnz = tot[1];
+  host_cluster_copies(c);  // (synchronises: the host arrays above are released on return)
+  c->tree = true;
+  c->tree_cap = cap;
+  c->tree_dmax = dmax;
   c->has_points = true;
 }
 
@@ -377,7 +609,7 @@ void matvec_impl(prbf_ctx* c, int nrhs, const double* g, double* w) {
   need(c->has_points, PRBF_ERR_STATE, "set_points first");
   need(nrhs >= 1 && nrhs <= 3, PRBF_ERR_ARG, "nrhs must be 1..3");
   need(g && w, PRBF_ERR_ARG, "null vector");
-  matvec(c->st, c->prm.dim, nrhs, c->g.ncell, c->ista, c->iend, c->tptr, c->tidx, c->xi, c->yi, c->zi,
+  matvec(c->st, c->prm.dim, nrhs, c->nclus, c->ista, c->iend, c->tptr, c->tidx, c->xi, c->yi, c->zi,
          c->xj_(), c->yj_(), c->zj_(), g, c->nj, w, c->ni, c->prm.sigma);
   PRBF_HIP(hipGetLastError());
 }
@@ -433,8 +665,7 @@ void launch_blocks(prbf_ctx* c, bool fp32, const CholCfg& cfg, long long nwork, 
                     c->ista, c->iend, c->xi, c->yi, c->zi, (float*)W, mptr, (float*)Mout, c->prm.sigma, status);
   else
     block_chol_fp64(c->st, g.dim, cfg.nw, cfg.npc, cfg.ll, nwork, order, next, nwg, ws_slot, nmax, tmax, c->bptr, c->bidx,
-                    c->ista, c->iend, c->xi, c->yi, c->zi, (double*)W, mptr, (double*)Mout, c->prm.sigma,
-                    c->d_err);
+                    c->ista, c->iend, c->xi, c->yi, c->zi, (double*)W, mptr, (double*)Mout, c->prm.sigma, status);
   PRBF_HIP(hipGetLastError());
 }
 
@@ -558,7 +789,7 @@ void largest_first(const prbf_ctx* c, std::vector<int>& order) {
 // serve, in cluster order; a list that holds every nonempty cluster is
 // replaced by a launch over all clusters (empty ones return at once).
 void build_apply_lists(prbf_ctx* c, bool learned) {
-  const long long nc = c->g.ncell;
+  const long long nc = c->nclus;
   std::vector<int> a32, a64;
   long long nonempty = 0;
   for (long long k = 0; k < nc; ++k) {
@@ -581,8 +812,8 @@ void build_apply_lists(prbf_ctx* c, bool learned) {
 }
 
 // Factorises the blocks of `order` (any precisions, per h_prec) and writes
-// their core rows; throws PRBF_ERR_FACTOR when an FP32 block breaks down or
-// an FP64 block is not numerically positive definite.
+// their core rows; the blocks whose modified Cholesky replaced a pivot by its
+// floor are counted in c->n_regularised.
 void factor_clusters(prbf_ctx* c, const std::vector<int>& order) {
   std::vector<int> o32, o64;
   int nmax32 = 0, tmax32 = 0, nmax64 = 0, tmax64 = 0;
@@ -600,39 +831,34 @@ void factor_clusters(prbf_ctx* c, const std::vector<int>& order) {
   }
   largest_first(c, o32);
   largest_first(c, o64);
+  const long long nc = c->nclus;
+  DBuf<int>& status = c->fb_status;
+  status.ensure(nc);
+  // blocks of order whose pivots were regularised, counted on the device
+  auto regularised = [&](const std::vector<int>& o) {
+    DBuf<int>& cnt = c->fb_next;  // the counter of the finished launch
+    c->fb_failed.ensure(o.size());
+    PRBF_HIP(hipMemsetAsync(cnt, 0, sizeof(int), c->st));
+    failed_blocks(c->st, (long long)o.size(), c->fb_order, status, c->fb_failed, cnt);
+    int nf = 0;
+    PRBF_HIP(hipMemcpyAsync(&nf, cnt, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    sync(c);
+    return nf;
+  };
   double t_k = 0, fl = 0;
   int used[3] = {0, 0, 0};
   if (!o64.empty()) {
-    PRBF_HIP(hipMemsetAsync(c->d_err, 0, sizeof(int), c->st));
-    factor_blocks<double>(c, false, o64, nmax64, tmax64, c->mptr64, c->M.p, nullptr, true);
-    int herr = 0;
-    PRBF_HIP(hipMemcpyAsync(&herr, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->st));
-    sync(c);
-    need(herr == 0, PRBF_ERR_FACTOR, "a RASM block is not numerically positive definite");
+    PRBF_HIP(hipMemsetAsync(status, 0, sizeof(int) * nc, c->st));
+    factor_blocks<double>(c, false, o64, nmax64, tmax64, c->mptr64, c->M.p, status, true);
+    c->n_regularised += regularised(o64);
     t_k += c->t_chol_s;
     fl += c->chol_flops;
     used[0] = c->used_nw, used[1] = c->used_npc, used[2] = c->used_order;
   }
   if (!o32.empty()) {
-    const long long nc = c->g.ncell;
-    DBuf<int>& status = c->fb_status;
-    status.ensure(nc);
     PRBF_HIP(hipMemsetAsync(status, 0, sizeof(int) * nc, c->st));
     factor_blocks<float>(c, true, o32, nmax32, tmax32, c->mptr, c->M32.p, status, true);
-    // the blocks whose FP32 factorisation broke down, counted on the device
-    DBuf<int>& cnt = c->fb_next;  // the counter of the finished launch
-    c->fb_failed.ensure(o32.size());
-    PRBF_HIP(hipMemsetAsync(cnt, 0, sizeof(int), c->st));
-    failed_blocks(c->st, (long long)o32.size(), c->fb_order, status, c->fb_failed, cnt);
-    int nf = 0;
-    PRBF_HIP(hipMemcpyAsync(&nf, cnt, sizeof(int), hipMemcpyDeviceToHost, c->st));
-    sync(c);
-    // a block whose FP32 factorisation broke down is reported, not redone
-    if (nf > 0) {
-      const std::string msg = "the FP32 factorisation of " + std::to_string(nf) +
-                              " RASM blocks broke down (not positive definite in FP32); use PRBF_PC_RASM_FP64";
-      need(false, PRBF_ERR_FACTOR, msg.c_str());
-    }
+    c->n_regularised += regularised(o32);
     t_k += c->t_chol_s;
     fl += c->chol_flops;
   } else if (used[0]) {
@@ -650,9 +876,8 @@ void rasm_setup_impl(prbf_ctx* c, int pc) {
   const bool mixed = (pc != PRBF_PC_RASM_FP64);
   // with the learned preconditioner only the clusters off the network get blocks
   const bool learned = (pc == PRBF_PC_LEARNED);
-  need(!learned || c->h_skip.size() == (size_t)c->g.ncell, PRBF_ERR_STATE, "prbf_learned_attach first");
-  const Grid& g = c->g;
-  const long long nc = g.ncell;
+  need(!learned || c->h_skip.size() == (size_t)c->nclus, PRBF_ERR_STATE, "prbf_learned_attach first");
+  const long long nc = c->nclus;
   c->rasm_ready = false;
   // block precision per cluster, and storage for every block of each
   // precision (network clusters too, so a later fallback can add theirs)
@@ -679,7 +904,7 @@ void rasm_setup_impl(prbf_ctx* c, int pc) {
                           c->st));
   c->M.ensure(c->h_mptr64[nc]);
   c->M32.ensure(c->h_mptr[nc]);
-  c->n_fallback = 0;
+  c->n_regularised = 0;
   build_apply_lists(c, learned);
   factor_clusters(c, order);
   c->rasm_ready = true;
@@ -755,7 +980,7 @@ void learned_fallback(prbf_ctx* c, const std::vector<int>& newc) {
   c->rasm_ready = false;
   for (int k : newc) c->h_skip[k] = 0;
   std::vector<int> list;
-  for (long long k = 0; k < c->g.ncell; ++k)
+  for (long long k = 0; k < c->nclus; ++k)
     if (c->h_skip[k]) list.push_back((int)k);
   c->n_mlp = (long long)list.size();
   if (!list.empty())
@@ -816,7 +1041,7 @@ void ensure_graphs(prbf_ctx* c, int pc, bool flex, long long n, int m) {
   std::memcpy(&sig, &c->prm.sigma, sizeof(sig));
   std::vector<unsigned long long> key = {
       (unsigned long long)n, (unsigned long long)m, flex ? 1ULL : 0ULL, (unsigned long long)pc,
-      (unsigned long long)c->rasm_pc, (unsigned long long)c->g.ncell, (unsigned long long)c->ni,
+      (unsigned long long)c->rasm_pc, (unsigned long long)c->nclus, (unsigned long long)c->ni,
       (unsigned long long)c->nj, (unsigned long long)c->prm.dim, sig, u(c->V.p), u(c->Z.p), u(c->v1.p),
       u(c->part.p), u(c->scal.p), u(c->kst.p), u(c->pin), u(c->ista.p), u(c->iend.p), u(c->tptr.p), u(c->tidx.p),
       u(c->xi.p), u(c->yi.p), u(c->zi.p), u(c->xj_()), u(c->yj_()), u(c->zj_()), u(c->bptr.p), u(c->bidx.p),
@@ -939,7 +1164,7 @@ void krylov_impl(prbf_ctx* c, const prbf_solve_opts& o, const double* b, double*
           const int cl = c->h_mlp_list[k];
           if (!(cur[k] <= kNetCycleDrop * kNetCycleDrop * prev_core[cl])) newc.push_back(cl);
         }
-      prev_core.assign(c->g.ncell, 0.0);
+      prev_core.assign(c->nclus, 0.0);
       for (long long k = 0; k < nm; ++k) prev_core[c->h_mlp_list[k]] = cur[k];
       if (!newc.empty()) {
         learned_fallback(c, newc);
@@ -1318,7 +1543,7 @@ void solve_multi_dev_impl(prbf_ctx* c, const prbf_solve_opts* opts, int k, const
   for (int r = 0; r < k; ++r) scatter(c->st, n, c->perm_i, xs + r * n, gamma + r * n);
   for (int r = 0; r < k; ++r) {
     local[r].t_setup_s = t_setup;
-    local[r].n_fallback = c->n_fallback;
+    local[r].n_regularised = c->n_regularised;
     if (info) info[r] = local[r];
   }
 }
@@ -1345,7 +1570,7 @@ void solve_dev_impl(prbf_ctx* c, const prbf_solve_opts* opts, const double* w, d
   krylov_impl(c, o, bs, xs, &local, h, cap);
   scatter(c->st, n, c->perm_i, xs, gamma);
   local.t_setup_s = t_setup;
-  local.n_fallback = c->n_fallback;
+  local.n_regularised = c->n_regularised;
   if (o.pc == PRBF_PC_LEARNED) {
     local.n_network = (int)c->n_mlp;
     local.n_net_fallback = c->n_net_fallback;
@@ -1456,12 +1681,54 @@ prbf_status prbf_set_points(prbf_ctx* c, int64_t ni, const double* xi, const dou
   });
 }
 
+static void d2h(prbf_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (dst && bytes) PRBF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->st));
+}
+
+prbf_status prbf_set_points_adaptive_dev(prbf_ctx* c, int64_t n, const double* x, const double* y, const double* z,
+                                         int cap, int max_depth) {
+  return guarded(c, [&] { set_points_tree_impl(c, n, x, y, z, cap, max_depth); });
+}
+
+prbf_status prbf_set_points_adaptive(prbf_ctx* c, int64_t n, const double* x, const double* y, const double* z,
+                                     int cap, int max_depth) {
+  return guarded(c, [&] {
+    need(n > 0, PRBF_ERR_ARG, "n must be positive");
+    const int dim = c->prm.dim;
+    need(x && y && (dim == 2 || z), PRBF_ERR_ARG, "missing coordinates");
+    DBuf<double> buf;
+    buf.ensure((size_t)3 * n);
+    double* p = buf;
+    PRBF_HIP(hipMemcpyAsync(p, x, sizeof(double) * n, hipMemcpyHostToDevice, c->st));
+    PRBF_HIP(hipMemcpyAsync(p + n, y, sizeof(double) * n, hipMemcpyHostToDevice, c->st));
+    if (dim == 3) PRBF_HIP(hipMemcpyAsync(p + 2 * n, z, sizeof(double) * n, hipMemcpyHostToDevice, c->st));
+    set_points_tree_impl(c, n, p, p + n, dim == 3 ? p + 2 * n : nullptr, cap, max_depth);
+    sync(c);
+  });
+}
+
+prbf_status prbf_get_leaves(prbf_ctx* c, double* xc, double* yc, double* zc, int32_t* depth, int32_t* cell0,
+                            double* blen, double* tlen) {
+  return guarded(c, [&] {
+    need(c->has_points && c->tree, PRBF_ERR_STATE, "prbf_set_points_adaptive first");
+    const size_t nl = (size_t)c->nclus;
+    d2h(c, xc, c->lcx, sizeof(double) * nl);
+    d2h(c, yc, c->lcy, sizeof(double) * nl);
+    if (c->prm.dim == 3) d2h(c, zc, c->lcz, sizeof(double) * nl);
+    d2h(c, cell0, c->lcell, sizeof(int) * nl);
+    d2h(c, blen, c->lbl, sizeof(double) * nl);
+    d2h(c, tlen, c->ltl, sizeof(double) * nl);
+    sync(c);
+    if (depth) std::memcpy(depth, c->h_ldepth.data(), sizeof(int) * nl);
+  });
+}
+
 prbf_status prbf_get_info(prbf_ctx* c, int64_t* info, double* geom) {
   return guarded(c, [&] {
     need(c->has_points, PRBF_ERR_STATE, "set_points first");
     const Grid& g = c->g;
     if (info) {
-      int64_t v[16] = {g.ncell, g.nx, g.ny, g.nz, g.nb_buffer, g.nb_trunc, c->buffer_nnz, c->trunc_nnz,
+      int64_t v[16] = {c->nclus, g.nx, g.ny, g.nz, g.nb_buffer, g.nb_trunc, c->buffer_nnz, c->trunc_nnz,
                        c->max_buffer, c->max_core, c->max_trunc, c->ni, c->nj, c->used_nw, c->used_npc,
                        c->used_order};
       std::memcpy(info, v, sizeof(v));
@@ -1473,9 +1740,6 @@ prbf_status prbf_get_info(prbf_ctx* c, int64_t* info, double* geom) {
   });
 }
 
-static void d2h(prbf_ctx* c, void* dst, const void* src, size_t bytes) {
-  if (dst && bytes) PRBF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->st));
-}
 
 prbf_status prbf_get_permutation(prbf_ctx* c, int32_t* pi, int32_t* pj) {
   return guarded(c, [&] {
@@ -1489,7 +1753,7 @@ prbf_status prbf_get_permutation(prbf_ctx* c, int32_t* pi, int32_t* pj) {
 prbf_status prbf_get_cells(prbf_ctx* c, int32_t* ista, int32_t* iend, int32_t* jsta, int32_t* jend) {
   return guarded(c, [&] {
     need(c->has_points, PRBF_ERR_STATE, "set_points first");
-    size_t b = sizeof(int) * c->g.ncell;
+    size_t b = sizeof(int) * c->nclus;
     d2h(c, ista, c->ista, b);
     d2h(c, iend, c->iend, b);
     d2h(c, jsta, c->jsta_(), b);
@@ -1501,7 +1765,7 @@ prbf_status prbf_get_cells(prbf_ctx* c, int32_t* ista, int32_t* iend, int32_t* j
 prbf_status prbf_get_buffer_csr(prbf_ctx* c, int64_t* ptr, int32_t* idx) {
   return guarded(c, [&] {
     need(c->has_points, PRBF_ERR_STATE, "set_points first");
-    d2h(c, ptr, c->bptr, sizeof(long long) * (c->g.ncell + 1));
+    d2h(c, ptr, c->bptr, sizeof(long long) * (c->nclus + 1));
     d2h(c, idx, c->bidx, sizeof(int) * c->buffer_nnz);
     sync(c);
   });
@@ -1510,7 +1774,7 @@ prbf_status prbf_get_buffer_csr(prbf_ctx* c, int64_t* ptr, int32_t* idx) {
 prbf_status prbf_get_trunc_csr(prbf_ctx* c, int64_t* ptr, int32_t* idx) {
   return guarded(c, [&] {
     need(c->has_points, PRBF_ERR_STATE, "set_points first");
-    d2h(c, ptr, c->tptr, sizeof(long long) * (c->g.ncell + 1));
+    d2h(c, ptr, c->tptr, sizeof(long long) * (c->nclus + 1));
     d2h(c, idx, c->tidx, sizeof(int) * c->trunc_nnz);
     sync(c);
   });
@@ -1625,7 +1889,7 @@ prbf_status prbf_rasm_apply(prbf_ctx* c, const double* r, double* z) {
 prbf_status prbf_rasm_get_core_rows(prbf_ctx* c, int64_t k, double* M, int32_t* n, int32_t* T) {
   return guarded(c, [&] {
     need(c->rasm_ready && c->rasm_pc == PRBF_PC_RASM_FP64, PRBF_ERR_STATE, "FP64 rasm_setup first");
-    need(k >= 0 && k < c->g.ncell, PRBF_ERR_ARG, "cluster index out of range");
+    need(k >= 0 && k < c->nclus, PRBF_ERR_ARG, "cluster index out of range");
     int nn = (int)(c->h_bptr[k + 1] - c->h_bptr[k]);
     int tt = nn ? c->h_iend[k] - c->h_ista[k] + 1 : 0;
     if (n) *n = nn;
@@ -1643,7 +1907,7 @@ prbf_status prbf_rasm_get_core_rows(prbf_ctx* c, int64_t k, double* M, int32_t* 
 prbf_status prbf_rasm_get_block(prbf_ctx* c, int64_t k, double* A, int32_t* idx, int32_t* n) {
   return guarded(c, [&] {
     need(c->has_points && c->same, PRBF_ERR_STATE, "set_points (one set) first");
-    need(k >= 0 && k < c->g.ncell, PRBF_ERR_ARG, "cluster index out of range");
+    need(k >= 0 && k < c->nclus, PRBF_ERR_ARG, "cluster index out of range");
     const long long b0 = c->h_bptr[k];
     const int nn = (int)(c->h_bptr[k + 1] - b0);
     if (n) *n = nn;
@@ -1806,6 +2070,7 @@ void mlp_refresh_bf16(prbf_mlp* net) { mlp_weights_bf16(net->st, net->P, net->Wb
 void ensure_patches(prbf_ctx* c) {
   need(c->has_points && c->same, PRBF_ERR_STATE, "set_points (one set) first");
   need(c->prm.dim == 2, PRBF_ERR_ARG, "the learned preconditioner is two-dimensional");
+  need(!c->tree, PRBF_ERR_ARG, "the learned preconditioner works on the uniform grid's clusters");
   if (c->patches_ready) return;
   const long long nc = c->g.ncell;
   c->pidx.ensure((size_t)nc * kPatch);

@@ -32,8 +32,14 @@
 // instead of being re-read and re-written by every trailing update, and the
 // entries of A' are made where they enter instead of being stored.
 // T = double is the plain path (exact FP64 block solves); T = float runs the
-// same kernel in FP32 on the FP32 MFMA (mixed precision), with a per-block
-// status so that a block whose FP32 factorisation breaks down is reported.
+// same kernel in FP32 on the FP32 MFMA (mixed precision).  The factorisation
+// is a modified Cholesky: a pivot at or below tau times the Gaussian's
+// diagonal 1/(2 pi sigma^2) (tau = 2^-20 in FP32, 2^-46 in FP64, a few
+// hundred units of rounding) is replaced by that floor, the block's status is
+// set, and the block is finished like any other, so numerically singular
+// blocks (clustered points) still give a preconditioner.  Well-separated
+// blocks never reach the floor: their pivots stay near the smallest
+// eigenvalue, orders of magnitude above it.
 #include <mutex>
 #include <vector>
 
@@ -173,6 +179,13 @@ __device__ __forceinline__ float row_bcast(float v) {
 __device__ __forceinline__ float pivot_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ double pivot_rsqrt(double x) { return rsqrt(x); }
 
+// Smallest pivot the modified Cholesky keeps: tau times the Gaussian's
+// diagonal norm = 1/(2 pi sigma^2), tau = 2^-20 (FP32) or 2^-46 (FP64).
+template <typename Real>
+__device__ __forceinline__ Real pivot_floor(double norm) {
+  return sizeof(Real) == 4 ? (Real)(norm * 0x1p-20) : (Real)(norm * 0x1p-46);
+}
+
 // Step P of the symmetric elimination on [D | I]: every 16-lane row of the
 // wavefront holds the whole 16x16 problem, lane i of the row holding row i.
 // Row P is read through row_newbcast; rows below P subtract
@@ -180,12 +193,12 @@ __device__ __forceinline__ double pivot_rsqrt(double x) { return rsqrt(x); }
 // and lane P keeps 1/sqrt(a_PP), the scale of its row of L^{-1}.
 template <typename Real, int P>
 struct DiagStep {
-  __device__ static __forceinline__ void run(Real (&a)[16], Real (&e)[16], Real& rs, bool& bad, int i) {
+  __device__ static __forceinline__ void run(Real (&a)[16], Real (&e)[16], Real& rs, bool& bad, int i, Real pfloor) {
     e[P] = (i == P) ? Real(1) : Real(0);  // column P of the right half is untouched until now
     const Real piv = row_bcast<P>(a[P]);
-    const bool b = !(piv > Real(0));
+    const bool b = !(piv > pfloor);  // (NaN included)
     bad |= b;
-    const Real rinv = pivot_rsqrt(b ? Real(1) : piv);  // (branch-free: one block per step)
+    const Real rinv = pivot_rsqrt(b ? pfloor : piv);  // (branch-free: one block per step)
     rs = i == P ? rinv : rs;
     const Real m = i > P ? a[P] * (rinv * rinv) : Real(0);
 #pragma unroll
@@ -196,12 +209,12 @@ struct DiagStep {
     for (int q = 0; q <= P; ++q) {
       e[q] -= m * row_bcast<P>(e[q]);
     }
-    DiagStep<Real, P + 1>::run(a, e, rs, bad, i);
+    DiagStep<Real, P + 1>::run(a, e, rs, bad, i, pfloor);
   }
 };
 template <typename Real>
 struct DiagStep<Real, 16> {
-  __device__ static __forceinline__ void run(Real (&)[16], Real (&)[16], Real&, bool&, int) {}
+  __device__ static __forceinline__ void run(Real (&)[16], Real (&)[16], Real&, bool&, int, Real) {}
 };
 
 // Value of v in lane l (l uniform) via readlane.
@@ -213,9 +226,10 @@ __device__ inline double bcast(double v, int l) {
 }
 
 // L^{-1} of the 16x16 diagonal tile D (row-major, row stride DS) into Li (row
-// stride OS, zero above the diagonal), by one wavefront.  Returns true (in
-// every lane) when a pivot is not positive.  Li may be D: every lane holds
-// its rows in registers before any lane writes.
+// stride OS, zero above the diagonal), by one wavefront, with every pivot at
+// or below pfloor replaced by pfloor.  Returns true (in every lane) when a
+// pivot was replaced.  Li may be D: every lane holds its rows in registers
+// before any lane writes.
 //
 // FP32: the elimination of DiagStep on every 16-lane row at once, without
 // scaling rows as it goes (row i of L^{-1} is the right half's row i times
@@ -225,7 +239,7 @@ __device__ inline double bcast(double v, int l) {
 // and row p scaled at its step: the DPP form needs ~30 more VGPRs in FP64,
 // which the FP64 factorisation (at its 256-VGPR cap) would spill.
 template <typename Real, int DS, int OS>
-__device__ __forceinline__ bool diag_inverse(const Real* D, Real* Li, int lane) {
+__device__ __forceinline__ bool diag_inverse(const Real* D, Real* Li, int lane, Real pfloor) {
   if constexpr (sizeof(Real) == 4) {
     const int i = lane & 15;
     Real a[16], e[16];
@@ -233,7 +247,7 @@ __device__ __forceinline__ bool diag_inverse(const Real* D, Real* Li, int lane) 
     for (int q = 0; q < 16; ++q) a[q] = D[i * DS + q];
     Real rs = Real(1);
     bool bad = false;
-    DiagStep<Real, 0>::run(a, e, rs, bad, i);
+    DiagStep<Real, 0>::run(a, e, rs, bad, i, pfloor);
     if (lane < 16) {
 #pragma unroll
       for (int q = 0; q < 16; ++q) Li[i * OS + q] = q <= i ? e[q] * rs : Real(0);
@@ -250,9 +264,9 @@ __device__ __forceinline__ bool diag_inverse(const Real* D, Real* Li, int lane) 
 #pragma unroll
     for (int p = 0; p < 16; ++p) {
       const Real piv = bcast(ra[p], p);
-      const bool b = !(piv > Real(0));
+      const bool b = !(piv > pfloor);
       bad |= b;
-      const Real rinv = b ? Real(1) : pivot_rsqrt(piv);
+      const Real rinv = pivot_rsqrt(b ? pfloor : piv);
       const Real m = ra[p] * rinv;
       if (lane == p) {
 #pragma unroll
@@ -314,7 +328,7 @@ __host__ __device__ inline size_t blk_lds_bytes(int nmax, int tmax, int nw, int 
 template <typename Real, int NW, bool LM>
 __device__ void core_rows(long long c, const Real* __restrict__ Wb, const Real* __restrict__ Wd,
                           unsigned char* __restrict__ lds_raw, int n, int T, const long long* __restrict__ mptr,
-                          Real* __restrict__ M, int* __restrict__ blk_status) {
+                          Real* __restrict__ M) {
   typedef typename MF<Real>::v4 v4;
   const int KT = (n + 15) / 16, TC = (T + 15) / 16;
   const int ncore0 = n - T;
@@ -456,7 +470,6 @@ __device__ void core_rows(long long c, const Real* __restrict__ Wb, const Real* 
     }
     __syncthreads();
   }
-  if (blk_status && tid == 0) blk_status[c] = 0;
 }
 
 // One block: cluster c, workspace Wb (blk_ws elements), LDS as laid out below.
@@ -465,7 +478,7 @@ __device__ void factor_block(
     long long c, Real* __restrict__ Wb, unsigned char* __restrict__ lds_raw, const long long* __restrict__ bptr,
     const int* __restrict__ bidx, const int* __restrict__ ista, const int* __restrict__ iend,
     const double* __restrict__ xs, const double* __restrict__ ys, const double* __restrict__ zs,
-    const long long* __restrict__ mptr, Real* __restrict__ M, double inv_two_s2, double norm, int* __restrict__ err,
+    const long long* __restrict__ mptr, Real* __restrict__ M, double inv_two_s2, double norm,
     int* __restrict__ blk_status) {
   typedef typename MF<Real>::v4 v4;
   const long long b0 = bptr[c];
@@ -486,7 +499,7 @@ __device__ void factor_block(
   // LDS carve-up: one region holding, by phase, the coordinates (0), the
   // panel (1), and the right-hand sides X with the per-wave scratch tiles
   // (2, 3); then the diagonal tile, which its inverse overwrites in place,
-  // and the breakdown flag
+  // and the flag of regularised pivots
   typedef Pan<Real> PT;
   size_t pan = (size_t)pan_tiles(KT, min(NPC, KT)) * PT::kT * sizeof(Real), crd = (size_t)3 * NP * sizeof(double),
          red = ((size_t)NP * kLS + (size_t)NW * kLT) * sizeof(Real);
@@ -564,7 +577,7 @@ __device__ void factor_block(
           for (int r = 0; r < 4; ++r) D[MF<Real>::row(r, lq) * kLS + lr] = tkk[r * 64 + lane];
         }
         wave_fence();
-        if (diag_inverse<Real, kLS, kLS>(D, Li, lane) && lane == 0) *flag = 1;
+        if (diag_inverse<Real, kLS, kLS>(D, Li, lane, pivot_floor<Real>(norm)) && lane == 0) *flag = 1;
         wave_fence();
         Real* wd = Wd + (long long)kk * kTE;
 #pragma unroll
@@ -653,15 +666,8 @@ __device__ void factor_block(
     }
     __syncthreads();
   }
-  if (*flag) {
-    if (tid == 0) {
-      if (blk_status) blk_status[c] = 1;
-      else atomicOr(err, 2);
-    }
-    return;
-  }
-
-  core_rows<Real, NW, false>(c, Wb, Wd, lds_raw, n, T, mptr, M, blk_status);
+  if (tid == 0 && *flag) blk_status[c] = 1;  // regularised pivots (read after the last barrier)
+  core_rows<Real, NW, false>(c, Wb, Wd, lds_raw, n, T, mptr, M);
 }
 
 // ---- left-looking variant ------------------------------------------------
@@ -716,7 +722,7 @@ __device__ void factor_block_ll(
     long long c, Real* __restrict__ Wb, unsigned char* __restrict__ lds_raw, const long long* __restrict__ bptr,
     const int* __restrict__ bidx, const int* __restrict__ ista, const int* __restrict__ iend,
     const double* __restrict__ xs, const double* __restrict__ ys, const double* __restrict__ zs,
-    const long long* __restrict__ mptr, Real* __restrict__ M, double inv_two_s2, double norm, int* __restrict__ err,
+    const long long* __restrict__ mptr, Real* __restrict__ M, double inv_two_s2, double norm,
     int* __restrict__ blk_status) {
   typedef typename MF<Real>::v4 v4;
   constexpr bool PG = ll_rows_global(NPC);  // panel rows from the workspace, not LDS
@@ -742,7 +748,7 @@ __device__ void factor_block_ll(
   // and the right-hand sides with the per-wave scratch tiles (2, 3); then the
   // panel's diagonal tiles (row-major, each replaced by its inverse), the L
   // tiles of its diagonal block below the diagonal (lane-major), the panel
-  // columns' coordinates and the breakdown flag
+  // columns' coordinates and the flag of regularised pivots
   const size_t region = ll_region<Real>(KT, NW, NPC);
   Real* Rw = (Real*)lds_raw;
   Real* Dg = (Real*)(lds_raw + region);
@@ -878,7 +884,7 @@ __device__ void factor_block_ll(
       for (int q2 = 0; q2 < npc; ++q2) {
         Real* D = Dg + q2 * kLT;
         wave_fence();
-        if (diag_inverse<Real, kLS, kLS>(D, D, lane) && lane == 0) *flag = 1;
+        if (diag_inverse<Real, kLS, kLS>(D, D, lane, pivot_floor<Real>(norm)) && lane == 0) *flag = 1;
         wave_fence();
         Real* wd = Wd + (long long)(k0 + q2) * kTE;
 #pragma unroll
@@ -959,14 +965,8 @@ __device__ void factor_block_ll(
     }
     __syncthreads();
   }
-  if (*flag) {
-    if (tid == 0) {
-      if (blk_status) blk_status[c] = 1;
-      else atomicOr(err, 2);
-    }
-    return;
-  }
-  core_rows<Real, NW, true>(c, Wb, Wd, lds_raw, n, T, mptr, M, blk_status);
+  if (tid == 0 && *flag) blk_status[c] = 1;  // regularised pivots (read after the last barrier)
+  core_rows<Real, NW, true>(c, Wb, Wd, lds_raw, n, T, mptr, M);
 }
 
 // Persistent batched launch: each workgroup owns one workspace slot and takes
@@ -982,7 +982,7 @@ __global__ __launch_bounds__(NW * 64, sizeof(Real) == 4 ? 3 : 2) void k_block_ch
     const long long* __restrict__ bptr, const int* __restrict__ bidx, const int* __restrict__ ista,
     const int* __restrict__ iend, const double* __restrict__ xs, const double* __restrict__ ys,
     const double* __restrict__ zs, Real* __restrict__ Wbase, const long long* __restrict__ mptr, Real* __restrict__ M,
-    double inv_two_s2, double norm, size_t item_slot, int* __restrict__ err, int* __restrict__ blk_status) {
+    double inv_two_s2, double norm, size_t item_slot, int* __restrict__ blk_status) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   int* s_item = (int*)(lds_raw + item_slot);
   Real* Wb = Wbase + (long long)blockIdx.x * ws_slot;
@@ -994,10 +994,10 @@ __global__ __launch_bounds__(NW * 64, sizeof(Real) == 4 ? 3 : 2) void k_block_ch
     if (item >= nwork) break;
     if constexpr (LL)
       factor_block_ll<Real, DIM, NW, NPC>(order[item], Wb, lds_raw, bptr, bidx, ista, iend, xs, ys, zs, mptr, M,
-                                          inv_two_s2, norm, err, blk_status);
+                                          inv_two_s2, norm, blk_status);
     else
       factor_block<Real, DIM, NW, NPC>(order[item], Wb, lds_raw, bptr, bidx, ista, iend, xs, ys, zs, mptr, M,
-                                       inv_two_s2, norm, err, blk_status);
+                                       inv_two_s2, norm, blk_status);
   }
 }
 
@@ -1064,7 +1064,7 @@ static void launch_block_chol(hipStream_t st, int dim, int nw, int npc, bool ll,
                               int* next,
                               int nwg, long long ws_slot, int nmax, int tmax, const long long* bptr, const int* bidx,
                               const int* ista, const int* iend, const double* xs, const double* ys, const double* zs,
-                              T* W, const long long* mptr, T* M, double sigma, int* err, int* blk_status) {
+                              T* W, const long long* mptr, T* M, double sigma, int* blk_status) {
   const void* fn = chol_fn<T>(dim, nw, npc, ll);
   if (!fn) throw ApiError(PRBF_ERR_ARG, "no block factorisation kernel for this launch configuration");
   Kern k = make_kern(sigma);
@@ -1074,7 +1074,7 @@ static void launch_block_chol(hipStream_t st, int dim, int nw, int npc, bool ll,
   PRBF_HIP(hipMemsetAsync(next, 0, sizeof(int), st));
   double inv_two_s2 = k.inv_two_s2, norm = k.norm;
   void* args[] = {&nwork, &order, &next, &ws_slot, &bptr, &bidx,       &ista, &iend, &xs,  &ys,
-                  &zs,    &W,     &mptr, &M,       &inv_two_s2, &norm, &item_slot, &err, &blk_status};
+                  &zs,    &W,     &mptr, &M,       &inv_two_s2, &norm, &item_slot, &blk_status};
   PRBF_HIP(hipLaunchKernel(fn, dim3((unsigned)nwg), dim3(nw * 64), args, lds, st));
 }
 
@@ -1137,9 +1137,9 @@ int block_chol_workgroups(bool fp32, int dim, int nw, int npc, bool ll, int nmax
 void block_chol_fp64(hipStream_t st, int dim, int nw, int npc, bool ll, long long nwork, const int* order, int* next,
                      int nwg, long long ws_slot, int nmax, int tmax, const long long* bptr, const int* bidx,
                      const int* ista, const int* iend, const double* xs, const double* ys, const double* zs, double* W,
-                     const long long* mptr, double* M, double sigma, int* err) {
+                     const long long* mptr, double* M, double sigma, int* blk_status) {
   launch_block_chol<double>(st, dim, nw, npc, ll, nwork, order, next, nwg, ws_slot, nmax, tmax, bptr, bidx, ista, iend,
-                            xs, ys, zs, W, mptr, M, sigma, err, nullptr);
+                            xs, ys, zs, W, mptr, M, sigma, blk_status);
 }
 
 void block_chol_fp32(hipStream_t st, int dim, int nw, int npc, bool ll, long long nwork, const int* order, int* next,
@@ -1147,7 +1147,7 @@ void block_chol_fp32(hipStream_t st, int dim, int nw, int npc, bool ll, long lon
                      const int* ista, const int* iend, const double* xs, const double* ys, const double* zs, float* W,
                      const long long* mptr, float* M, double sigma, int* blk_status) {
   launch_block_chol<float>(st, dim, nw, npc, ll, nwork, order, next, nwg, ws_slot, nmax, tmax, bptr, bidx, ista, iend,
-                           xs, ys, zs, W, mptr, M, sigma, nullptr, blk_status);
+                           xs, ys, zs, W, mptr, M, sigma, blk_status);
 }
 
 }  // namespace prbf

@@ -164,15 +164,23 @@ __global__ void k_cell_scatter(long long n, const int* __restrict__ cell, int* _
   perm[s] = (int)p;
 }
 
-__global__ void k_cell_sortseg(long long ncell, const long long* __restrict__ start, int* __restrict__ perm) {
-  long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+// One wavefront per cell: each particle's place in its cell is the number of
+// the cell's particles with a smaller original index (indices are distinct),
+// read from the unordered slots `slot` and written to `perm`.  k^2/64 steps
+// per cell of k particles on one wavefront, so a crowded cell costs what its
+// size says and no more.
+__global__ __launch_bounds__(256) void k_cell_sortseg(long long ncell, const long long* __restrict__ start,
+                                                      const int* __restrict__ slot, int* __restrict__ perm) {
+  const int lane = threadIdx.x & 63;
+  const long long c = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (c >= ncell) return;
-  long long a = start[c], b = start[c + 1];
-  for (long long i = a + 1; i < b; ++i) {
-    int v = perm[i];
-    long long j = i - 1;
-    while (j >= a && perm[j] > v) { perm[j + 1] = perm[j]; --j; }
-    perm[j + 1] = v;
+  const long long a = start[c], b = start[c + 1];
+  for (long long i0 = a; i0 < b; i0 += 64) {
+    const long long i = i0 + lane;
+    const int v = i < b ? slot[i] : 0;
+    long long rank = 0;
+    for (long long j = a; j < b; ++j) rank += slot[j] < v;
+    if (i < b) perm[a + rank] = v;
   }
 }
 
@@ -195,21 +203,39 @@ __global__ void k_gather_sorted(long long n, const int* __restrict__ perm, const
   if (zs) zs[s] = z[p];
 }
 
-void sort_points(hipStream_t st, const Grid& g, long long n, const double* x, const double* y, const double* z,
-                 int* d_cell, int* d_count, long long* d_start, int* d_cursor, int* d_perm,
-                 int* d_sta, int* d_end, double* xs, double* ys, double* zs, int* d_err) {
-  int nbp = (int)((n + 255) / 256);
-  int nbc = (int)((g.ncell + 255) / 256);
+// Stable counting sort of n particles by key[p] in [0, nkey) with the
+// per-key counts already in d_count: key offsets (d_start, nkey + 1), the
+// permutation (sorted position -> original index), key ranges sta/end and the
+// sorted coordinates.  d_slot is n ints of scratch.
+void sort_by_key(hipStream_t st, long long nkey, long long n, const int* key, const int* d_count, long long* d_start,
+                 int* d_cursor, int* d_slot, int* d_perm, int* d_sta, int* d_end, const double* x, const double* y,
+                 const double* z, double* xs, double* ys, double* zs) {
+  const int nbp = (int)((n + 255) / 256);
+  const int nbc = (int)((nkey + 255) / 256);
+  scan_excl(st, nkey, d_count, d_start);
+  hipLaunchKernelGGL(k_copy_ll_to_int, dim3(nbc), dim3(256), 0, st, nkey, d_start, d_cursor);
+  // slots of rejected points stay 0 (a valid index) until the caller reports the error
+  PRBF_HIP(hipMemsetAsync(d_slot, 0, sizeof(int) * n, st));
+  PRBF_HIP(hipMemsetAsync(d_perm, 0, sizeof(int) * n, st));
+  hipLaunchKernelGGL(k_cell_scatter, dim3(nbp), dim3(256), 0, st, n, key, d_cursor, d_slot);
+  hipLaunchKernelGGL(k_cell_sortseg, dim3((unsigned)((nkey + 3) / 4)), dim3(256), 0, st, nkey, d_start, d_slot,
+                     d_perm);
+  hipLaunchKernelGGL(k_cell_ranges, dim3(nbc), dim3(256), 0, st, nkey, d_start, d_sta, d_end);
+  hipLaunchKernelGGL(k_gather_sorted, dim3(nbp), dim3(256), 0, st, n, d_perm, x, y, z, xs, ys, zs);
+}
+
+void cell_assign(hipStream_t st, const Grid& g, long long n, const double* x, const double* y, const double* z,
+                 int* d_cell, int* d_count, int* d_err) {
+  const int nbp = (int)((n + 255) / 256);
   PRBF_HIP(hipMemsetAsync(d_count, 0, sizeof(int) * g.ncell, st));
   hipLaunchKernelGGL(k_cell_assign, dim3(nbp), dim3(256), 0, st, n, x, y, z, g, d_cell, d_count, d_err);
-  scan_excl(st, g.ncell, d_count, d_start);
-  hipLaunchKernelGGL(k_copy_ll_to_int, dim3(nbc), dim3(256), 0, st, g.ncell, d_start, d_cursor);
-  // slots of rejected points stay 0 (a valid index) until the caller reports the error
-  PRBF_HIP(hipMemsetAsync(d_perm, 0, sizeof(int) * n, st));
-  hipLaunchKernelGGL(k_cell_scatter, dim3(nbp), dim3(256), 0, st, n, d_cell, d_cursor, d_perm);
-  hipLaunchKernelGGL(k_cell_sortseg, dim3(nbc), dim3(256), 0, st, g.ncell, d_start, d_perm);
-  hipLaunchKernelGGL(k_cell_ranges, dim3(nbc), dim3(256), 0, st, g.ncell, d_start, d_sta, d_end);
-  hipLaunchKernelGGL(k_gather_sorted, dim3(nbp), dim3(256), 0, st, n, d_perm, x, y, z, xs, ys, zs);
+}
+
+void sort_points(hipStream_t st, const Grid& g, long long n, const double* x, const double* y, const double* z,
+                 int* d_cell, int* d_count, long long* d_start, int* d_cursor, int* d_slot, int* d_perm,
+                 int* d_sta, int* d_end, double* xs, double* ys, double* zs, int* d_err) {
+  cell_assign(st, g, n, x, y, z, d_cell, d_count, d_err);
+  sort_by_key(st, g.ncell, n, d_cell, d_count, d_start, d_cursor, d_slot, d_perm, d_sta, d_end, x, y, z, xs, ys, zs);
 }
 
 __global__ void k_copy_ll_to_int(long long n, const long long* __restrict__ a, int* __restrict__ b) {

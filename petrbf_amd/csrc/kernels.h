@@ -10,8 +10,15 @@ void bbox(hipStream_t s, long long n, const double* x, const double* y, const do
           double* d_part, double* d_out);
 void scan_excl(hipStream_t s, long long n, const int* in, long long* out);
 void sort_points(hipStream_t st, const Grid& g, long long n, const double* x, const double* y, const double* z,
-                 int* d_cell, int* d_count, long long* d_start, int* d_cursor, int* d_perm,
+                 int* d_cell, int* d_count, long long* d_start, int* d_cursor, int* d_slot, int* d_perm,
                  int* d_sta, int* d_end, double* xs, double* ys, double* zs, int* d_err);
+// cell of each particle (-1 and *d_err set when outside the grid) and the counts per cell
+void cell_assign(hipStream_t st, const Grid& g, long long n, const double* x, const double* y, const double* z,
+                 int* d_cell, int* d_count, int* d_err);
+// stable counting sort by a precomputed key with its counts (see cells.hip)
+void sort_by_key(hipStream_t st, long long nkey, long long n, const int* key, const int* d_count, long long* d_start,
+                 int* d_cursor, int* d_slot, int* d_perm, int* d_sta, int* d_end, const double* x, const double* y,
+                 const double* z, double* xs, double* ys, double* zs);
 void list_count(hipStream_t st, const Grid& g, int reach, double len, const int* tsta, const int* tend,
                 const int* sta, const int* end, const double* xs, const double* ys, const double* zs,
                 int* count);
@@ -24,6 +31,21 @@ void lists2_count(hipStream_t st, const Grid& g, const int* sta, const int* end,
                   const double* zs, int* count_b, int* count_t);
 void lists2_fill(hipStream_t st, const Grid& g, const int* sta, const int* end, const double* xs, const double* ys,
                  const double* zs, const long long* ptr_b, int* idx_b, const long long* ptr_t, int* idx_t);
+
+// tree.hip (adaptive cluster structure) -------------------------------------
+// counts of the particles whose node lies in the frontier [lo, hi)
+void tree_count(hipStream_t st, long long n, const int* node, int lo, int hi, int* cnt);
+// particles of split frontier nodes (cbase >= 0) move to their child
+void tree_descend(hipStream_t st, int dim, long long n, const double* x, const double* y, const double* z, int* node,
+                  int lo, int hi, const int* cbase, const double* cx, const double* cy, const double* cz);
+// key[p] = leaf_of[node[p]], with the counts per leaf
+void tree_leaf_keys(hipStream_t st, long long n, long long nleaf, const int* node, const int* leaf_of, int* key,
+                    int* cnt);
+// count (fill false: cb, ct) or fill (pb/ib, pt/it) both lists of every leaf
+void tree_lists(hipStream_t st, const Grid& g, bool fill, long long nleaf, const int* sta, const int* end,
+                const int* c0sta, const int* c0end, const int* lcell, const double* lcx, const double* lcy,
+                const double* lcz, const double* lbl, const double* ltl, const double* xs, const double* ys,
+                const double* zs, int* cb, int* ct, const long long* pb, int* ib, const long long* pt, int* it);
 
 // matvec.hip ------------------------------------------------------------------
 void matvec(hipStream_t st, int dim, int nrhs, long long ncell, const int* tsta, const int* tend,
@@ -68,7 +90,8 @@ void dense_block(hipStream_t st, int dim, long long b0, int n, const int* bidx, 
 // left-looking kernels).
 long long block_workspace(int n, int T);
 size_t block_lds_bytes(bool fp32, int nmax, int tmax, int nw, int npc, bool ll);
-// Clusters of order[0..nwork) with status[c] != 0 into list (unordered), their number in *count.
+// Clusters of order[0..nwork) with status[c] != 0 (blocks whose factorisation
+// regularised a pivot) into list (unordered), their number in *count.
 void failed_blocks(hipStream_t st, long long nwork, const int* order, const int* status, int* list, int* count);
 // Whether the block factorisation has a kernel for (precision, wavefronts,
 // panel columns, order).
@@ -81,7 +104,7 @@ int block_chol_workgroups(bool fp32, int dim, int nw, int npc, bool ll, int nmax
 void block_chol_fp64(hipStream_t st, int dim, int nw, int npc, bool ll, long long nwork, const int* order, int* next,
                      int nwg, long long ws_slot, int nmax, int tmax, const long long* bptr, const int* bidx,
                      const int* ista, const int* iend, const double* xs, const double* ys, const double* zs, double* W,
-                     const long long* mptr, double* M, double sigma, int* err);
+                     const long long* mptr, double* M, double sigma, int* blk_status);
 void block_chol_fp32(hipStream_t st, int dim, int nw, int npc, bool ll, long long nwork, const int* order, int* next,
                      int nwg, long long ws_slot, int nmax, int tmax, const long long* bptr, const int* bidx,
                      const int* ista, const int* iend, const double* xs, const double* ys, const double* zs, float* W,

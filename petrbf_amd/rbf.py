@@ -49,7 +49,7 @@ class SolveInfo:
     t_pc_s: float
     n_matvec: int
     n_pc: int
-    n_fallback: int
+    n_regularised: int
     hist: np.ndarray
     n_network: int = 0
     n_net_fallback: int = 0
@@ -108,6 +108,31 @@ class Context:
         nj = 0 if xj is None else xj.shape[0]
         self._check(self._L.prbf_set_points_dev(self._h, ni, _p(xi), _p(yi), _p(zi), nj, _p(xj), _p(yj), _p(zj)))
         self._after_points()
+
+    def set_points_adaptive(self, x, y, z=None, cap=16, max_depth=6):
+        """The adaptive cluster structure over one point set: grid cells of
+        more than `cap` particles split recursively down to `max_depth`; the
+        leaves are the clusters of every later call."""
+        x, y, z = (_f64(a) for a in (x, y, z))
+        self._check(self._L.prbf_set_points_adaptive(self._h, x.shape[0], _p(x), _p(y), _p(z), int(cap),
+                                                     int(max_depth)))
+        self._after_points()
+
+    def set_points_adaptive_dev(self, x, y, z=None, cap=16, max_depth=6):
+        self._check(self._L.prbf_set_points_adaptive_dev(self._h, x.shape[0], _p(x), _p(y), _p(z), int(cap),
+                                                         int(max_depth)))
+        self._after_points()
+
+    def leaves(self):
+        """Per leaf: centre, depth, level-0 cell, buffer and truncation half-widths."""
+        n = self.ncell
+        xc, yc, zc = np.zeros(n), np.zeros(n), np.zeros(n)
+        depth = np.zeros(n, np.int32)
+        cell0 = np.zeros(n, np.int32)
+        blen, tlen = np.zeros(n), np.zeros(n)
+        self._check(self._L.prbf_get_leaves(self._h, _p(xc), _p(yc), _p(zc), _p(depth), _p(cell0), _p(blen),
+                                            _p(tlen)))
+        return dict(xc=xc, yc=yc, zc=zc, depth=depth, cell0=cell0, blen=blen, tlen=tlen)
 
     def _after_points(self):
         info = self.info()
@@ -277,7 +302,7 @@ class Context:
     def _info(self, inf, hist):
         n = min(hist.shape[0], inf.its + 64)
         return SolveInfo(inf.its, inf.reason, inf.rnorm0, inf.rnorm, inf.t_setup_s, inf.t_solve_s,
-                         inf.t_matvec_s, inf.t_pc_s, inf.n_matvec, inf.n_pc, inf.n_fallback, hist[:n],
+                         inf.t_matvec_s, inf.t_pc_s, inf.n_matvec, inf.n_pc, inf.n_regularised, hist[:n],
                          inf.n_network, inf.n_net_fallback)
 
     def solve(self, w, ksp="gmres", pc="rasm", rtol=1e-10, restart=30, max_it=1000, graph=True):

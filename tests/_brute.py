@@ -128,3 +128,96 @@ def dense_truncated(dim, sigma, L):
         r2 = sum((ts[d][rows, None] - ss[d][None, tl]) ** 2 for d in range(dim))
         A[np.ix_(rows, tl)] = np.exp(-r2 / (2 * s2)) / (2 * np.pi * s2)
     return A
+
+
+def tree_lists(dim, sigma, nsigma_box, sigma_buffer, sigma_trunc, pts, cap, dmax):
+    """The adaptive structure by its definition, for tiny inputs: nodes are
+    (grid cell, path of child digits); a node with more than `cap` particles
+    below depth `dmax` is split (child digit 2 qx + qy, 3D 4 qz + 2 qx + qy,
+    q = coordinate >= node centre); leaves (empty ones included) in the
+    lexicographic order of (cell, path); particles by (leaf, original index);
+    a leaf's list is its own particles, then every particle of the level-0
+    cells within reach of its cell except its own, ordered by (cell x, y, z,
+    sorted position), that passes the per-axis box test around the leaf
+    centre with the depth's half-width."""
+    g = grid(dim, sigma, nsigma_box, sigma_buffer, sigma_trunc, [[p] for p in pts])
+    n = pts[0].shape[0]
+    ix, iy, iz = cell_ijk(g, pts)
+    cell = cell_id(g, ix, iy, iz, dim)
+    ncell = g["nc"][0] * g["nc"][1] * g["nc"][2]
+    path = [() for _ in range(n)]
+    centres = {}
+    for c in range(ncell):
+        _, ctr = centre(g, c, dim)
+        centres[(c, ())] = ctr
+    leaves = []
+    frontier = [(c, ()) for c in range(ncell)]
+    depth = 0
+    while frontier:
+        members = {}
+        for p in range(n):
+            members.setdefault((int(cell[p]), path[p]), []).append(p)
+        nxt = []
+        for node in frontier:
+            mem = members.get(node, [])
+            if len(mem) <= cap or depth >= dmax:
+                leaves.append(node)
+                continue
+            ctr = centres[node]
+            q = math.ldexp(g["box"], -(depth + 2))
+            for k in range(1 << dim):
+                qx, qy, qz = (k >> 1) & 1, k & 1, (k >> 2) & 1
+                cc = [ctr[0] + (q if qx else -q), ctr[1] + (q if qy else -q),
+                      (ctr[2] + (q if qz else -q)) if dim == 3 else 0.0]
+                child = (node[0], node[1] + (k,))
+                centres[child] = cc
+                nxt.append(child)
+            for p in mem:
+                qx = int(pts[0][p] >= ctr[0])
+                qy = int(pts[1][p] >= ctr[1])
+                qz = int(dim == 3 and pts[2][p] >= ctr[2])
+                path[p] = path[p] + ((4 * qz + 2 * qx + qy) if dim == 3 else (2 * qx + qy),)
+        frontier = nxt
+        depth += 1
+    leaves.sort()
+    rank = {node: i for i, node in enumerate(leaves)}
+    leaf_of = np.array([rank[(int(cell[p]), path[p])] for p in range(n)])
+    perm = np.lexsort((np.arange(n), leaf_of)).astype(np.int32)
+    nl = len(leaves)
+    counts = np.bincount(leaf_of, minlength=nl)
+    start = np.concatenate([[0], np.cumsum(counts)])
+    ista, iend = start[:-1].astype(np.int32), (start[1:] - 1).astype(np.int32)
+    sp = [p[perm] for p in pts]
+    scell = cell[perm]
+    nx, ny = g["nc"][0], g["nc"][1]
+    if dim == 2:
+        sx, sy, sz = scell // ny, scell % ny, np.zeros_like(scell)
+    else:
+        sz, r = np.divmod(scell, nx * ny)
+        sx, sy = np.divmod(r, ny)
+    tlen0 = g["tlen"]
+    out_b, out_t, geo = [], [], []
+    for l, (c, pth) in enumerate(leaves):
+        d = len(pth)
+        blen = math.ldexp(g["blen"], -d)
+        tlen = tlen0 if d == 0 else (tlen0 - g["box"] * 0.5) + math.ldexp(g["box"], -(d + 1))
+        ctr = centres[(c, pth)]
+        geo.append((ctr, d, c, blen, tlen))
+        (cx, cy, cz), _ = centre(g, c, dim)
+        own = np.arange(ista[l], iend[l] + 1)
+        lists_ = []
+        for reach, length in ((g["reach_b"], blen), (g["reach_t"], tlen)):
+            if own.size == 0:
+                lists_.append(np.zeros(0, np.int32))
+                continue
+            near = (np.abs(sx - cx) <= reach) & (np.abs(sy - cy) <= reach) & (np.abs(sz - cz) <= reach)
+            near[own] = False
+            inbox = np.ones(n, bool)
+            for dd in range(dim):
+                inbox &= np.abs(sp[dd] - ctr[dd]) < length
+            sel = np.nonzero(near & inbox)[0]
+            order = np.lexsort((sel, sz[sel], sy[sel], sx[sel]))
+            lists_.append(np.concatenate([own, sel[order]]).astype(np.int32))
+        out_b.append(lists_[0])
+        out_t.append(lists_[1])
+    return dict(perm=perm, ista=ista, iend=iend, buffer=out_b, trunc=out_t, geo=geo, grid=g)
