@@ -101,8 +101,8 @@ typedef struct {
   int n_matvec;
   int n_pc;
   int n_regularised; /* RASM blocks whose Cholesky met a pivot at or below tau times the
-                        diagonal 1/(2 pi sigma^2) and used that floor instead (tau = 2^-20
-                        for FP32 blocks, 2^-46 for FP64): numerically singular blocks of
+                        diagonal 1/(2 pi sigma^2) and used that floor instead (tau = 2^-14
+                        for FP32 blocks, 2^-40 for FP64): numerically singular blocks of
                         clustered points; 0 on well-separated points */
   int n_network;     /* LEARNED: clusters served by the network at the end of the solve */
   int n_net_fallback; /* LEARNED: clusters the solve moved from the network to exact blocks */

@@ -34,12 +34,15 @@
 // T = double is the plain path (exact FP64 block solves); T = float runs the
 // same kernel in FP32 on the FP32 MFMA (mixed precision).  The factorisation
 // is a modified Cholesky: a pivot at or below tau times the Gaussian's
-// diagonal 1/(2 pi sigma^2) (tau = 2^-20 in FP32, 2^-46 in FP64, a few
-// hundred units of rounding) is replaced by that floor, the block's status is
+// diagonal 1/(2 pi sigma^2) is replaced by that floor, the block's status is
 // set, and the block is finished like any other, so numerically singular
-// blocks (clustered points) still give a preconditioner.  Well-separated
-// blocks never reach the floor: their pivots stay near the smallest
-// eigenvalue, orders of magnitude above it.
+// blocks (clustered points) still give a preconditioner.  tau = 2^-14 in
+// FP32 and 2^-40 in FP64 lies above the rounding noise of a block's Schur
+// complements (~n eps of the diagonal for n <= ~800), so every multiplier of
+// a replaced pivot stays O(1) (|s_ip|^2 <= s_ii s_pp + noise), and below the
+// smallest pivots of well-separated points (lattice blocks at h/sigma = 0.8,
+// 2D and 3D, and the reference's cylinder input: >= 2.4e-3 of the diagonal),
+// which therefore never reach it.
 #include <mutex>
 #include <vector>
 
@@ -180,10 +183,10 @@ __device__ __forceinline__ float pivot_rsqrt(float x) { return __builtin_amdgcn_
 __device__ __forceinline__ double pivot_rsqrt(double x) { return rsqrt(x); }
 
 // Smallest pivot the modified Cholesky keeps: tau times the Gaussian's
-// diagonal norm = 1/(2 pi sigma^2), tau = 2^-20 (FP32) or 2^-46 (FP64).
+// diagonal norm = 1/(2 pi sigma^2), tau = 2^-14 (FP32) or 2^-40 (FP64).
 template <typename Real>
 __device__ __forceinline__ Real pivot_floor(double norm) {
-  return sizeof(Real) == 4 ? (Real)(norm * 0x1p-20) : (Real)(norm * 0x1p-46);
+  return sizeof(Real) == 4 ? (Real)(norm * 0x1p-14) : (Real)(norm * 0x1p-40);
 }
 
 // Step P of the symmetric elimination on [D | I]: every 16-lane row of the

@@ -142,7 +142,7 @@ def test_mixture_blocks_are_regularised_and_solve_stays_finite():
                   f"residual {h[-1] / h[0]:.3e}")
         assert inf.n_regularised > 0
         assert np.isfinite(g).all() and np.isfinite(h).all()
-        assert (np.diff(h[1:31]) <= 1e-12 * h[0]).all()  # GMRES's estimate within a cycle never grows
+        assert (np.diff(h[:31]) <= 1e-12 * h[0]).all()  # the estimate never grows within the first cycle
     ctx.close()
 
 
