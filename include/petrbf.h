@@ -101,9 +101,12 @@ typedef struct {
   int n_matvec;
   int n_pc;
   int n_regularised; /* RASM blocks whose Cholesky met a pivot at or below tau times the
-                        diagonal 1/(2 pi sigma^2) and used that floor instead (tau = 2^-14
-                        for FP32 blocks, 2^-40 for FP64): numerically singular blocks of
-                        clustered points; 0 on well-separated points */
+                        diagonal 1/(2 pi sigma^2) (tau = 2^-14 for FP32 blocks, 2^-40 for
+                        FP64) and were factorised again as A_c + s I, s = 2^-12, 2^-8,
+                        2^-4 (FP32) or 2^-36, 2^-28, 2^-20 (FP64) times the diagonal,
+                        until no pivot did (after the last, the floor replaces such
+                        pivots): numerically singular blocks of clustered points; 0 on
+                        well-separated points */
   int n_network;     /* LEARNED: clusters served by the network at the end of the solve */
   int n_net_fallback; /* LEARNED: clusters the solve moved from the network to exact blocks */
 } prbf_solve_info;

@@ -32,11 +32,12 @@
 // instead of being re-read and re-written by every trailing update, and the
 // entries of A' are made where they enter instead of being stored.
 // T = double is the plain path (exact FP64 block solves); T = float runs the
-// same kernel in FP32 on the FP32 MFMA (mixed precision).  The factorisation
-// is a modified Cholesky: a pivot at or below tau times the Gaussian's
-// diagonal 1/(2 pi sigma^2) is replaced by that floor, the block's status is
-// set, and the block is finished like any other, so numerically singular
-// blocks (clustered points) still give a preconditioner.  tau = 2^-14 in
+// same kernel in FP32 on the FP32 MFMA (mixed precision).  Numerically
+// singular blocks (clustered points) still give a preconditioner: a pivot at
+// or below tau times the Gaussian's diagonal 1/(2 pi sigma^2) is replaced by
+// that floor and the block is factorised again with a growing diagonal shift
+// (diag_shift) until no pivot reaches the floor; its status records the
+// attempt.  tau = 2^-14 in
 // FP32 and 2^-40 in FP64 lies above the rounding noise of a block's Schur
 // complements (~n eps of the diagonal for n <= ~800), so every multiplier of
 // a replaced pivot stays O(1) (|s_ip|^2 <= s_ii s_pp + noise), and below the
@@ -187,6 +188,18 @@ __device__ __forceinline__ double pivot_rsqrt(double x) { return rsqrt(x); }
 template <typename Real>
 __device__ __forceinline__ Real pivot_floor(double norm) {
   return sizeof(Real) == 4 ? (Real)(norm * 0x1p-14) : (Real)(norm * 0x1p-40);
+}
+
+// Diagonal shift of factorisation attempt a (0: none): a block whose pivots
+// reach the floor is factorised again as A' + shift I (Cholesky with an added
+// multiple of the identity, Nocedal & Wright, Numerical Optimization,
+// Alg. 3.3), shift = norm 2^-12, 2^-8, 2^-4 in FP32 and norm 2^-36, 2^-28,
+// 2^-20 in FP64; after the last attempt the floor stays in place.
+constexpr int kShiftLevels = 4;
+template <typename Real>
+__device__ __forceinline__ Real diag_shift(double norm, int attempt) {
+  if (attempt == 0) return Real(0);
+  return sizeof(Real) == 4 ? (Real)ldexp(norm, -16 + 4 * attempt) : (Real)ldexp(norm, -44 + 8 * attempt);
 }
 
 // Step P of the symmetric elimination on [D | I]: every 16-lane row of the
@@ -517,6 +530,10 @@ __device__ void factor_block(
   Real* Li = D;
   int* flag = (int*)(D + kLT);
 
+  // phases 0 and 1 once more, on A' + shift I, while a pivot reaches the floor
+  // (the panel overwrites the coordinates, so phase 0 is redone as well)
+  for (int attempt = 0;; ++attempt) {
+  const Real dshift = diag_shift<Real>(norm, attempt);
   // ---- phase 0: coordinates (core last) and the lower tiles of A'
   for (int p = tid; p < n; p += NW * 64) {
     int b = p < ncore0 ? p + T : p - ncore0;
@@ -536,10 +553,12 @@ __device__ void factor_block(
     for (int r = 0; r < 4; ++r) {
       const int i = 16 * I + MF<Real>::row(r, lq);
       Real v;
-      if (i < n && j < n)
+      if (i < n && j < n) {
         v = gauss_entry<Real>(cx[i] - cx[j], cy[i] - cy[j], DIM == 3 ? cz[i] - cz[j] : 0.0, inv_two_s2, norm);
-      else
+        if (i == j) v += dshift;
+      } else {
         v = (i == j) ? Real(1) : Real(0);
+      }
       tile[r * 64 + lane] = v;
     }
   }
@@ -669,7 +688,14 @@ __device__ void factor_block(
     }
     __syncthreads();
   }
-  if (tid == 0 && *flag) blk_status[c] = 1;  // regularised pivots (read after the last barrier)
+  // (the flag is final after the last panel's barrier; `again` is uniform)
+  const bool flagged = *flag != 0, again = flagged && attempt + 1 < kShiftLevels;
+  __syncthreads();
+  if (!again) {
+    if (tid == 0 && (attempt > 0 || flagged)) blk_status[c] = attempt + (flagged ? 4 : 0);
+    break;
+  }
+  }
   core_rows<Real, NW, false>(c, Wb, Wd, lds_raw, n, T, mptr, M);
 }
 
@@ -769,9 +795,13 @@ __device__ void factor_block_ll(
     cy[p] = ys[j];
     cz[p] = DIM == 3 ? zs[j] : 0.0;
   }
-  if (tid == 0) *flag = 0;
   __syncthreads();
 
+  // phase 1 once more, on A' + shift I, while a pivot reaches the floor
+  for (int attempt = 0;; ++attempt) {
+  const Real dshift = diag_shift<Real>(norm, attempt);
+  if (tid == 0) *flag = 0;
+  __syncthreads();
   // ---- phase 1: left-looking Cholesky over panels of NPC tile columns
   for (int k0 = 0; k0 < KT; k0 += NPC) {
     const int npc = min(NPC, KT - k0);
@@ -799,11 +829,13 @@ __device__ void factor_block_ll(
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int jj = 16 * q2 + MF<Real>::row(q, lq), j = 16 * k0 + jj;
-        if (i < n && j < n)
+        if (i < n && j < n) {
           v[q] = gauss_entry<Real>(xi - Cc[jj], yi - Cc[16 * NPC + jj], DIM == 3 ? zi - Cc[32 * NPC + jj] : 0.0,
                                    inv_two_s2, norm);
-        else
+          if (i == j) v[q] += dshift;
+        } else {
           v[q] = (i == j) ? Real(1) : Real(0);
+        }
       }
       return v;
     };
@@ -968,7 +1000,14 @@ __device__ void factor_block_ll(
     }
     __syncthreads();
   }
-  if (tid == 0 && *flag) blk_status[c] = 1;  // regularised pivots (read after the last barrier)
+  // (the flag is final after the last panel's barrier; `again` is uniform)
+  const bool flagged = *flag != 0, again = flagged && attempt + 1 < kShiftLevels;
+  __syncthreads();
+  if (!again) {
+    if (tid == 0 && (attempt > 0 || flagged)) blk_status[c] = attempt + (flagged ? 4 : 0);
+    break;
+  }
+  }
   core_rows<Real, NW, true>(c, Wb, Wd, lds_raw, n, T, mptr, M);
 }
 
