@@ -9,8 +9,9 @@
 // (T x n, stored as apply_offset lays it out) and the apply is a batched GEMV
 // over M_c.
 //
-//   phase 0  order the buffer with the core LAST and padding (identity) up to
-//            NP = 16*ceil(n/16); write the lower tiles of A' to a workspace in
+//   phase 0  order the buffer with the core LAST and padding (the diagonal
+//            1/(2 pi sigma^2) alone) up to NP = 16*ceil(n/16); write the lower
+//            tiles of A' to a workspace in
 //            the MFMA accumulator layout (one coalesced access per register);
 //   phase 1  right-looking tiled Cholesky A' = L L^T over panels of NPC tile
 //            columns: per column, the 16x16 diagonal tile by one wavefront
@@ -557,7 +558,7 @@ __device__ void factor_block(
         v = gauss_entry<Real>(cx[i] - cx[j], cy[i] - cy[j], DIM == 3 ? cz[i] - cz[j] : 0.0, inv_two_s2, norm);
         if (i == j) v += dshift;
       } else {
-        v = (i == j) ? Real(1) : Real(0);
+        v = (i == j) ? (Real)norm : Real(0);  // padding: decoupled, at the Gaussian's diagonal
       }
       tile[r * 64 + lane] = v;
     }
@@ -822,7 +823,7 @@ __device__ void factor_block_ll(
     __syncthreads();
     // this lane's entries of tile (I, k0 + q2) of A' as the accumulator of
     // its transpose, A'(16 I + lr, 16 (k0 + q2) + row(q, lq)), from the row's
-    // coordinates (xi, yi, zi) and the panel columns'; identity on the padding
+    // coordinates (xi, yi, zi) and the panel columns'; the padding decoupled
     auto aprime = [&](int I, int q2, double xi, double yi, double zi) {
       const int i = 16 * I + lr;
       v4 v;
@@ -834,7 +835,7 @@ __device__ void factor_block_ll(
                                    inv_two_s2, norm);
           if (i == j) v[q] += dshift;
         } else {
-          v[q] = (i == j) ? Real(1) : Real(0);
+          v[q] = (i == j) ? (Real)norm : Real(0);  // padding: decoupled, at the Gaussian's diagonal
         }
       }
       return v;

@@ -268,9 +268,32 @@ def test_full_size_lattice_against_oracle():
         _announce(f"full size N=4M: solve {ksp}/{pc}")
         gam, inf = ctx.solve(w, ksp=ksp, pc=pc, rtol=1e-10, restart=30, max_it=1000)
         assert inf.reason == 2, inf.reason
+        assert inf.n_regularised == 0
         aw = np.empty(p.n)
         aw[pi] = o.matvec(gam[pi])
         res = np.linalg.norm(aw - w) / np.linalg.norm(w)
         _announce(f"{ksp}/{pc}: its {inf.its}, true residual with the oracle's operator {res:.3e}")
         assert res <= 1.05e-10
     ctx.close()
+
+
+@pytest.mark.parametrize("scale", [1e-3, 1e3])
+def test_lattice_at_any_scale_is_not_regularised(scale):
+    """The pivot floor is relative to the Gaussian's diagonal 1/(2 pi sigma^2)
+    and the padding of every block sits at that diagonal, so a well-separated
+    lattice is factorised exactly at any length scale (sigma 1e-3 or 1e3 times
+    the unit square's): no regularised block, and the mixed and plain solves
+    converge as on the unit square."""
+    _announce(f"lattice scaled by {scale}")
+    p = problems.lattice_2d_problem(40, seed=0x5EED0400)
+    q = problems.lattice_2d_problem(40, seed=0x5EED0400)
+    for a in ("x", "y"):
+        setattr(q, a, getattr(p, a) * scale)
+    q.sigma = p.sigma * scale
+    for prob in (p, q):
+        ctx = Context.from_problem(prob)
+        ctx.set_points(prob.x, prob.y)
+        for ksp, pc in (("fgmres", "rasm32"), ("gmres", "rasm")):
+            _, inf = ctx.solve(p.w[0], ksp=ksp, pc=pc, rtol=1e-10, restart=30, max_it=1000)
+            assert inf.n_regularised == 0 and inf.reason == 2, (scale, pc, inf.n_regularised)
+        ctx.close()
