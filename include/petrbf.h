@@ -194,6 +194,16 @@ prbf_status prbf_matvec_dev(prbf_ctx* ctx, int nrhs, const double* g_sorted, dou
 prbf_status prbf_evaluate(prbf_ctx* ctx, int nrhs, const double* gj, double* wi);
 prbf_status prbf_evaluate_dev(prbf_ctx* ctx, int nrhs, const double* gj, double* wi);
 
+/* Allocation-free evaluation: prbf_evaluate_dev with its sorted-order staging
+ * (nrhs (nj + ni) doubles) in a static device array of 2^21 doubles (16 MiB,
+ * one per GPU, held until the process ends) instead of per-call device
+ * allocations; stream-ordered, no hipMalloc, hipFree or synchronisation, so
+ * it may run under HIP graph capture.  No counterpart in the reference: it is
+ * for callers that cannot allocate (graph capture, tight serving loops).
+ * Larger inputs return PRBF_ERR_LIMIT.  All calls on one GPU share the array:
+ * calls that overlap on one GPU (several contexts or streams) race. */
+prbf_status prbf_evaluate_dev_noalloc(prbf_ctx* ctx, int nrhs, const double* gj, double* wi);
+
 /* ---- RASM preconditioner ----------------------------------------------
  * MatShell GET_SUBMATRICES callback mysubmat (2d/matmult.cxx:53-104) + the
  * PETSc dense sub-factorisation, for the subdomains set up by

@@ -28,7 +28,7 @@ EXPORTS = [
     "prbf_synchronize", "prbf_set_points", "prbf_set_points_dev", "prbf_set_points_adaptive",
     "prbf_set_points_adaptive_dev", "prbf_get_leaves", "prbf_get_info",
     "prbf_get_permutation", "prbf_get_cells", "prbf_get_buffer_csr", "prbf_get_trunc_csr",
-    "prbf_get_sorted_points", "prbf_matvec", "prbf_matvec_dev", "prbf_evaluate", "prbf_evaluate_dev",
+    "prbf_get_sorted_points", "prbf_matvec", "prbf_matvec_dev", "prbf_evaluate", "prbf_evaluate_dev", "prbf_evaluate_dev_noalloc",
     "prbf_rasm_setup", "prbf_rasm_setup_pc", "prbf_set_tuning", "prbf_rasm_apply", "prbf_rasm_apply_dev", "prbf_rasm_get_core_rows",
     "prbf_rasm_get_block",
     "prbf_solve", "prbf_solve_dev", "prbf_solve_multi", "prbf_solve_multi_dev", "prbf_rbf_interpolation_f32",
@@ -102,7 +102,7 @@ def load():
     L.prbf_get_buffer_csr.argtypes = [ctx, dp, dp]
     L.prbf_get_trunc_csr.argtypes = [ctx, dp, dp]
     L.prbf_get_sorted_points.argtypes = [ctx, dp, dp, dp]
-    for f in ("prbf_matvec", "prbf_matvec_dev", "prbf_evaluate", "prbf_evaluate_dev"):
+    for f in ("prbf_matvec", "prbf_matvec_dev", "prbf_evaluate", "prbf_evaluate_dev", "prbf_evaluate_dev_noalloc"):
         getattr(L, f).argtypes = [ctx, i32, dp, dp]
     L.prbf_rasm_setup.argtypes = [ctx]
     L.prbf_rasm_setup_pc.argtypes = [ctx, i32]

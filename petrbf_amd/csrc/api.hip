@@ -102,6 +102,9 @@ struct prbf_ctx {
   std::string err;
   bool has_points = false, same = true;
   long long ni = 0, nj = 0;
+  // this context's device's copy of the static scratch array (allocation-free
+  // evaluation), resolved at prbf_create
+  double* scratch = nullptr;
   // clusters: the grid's cells, or the leaves of the adaptive structure
   // (tree), with the leaves' geometry and the grid cells' particle ranges
   long long nclus = 0;
@@ -1578,6 +1581,34 @@ void solve_dev_impl(prbf_ctx* c, const prbf_solve_opts* opts, const double* w, d
   if (info) *info = local;
 }
 
+// ------------------------------------------------ allocation-free evaluation
+// The staging of prbf_evaluate_dev_noalloc: the strengths in sorted order
+// and the field before it is scattered back, nrhs (nj + ni) doubles at most
+// kScratchDoubles (16 MiB).  One copy per GPU, held until the process ends;
+// the host reaches it only through the address hipGetSymbolAddress gives on
+// that device (the array's name is a host-side stand-in).
+constexpr long long kScratchDoubles = 1LL << 21;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+// The scratch array's address on the current device, resolved once per device
+// (with that device current) and kept; null on failure.
+static double* device_scratch() {
+  static std::mutex m;
+  static std::vector<std::pair<int, double*>> known;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lock(m);
+  for (const auto& k : known)
+    if (k.first == dev) return k.second;
+  void* ptr = nullptr;
+  if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  known.push_back({dev, (double*)ptr});
+  return (double*)ptr;
+}
+
 }  // namespace
 
 // ============================================================== C ABI
@@ -1604,14 +1635,16 @@ prbf_status prbf_create(prbf_ctx** out, const prbf_params* p) {
   prbf_ctx* c = new (std::nothrow) prbf_ctx;
   if (!c) return PRBF_ERR_NOMEM;
   c->prm = *p;
-  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
-      hipHostMalloc((void**)&c->pin, 512 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
-      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+  const bool ok = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) == hipSuccess &&
+                  hipHostMalloc((void**)&c->pin, 512 * sizeof(double), hipHostMallocDefault) == hipSuccess &&
+                  hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess;
+  c->own_stream = c->st != nullptr;
+  c->scratch = ok ? device_scratch() : nullptr;
+  if (!ok || !c->scratch) {
     (void)hipGetLastError();
-    delete c;
+    prbf_destroy(c);
     return PRBF_ERR_HIP;
   }
-  c->own_stream = true;
   *out = c;
   return PRBF_OK;
 }
@@ -1838,6 +1871,24 @@ prbf_status prbf_evaluate(prbf_ctx* c, int nrhs, const double* gj, double* wi) {
     evaluate_dev_impl(c, nrhs, dg, dw);
     PRBF_HIP(hipMemcpyAsync(wi, dw, sizeof(double) * nrhs * c->ni, hipMemcpyDeviceToHost, c->st));
     sync(c);
+  });
+}
+
+prbf_status prbf_evaluate_dev_noalloc(prbf_ctx* c, int nrhs, const double* gj, double* wi) {
+  return guarded(c, [&] {
+    need(c->has_points, PRBF_ERR_STATE, "set_points first");
+    need(nrhs >= 1 && nrhs <= 3, PRBF_ERR_ARG, "nrhs must be 1..3");
+    need(gj && wi, PRBF_ERR_ARG, "null vector");
+    need((long long)nrhs * (c->nj + c->ni) <= kScratchDoubles, PRBF_ERR_LIMIT,
+         "nrhs (nj + ni) exceeds the 2^21 doubles of the allocation-free scratch; use prbf_evaluate_dev");
+    double* gs = c->scratch;  // the device address resolved at prbf_create, never the array's name
+    double* ws = gs + (long long)nrhs * c->nj;
+    for (int r = 0; r < nrhs; ++r)
+      gather(c->st, c->nj, c->perm_j_(), gj + (long long)r * c->nj, gs + (long long)r * c->nj);
+    matvec_impl(c, nrhs, gs, ws);
+    for (int r = 0; r < nrhs; ++r)
+      scatter(c->st, c->ni, c->perm_i, ws + (long long)r * c->ni, wi + (long long)r * c->ni);
+    PRBF_HIP(hipGetLastError());
   });
 }
 
