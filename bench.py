@@ -29,6 +29,20 @@ PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 PEAK_FP32_MFMA_TFLOPS = 157.3  # MI355X dense FP32 matrix
 PEAK_HBM_GBS = 8000.0     # MI355X HBM3E
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "block_chol_fp32_n4m_hbm_traffic.txt")
+CPU_VALIDATION_FILE = os.path.join(ROOT, "profiles", "cpu_baseline_validation.txt")
+
+
+def cpu_validation():
+    """The host-core baseline's estimator checked against full oracle solves
+    (the committed summary of `bench.py --cpu-validate` on the GPU box's host),
+    or None when the summary is absent."""
+    try:
+        for line in open(CPU_VALIDATION_FILE):
+            if line.startswith("{"):
+                return json.loads(line)
+    except (OSError, ValueError):
+        pass
+    return None
 
 
 def measured_traffic(kernel_key):
@@ -92,6 +106,33 @@ def cpu_baseline(p, its, pairs, stride=25, n_mv=3):
     }
 
 
+def cpu_validate():
+    """Full host-core solves (lists, LU of every block, GMRES(30) to rtol
+    1e-10) at two sizes against cpu_baseline's estimate from its bounded
+    sample at the same sizes, with the estimate's relative error."""
+    from oracle.oracle import Oracle, num_threads
+    from petrbf_amd import problems
+    rows = []
+    for n_side in (500, 1000):
+        p = problems.lattice_2d_problem(n_side, seed=0x5EED0001)
+        t0 = time.perf_counter()
+        o = Oracle.from_problem(p)
+        pi, _ = o.perm()
+        ref = o.solve_sorted(p.w[0][pi], rtol=1e-10, restart=30, max_it=1000)
+        t_full = time.perf_counter() - t0
+        ista, iend, _, _ = o.cells()
+        tptr, _ = o.trunc_csr()
+        pairs = int(((iend - ista + 1).clip(min=0) * np.diff(tptr)).sum())
+        del o
+        est = cpu_baseline(p, ref["its"], pairs)
+        rows.append({"n": p.n, "its": ref["its"], "full_solve_s": t_full, "estimate_s": est["value"],
+                     "relative_error": est["value"] / t_full - 1.0})
+        log(f"cpu validation N={p.n}: full {t_full:.1f}s, estimate {est['value']:.1f}s "
+            f"({100 * (est['value'] / t_full - 1):+.1f}%)")
+    print(json.dumps({"cpu_baseline_validation": rows, "threads": num_threads(), "nproc": os.cpu_count(),
+                      "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,7 +141,12 @@ def main():
     ap.add_argument("--n-side", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=10)
-    ap.add_argument("--mode", choices=["plain", "mixed", "both"], default="both")
+    ap.add_argument("--mode", choices=["plain", "mixed", "both", "learned"], default="both",
+                    help="learned: the C4 leg (network trained on ~2M pairs harvested from another seeded "
+                         "lattice of the same size, then the learned FGMRES step against the mixed one); "
+                         "its own JSON line")
+    ap.add_argument("--learned-vectors", type=int, default=8, help="Krylov vectors harvested (x ncell pairs)")
+    ap.add_argument("--learned-steps", type=int, default=2000, help="Adam steps of the training")
     ap.add_argument("--chol-waves", type=int, default=0, help="4 or 8 wavefronts per block factorisation")
     ap.add_argument("--chol-panel", type=int, default=0, help="1, 2 or 4 tile columns per panel")
     ap.add_argument("--chol-order", choices=["right", "left"], default=None,
@@ -116,7 +162,12 @@ def main():
                     help="only build the workload's lists and run one RASM setup (for counter runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of 1 GiB five times (HBM counter calibration)")
+    ap.add_argument("--cpu-validate", action="store_true",
+                    help="host only: the oracle's full plain solve at N = 500^2 and 1000^2 against the sampled "
+                         "estimator of cpu_baseline at the same sizes (a few minutes); one JSON line")
     args = ap.parse_args()
+    if args.cpu_validate:
+        return cpu_validate()
     if args.copy_calibration:
         import torch
         a = torch.empty(1 << 27, dtype=torch.float64, device="cuda:0").fill_(1.0)
@@ -139,6 +190,8 @@ def main():
         return bench_c5(args, torch, problems, Context, dev)
     if args.workload == "c3":
         return bench_c3(args, torch, problems, Context, dev)
+    if args.mode == "learned":
+        return bench_learned(args, torch, problems, Context, dev)
     log(f"generating lattice {args.n_side}^2")
     p = problems.lattice_2d_problem(args.n_side, seed=0x5EED0001)
     x = torch.from_numpy(p.x).to(dev)
@@ -302,8 +355,77 @@ def main():
     if not args.no_cpu:
         log("cpu baseline (oracle, bounded sample)")
         res["cpu_baseline"] = cpu_baseline(p, inf_its, pairs)
+        res["cpu_baseline"]["validation"] = cpu_validation()
         log(f"cpu baseline est {res['cpu_baseline']['value']:.1f}s on {res['cpu_baseline']['cores']} threads")
     print(json.dumps(res), flush=True)
+
+
+def bench_learned(args, torch, problems, Context, dev):
+    """C4: the learned preconditioner on the N = n_side^2 lattice.  Pairs
+    (patch features, exact FP64 block core solve) are harvested from the
+    first Krylov vectors of an exact-block FGMRES run on another seeded
+    lattice of the same size (vectors x eligible clusters, ~2M at N = 4M),
+    the network is trained on the device (Adam, BF16 MFMA), attached to every
+    eligible cluster, and the learned FGMRES step (lists, attach, solve with
+    the per-cluster fallback) is timed against the mixed FP32-block step."""
+    from petrbf_amd.learned import Mlp, harvest
+    log(f"learned: harvesting on lattice {args.n_side}^2 (seed 0x5EED0002)")
+    pt = problems.lattice_2d_problem(args.n_side, seed=0x5EED0002)
+    ct = Context.from_problem(pt)
+    ct.set_points_dev(torch.from_numpy(pt.x).to(dev), torch.from_numpy(pt.y).to(dev))
+    t0 = time.perf_counter()
+    X, T, osc = harvest(ct, torch.from_numpy(pt.w[0]).to(dev), args.learned_vectors, torch)
+    t_harvest = time.perf_counter() - t0
+    ct.close()
+    del pt
+    net = Mlp(seed=7)
+    net.set_params(net.params()[0], osc)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = net.train_dev(X, T, batch=4096, steps=args.learned_steps, lr=1e-3, seed=3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    npairs = int(X.shape[0])
+    log(f"learned: {npairs} pairs harvested in {t_harvest:.1f}s; {args.learned_steps} Adam steps in {t_train:.1f}s, "
+        f"loss {float(hist[:20].mean()):.3f} -> {float(hist[-50:].mean()):.3f}")
+    del X, T
+    p = problems.lattice_2d_problem(args.n_side, seed=0x5EED0001)
+    x, y, w = (torch.from_numpy(a).to(dev) for a in (p.x, p.y, p.w[0]))
+    gamma, wv = torch.empty_like(w), torch.empty_like(w)
+    ctx = Context.from_problem(p)
+    out = {}
+    for mode in ("mixed", "learned"):
+        def step():
+            ctx.set_points_dev(x, y)
+            if mode == "learned":
+                n_net = ctx.learned_attach(net, 1e30)
+                return n_net, ctx.solve_dev(w, gamma, ksp="fgmres", pc="learned", rtol=1e-10, restart=30,
+                                            max_it=1000)
+            return 0, ctx.solve_dev(w, gamma, ksp="fgmres", pc="rasm32", rtol=1e-10, restart=30, max_it=1000)
+        step()  # warm-up (tuner, graphs)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        n_net, inf = step()
+        torch.cuda.synchronize()
+        t_step = time.perf_counter() - t0
+        ctx.evaluate_dev(gamma, wv)
+        torch.cuda.synchronize()
+        true_rel = float(torch.linalg.norm(wv - w) / torch.linalg.norm(w))
+        out[mode] = {"step_s": t_step, "its": inf.its, "reason": inf.reason, "t_setup_s": inf.t_setup_s,
+                     "t_solve_s": inf.t_solve_s, "true_rel_residual": true_rel}
+        if mode == "learned":
+            out[mode].update(clusters_attached=n_net, clusters_on_network_at_end=inf.n_network,
+                             clusters_moved_to_exact_blocks=inf.n_net_fallback)
+        log(f"{mode}: step {t_step:.3f}s, its {inf.its} reason {inf.reason}, true residual {true_rel:.3e}"
+            + (f", attached {n_net}, moved to exact blocks {inf.n_net_fallback}" if mode == "learned" else ""))
+    print(json.dumps({
+        "metric": "C4 learned preconditioner on the N=4M lattice: step time and iterations vs exact FP32 blocks",
+        "n": p.n, "pairs": npairs, "harvest_s": t_harvest, "train_steps": args.learned_steps, "train_s": t_train,
+        "loss_first": float(hist[:20].mean()), "loss_last": float(hist[-50:].mean()), "modes": out,
+        "gain_s_vs_mixed": out["mixed"]["step_s"] - out["learned"]["step_s"],
+        "gain_its_vs_mixed": out["mixed"]["its"] - out["learned"]["its"],
+    }), flush=True)
+    net.close()
 
 
 def bench_c3(args, torch, problems, Context, dev):
