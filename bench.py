@@ -145,7 +145,7 @@ def main():
                     help="learned: the C4 leg (network trained on ~2M pairs harvested from another seeded "
                          "lattice of the same size, then the learned FGMRES step against the mixed one); "
                          "its own JSON line")
-    ap.add_argument("--learned-vectors", type=int, default=8, help="Krylov vectors harvested (x ncell pairs)")
+    ap.add_argument("--learned-vectors", type=int, default=18, help="Krylov vectors harvested (x ncell pairs)")
     ap.add_argument("--learned-steps", type=int, default=2000, help="Adam steps of the training")
     ap.add_argument("--chol-waves", type=int, default=0, help="4 or 8 wavefronts per block factorisation")
     ap.add_argument("--chol-panel", type=int, default=0, help="1, 2 or 4 tile columns per panel")
