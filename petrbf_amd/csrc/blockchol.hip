@@ -808,10 +808,25 @@ __device__ void factor_block_ll(
     const int npc = min(NPC, KT - k0);
     // the panel's L rows left of it, row r at Rw + r k0 tiles (rows are
     // contiguous in the workspace), and its columns' coordinates
-    for (int r = 0; r < npc && !PG; ++r) {
-      const Real* src = wtile(k0 + r, 0);
-      Real* dst = Rw + (size_t)r * k0 * kTE;
-      for (int e = tid * 4; e < k0 * kTE; e += NW * 256) *(v4*)(dst + e) = *(const v4*)(src + e);
+    // (the npc rows are contiguous in the workspace and in LDS; kCopyB
+    // 16-byte loads per lane are issued before their stores, so the copy
+    // costs a few load latencies per panel instead of one per 4 KB)
+    if constexpr (!PG) {
+      constexpr int kCopyB = sizeof(Real) == 4 ? 4 : 2;
+      const int per_row = k0 * (kTE / 4), total = npc * per_row;  // in 16-byte vectors
+      for (int b0 = tid; b0 < total; b0 += NW * 64 * kCopyB) {
+        v4 t[kCopyB];
+#pragma unroll
+        for (int k = 0; k < kCopyB; ++k) {
+          const int e = b0 + k * NW * 64, r = e / max(per_row, 1);
+          if (e < total) t[k] = *(const v4*)(wtile(k0 + r, 0) + 4 * (e - r * per_row));
+        }
+#pragma unroll
+        for (int k = 0; k < kCopyB; ++k) {
+          const int e = b0 + k * NW * 64, r = e / max(per_row, 1);
+          if (e < total) *(v4*)(Rw + (size_t)r * k0 * kTE + 4 * (e - r * per_row)) = t[k];
+        }
+      }
     }
     for (int p = tid; p < 16 * npc; p += NW * 64) {
       const int gp = 16 * k0 + p;
@@ -862,22 +877,21 @@ __device__ void factor_block_ll(
         *(v4*)(pd(r, q2) + lane * 4) = acc;
       }
     }
-    // tile rows below the diagonal block, G per wavefront pass: the update
-    // from the L rows left of the panel (L_ij streamed from the workspace,
-    // the next column's tiles loaded during the current column's products),
-    // then per panel column the update from the panel's earlier columns and
-    // the TRSM, all in registers.  The first pass streams its products while
-    // the diagonal block is factorised.
+    // tile rows below the diagonal block, G per wavefront pass (rows ib,
+    // ib + S, ..., ib + (G-1) S): the update from the L rows left of the
+    // panel (L_ij streamed from the workspace, the next column's tiles loaded
+    // during the current column's products), then per panel column the update
+    // from the panel's earlier columns and the TRSM, all in registers.
     v4 acc[G][NPC];
-    auto stream = [&](int ib) {
+    auto stream = [&](int ib, int S) {
       v4 b[G];
 #pragma unroll
       for (int g = 0; g < G; ++g)
-        if (k0 > 0 && ib + g * NW < KT) b[g] = *(const v4*)(wtile(ib + g * NW, 0) + lane * 4);
+        if (k0 > 0 && ib + g * S < KT) b[g] = *(const v4*)(wtile(ib + g * S, 0) + lane * 4);
       // A' of this pass's tiles, made while the first tiles load
 #pragma unroll
       for (int g = 0; g < G; ++g) {
-        const int i = ib + g * NW, ii = 16 * i + lr;
+        const int i = ib + g * S, ii = 16 * i + lr;
         if (i < KT) {
           const bool in = ii < n;
           const double xi = in ? cx[ii] : 0.0, yi = in ? cy[ii] : 0.0, zi = (in && DIM == 3) ? cz[ii] : 0.0;
@@ -890,14 +904,14 @@ __device__ void factor_block_ll(
         v4 bn[G];
 #pragma unroll
         for (int g = 0; g < G; ++g)
-          if (j + 1 < k0 && ib + g * NW < KT) bn[g] = *(const v4*)(wtile(ib + g * NW, j + 1) + lane * 4);
+          if (j + 1 < k0 && ib + g * S < KT) bn[g] = *(const v4*)(wtile(ib + g * S, j + 1) + lane * 4);
 #pragma unroll
         for (int q2 = 0; q2 < NPC; ++q2) {
           if (q2 < npc) {
             const v4 a = *(const v4*)((PG ? wtile(k0 + q2, j) : Rw + ((size_t)q2 * k0 + j) * kTE) + lane * 4);
 #pragma unroll
             for (int g = 0; g < G; ++g)
-              if (ib + g * NW < KT) {
+              if (ib + g * S < KT) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) acc[g][q2] = MF<Real>::mma(-a[q], b[g][q], acc[g][q2]);
               }
@@ -907,9 +921,48 @@ __device__ void factor_block_ll(
         for (int g = 0; g < G; ++g) b[g] = bn[g];
       }
     };
-    const int ib0 = k0 + npc + wave;
-    if (ib0 < KT) stream(ib0);
+    // the panel column's update from the panel's earlier columns and the
+    // TRSM with the diagonal inverses, for the pass streamed at ib
+    auto finish = [&](int ib, int S) {
+#pragma unroll
+      for (int q2 = 0; q2 < NPC; ++q2) {
+        if (q2 < npc) {
+#pragma unroll
+          for (int p2 = 0; p2 < q2; ++p2) {
+            const v4 a = *(const v4*)(pd(q2, p2) + lane * 4);
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+              if (ib + g * S < KT) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[g][q2] = MF<Real>::mma(-a[q], acc[g][p2][q], acc[g][q2]);
+              }
+          }
+          const Real* D = Dg + q2 * kLT;
+          Real li[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) li[q] = D[lr * kLS + MF<Real>::row(q, lq)];
+#pragma unroll
+          for (int g = 0; g < G; ++g) {
+            const int i = ib + g * S;
+            if (i < KT) {
+              v4 v = {0, 0, 0, 0};
+#pragma unroll
+              for (int q = 0; q < 4; ++q) v = MF<Real>::mma(li[q], acc[g][q2][q], v);
+              acc[g][q2] = v;
+              *(v4*)(wtile(i, k0 + q2) + lane * 4) = v;
+            }
+          }
+        }
+      }
+    };
+    // the diagonal block's sums are in LDS; wavefront 0 factorises it while
+    // the others stream the first (NW - 1) G tile rows below it (the streamed
+    // updates do not need the diagonal inverses), then every wavefront
+    // finishes its rows and takes the remaining ones NW G at a time
     __syncthreads();
+    constexpr int SO = NW - 1;
+    const int ibo = k0 + npc + wave - 1;
+    if (wave > 0 && ibo < KT) stream(ibo, SO);
     // factorise the diagonal block (one wavefront): per column q2 the inverse
     // of its diagonal tile, L below it (L^T = L_kk^{-1} A^T), and the update
     // of the block's later columns
@@ -966,38 +1019,10 @@ __device__ void factor_block_ll(
       __builtin_amdgcn_s_setprio(0);
     }
     __syncthreads();
-    for (int ib = ib0; ib < KT; ib += NW * G) {
-      if (ib != ib0) stream(ib);
-#pragma unroll
-      for (int q2 = 0; q2 < NPC; ++q2) {
-        if (q2 < npc) {
-#pragma unroll
-          for (int p2 = 0; p2 < q2; ++p2) {
-            const v4 a = *(const v4*)(pd(q2, p2) + lane * 4);
-#pragma unroll
-            for (int g = 0; g < G; ++g)
-              if (ib + g * NW < KT) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) acc[g][q2] = MF<Real>::mma(-a[q], acc[g][p2][q], acc[g][q2]);
-              }
-          }
-          const Real* D = Dg + q2 * kLT;
-          Real li[4];
-#pragma unroll
-          for (int q = 0; q < 4; ++q) li[q] = D[lr * kLS + MF<Real>::row(q, lq)];
-#pragma unroll
-          for (int g = 0; g < G; ++g) {
-            const int i = ib + g * NW;
-            if (i < KT) {
-              v4 v = {0, 0, 0, 0};
-#pragma unroll
-              for (int q = 0; q < 4; ++q) v = MF<Real>::mma(li[q], acc[g][q2][q], v);
-              acc[g][q2] = v;
-              *(v4*)(wtile(i, k0 + q2) + lane * 4) = v;
-            }
-          }
-        }
-      }
+    if (wave > 0 && ibo < KT) finish(ibo, SO);
+    for (int ib = k0 + npc + SO * G + wave; ib < KT; ib += NW * G) {
+      stream(ib, NW);
+      finish(ib, NW);
     }
     __syncthreads();
   }
