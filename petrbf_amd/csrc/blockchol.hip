@@ -712,14 +712,17 @@ __host__ __device__ inline int ll_row_tiles(int KT, int npc) {
   return m;
 }
 
-// Panels of more than two columns read their L rows straight from the
-// workspace (L2-resident: just written, and read by every wavefront of the
-// workgroup), so that LDS stays within three workgroups per CU.
-__host__ __device__ constexpr bool ll_rows_global(int npc) { return npc > 2; }
+// Panels of more than two columns (FP64: more than one) read their L rows
+// straight from the workspace (L2-resident: just written, and read by every
+// wavefront of the workgroup), one column ahead of their use, so that LDS
+// stays within three FP32 or two FP64 workgroups per CU (FP64 4x2 timed 24%
+// faster this way than with its panel rows in LDS at one workgroup per CU).
+template <typename T>
+__host__ __device__ constexpr bool ll_rows_global(int npc) { return npc >= (sizeof(T) == 8 ? 2 : 3); }
 
 template <typename T>
 __host__ __device__ inline size_t ll_region(int KT, int nw, int npc) {
-  size_t rows = ll_rows_global(npc) ? 0 : (size_t)ll_row_tiles(KT, npc) * kTE * sizeof(T);
+  size_t rows = ll_rows_global<T>(npc) ? 0 : (size_t)ll_row_tiles(KT, npc) * kTE * sizeof(T);
   size_t red = ((size_t)16 * KT * kLS + (size_t)nw * kLT) * sizeof(T);
   size_t region = rows > red ? rows : red;
   return (region + 31) & ~(size_t)31;
@@ -755,8 +758,8 @@ __device__ void factor_block_ll(
     const long long* __restrict__ mptr, Real* __restrict__ M, double inv_two_s2, double norm,
     int* __restrict__ blk_status) {
   typedef typename MF<Real>::v4 v4;
-  constexpr bool PG = ll_rows_global(NPC);  // panel rows from the workspace, not LDS
-  constexpr int G = PG ? 2 : 4;              // tile rows per wavefront pass
+  constexpr bool PG = ll_rows_global<Real>(NPC);  // panel rows from the workspace, not LDS
+  constexpr int G = NPC > 2 ? 2 : 4;  // tile rows per wavefront pass
   const long long b0 = bptr[c];
   const int n = (int)(bptr[c + 1] - b0);
   if (n == 0) return;
@@ -900,15 +903,30 @@ __device__ void factor_block_ll(
             if (q2 < npc) acc[g][q2] = aprime(i, q2, xi, yi, zi);
         }
       }
+      // panel tiles from the workspace are loaded one column ahead too
+      v4 pa[PG ? NPC : 1];
+      if constexpr (PG) {
+#pragma unroll
+        for (int q2 = 0; q2 < NPC; ++q2)
+          if (k0 > 0 && q2 < npc) pa[q2] = *(const v4*)(wtile(k0 + q2, 0) + lane * 4);
+      }
       for (int j = 0; j < k0; ++j) {
         v4 bn[G];
 #pragma unroll
         for (int g = 0; g < G; ++g)
           if (j + 1 < k0 && ib + g * S < KT) bn[g] = *(const v4*)(wtile(ib + g * S, j + 1) + lane * 4);
+        v4 pn[PG ? NPC : 1];
+        if constexpr (PG) {
+#pragma unroll
+          for (int q2 = 0; q2 < NPC; ++q2)
+            if (j + 1 < k0 && q2 < npc) pn[q2] = *(const v4*)(wtile(k0 + q2, j + 1) + lane * 4);
+        }
 #pragma unroll
         for (int q2 = 0; q2 < NPC; ++q2) {
           if (q2 < npc) {
-            const v4 a = *(const v4*)((PG ? wtile(k0 + q2, j) : Rw + ((size_t)q2 * k0 + j) * kTE) + lane * 4);
+            v4 a;
+            if constexpr (PG) a = pa[q2];
+            else a = *(const v4*)(Rw + ((size_t)q2 * k0 + j) * kTE + lane * 4);
 #pragma unroll
             for (int g = 0; g < G; ++g)
               if (ib + g * S < KT) {
@@ -919,6 +937,10 @@ __device__ void factor_block_ll(
         }
 #pragma unroll
         for (int g = 0; g < G; ++g) b[g] = bn[g];
+        if constexpr (PG) {
+#pragma unroll
+          for (int q2 = 0; q2 < NPC; ++q2) pa[q2] = pn[q2];
+        }
       }
     };
     // the panel column's update from the panel's earlier columns and the
