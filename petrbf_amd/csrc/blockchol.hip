@@ -157,6 +157,19 @@ __device__ inline void tri_decode(int t, int* I, int* J) {
 
 __device__ inline void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
 
+// Copies `bytes` (a multiple of 1 KB) from global memory to LDS, 1 KB per
+// wavefront instruction (global_load_lds_dwordx4: 16 bytes per lane, lane l
+// to dst + 16 l), without passing through registers; `dst` must be the same
+// in every lane.  The data is in LDS once this wavefront's vector-memory
+// counter is back to zero (lds_dma_wait).
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) void glb_void_t;
+__device__ __forceinline__ void lds_dma_1k(const void* src, void* dst, int lane) {
+  __builtin_amdgcn_global_load_lds((glb_void_t*)((const char*)src + 16 * lane), (lds_void_t*)dst, 16, 0, 0);
+}
+// s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maxima)
+__device__ __forceinline__ void lds_dma_wait() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 template <typename T>
 __device__ inline T gauss_entry(double dx, double dy, double dz, double inv_two_s2, double norm);
 template <>
@@ -810,26 +823,16 @@ __device__ void factor_block_ll(
   for (int k0 = 0; k0 < KT; k0 += NPC) {
     const int npc = min(NPC, KT - k0);
     // the panel's L rows left of it, row r at Rw + r k0 tiles (rows are
-    // contiguous in the workspace), and its columns' coordinates
-    // (the npc rows are contiguous in the workspace and in LDS; kCopyB
-    // 16-byte loads per lane are issued before their stores, so the copy
-    // costs a few load latencies per panel instead of one per 4 KB)
+    // contiguous in the workspace), copied by the LDS DMA, 1 KB per
+    // instruction, every wavefront issuing its share before waiting once;
+    // and the panel columns' coordinates
     if constexpr (!PG) {
-      constexpr int kCopyB = sizeof(Real) == 4 ? 4 : 2;
-      const int per_row = k0 * (kTE / 4), total = npc * per_row;  // in 16-byte vectors
-      for (int b0 = tid; b0 < total; b0 += NW * 64 * kCopyB) {
-        v4 t[kCopyB];
-#pragma unroll
-        for (int k = 0; k < kCopyB; ++k) {
-          const int e = b0 + k * NW * 64, r = e / max(per_row, 1);
-          if (e < total) t[k] = *(const v4*)(wtile(k0 + r, 0) + 4 * (e - r * per_row));
-        }
-#pragma unroll
-        for (int k = 0; k < kCopyB; ++k) {
-          const int e = b0 + k * NW * 64, r = e / max(per_row, 1);
-          if (e < total) *(v4*)(Rw + (size_t)r * k0 * kTE + 4 * (e - r * per_row)) = t[k];
-        }
+      const int per_row = k0 * (int)(kTE * sizeof(Real) / 1024), total = npc * per_row;  // 1 KB pieces
+      for (int b = wave; b < total; b += NW) {
+        const int r = b / per_row, o = b - r * per_row;
+        lds_dma_1k((const char*)wtile(k0 + r, 0) + 1024 * o, (char*)Rw + (size_t)1024 * b, lane);
       }
+      lds_dma_wait();
     }
     for (int p = tid; p < 16 * npc; p += NW * 64) {
       const int gp = 16 * k0 + p;
