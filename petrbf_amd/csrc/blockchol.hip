@@ -869,12 +869,25 @@ __device__ void factor_block_ll(
       v4 acc = aprime(k0 + r, q2, Cc[16 * r + lr], Cc[16 * NPC + 16 * r + lr], Cc[32 * NPC + 16 * r + lr]);
       const Real* ra = (PG ? wtile(k0 + q2, 0) : Rw + (size_t)q2 * k0 * kTE) + lane * 4;
       const Real* rb = (PG ? wtile(k0 + r, 0) : Rw + (size_t)r * k0 * kTE) + lane * 4;
-      for (int j = 0; j < k0; ++j) {
-        const v4 a = *(const v4*)(ra + j * kTE);
-        const v4 b = *(const v4*)(rb + j * kTE);
+      // two columns per step into two accumulators: both columns' loads are
+      // in flight together and the two product chains are independent
+      v4 acc2 = {0, 0, 0, 0};
+      int j = 0;
+      for (; j + 1 < k0; j += 2) {
+        const v4 a = *(const v4*)(ra + j * kTE), a2 = *(const v4*)(ra + (j + 1) * kTE);
+        const v4 b = *(const v4*)(rb + j * kTE), b2 = *(const v4*)(rb + (j + 1) * kTE);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          acc = MF<Real>::mma(-a[q], b[q], acc);
+          acc2 = MF<Real>::mma(-a2[q], b2[q], acc2);
+        }
+      }
+      if (j < k0) {
+        const v4 a = *(const v4*)(ra + j * kTE), b = *(const v4*)(rb + j * kTE);
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc = MF<Real>::mma(-a[q], b[q], acc);
       }
+      acc += acc2;
       if (r == q2) {
         Real* D = Dg + r * kLT;
 #pragma unroll
