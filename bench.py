@@ -28,7 +28,7 @@ sys.path.insert(0, ROOT)
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 PEAK_FP32_MFMA_TFLOPS = 157.3  # MI355X dense FP32 matrix
 PEAK_HBM_GBS = 8000.0     # MI355X HBM3E
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "block_chol_fp32_n4m_hbm_traffic.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "block_chol_n4m_hbm_traffic.txt")
 CPU_VALIDATION_FILE = os.path.join(ROOT, "profiles", "cpu_baseline_validation.txt")
 
 
