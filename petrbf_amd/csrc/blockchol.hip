@@ -85,6 +85,11 @@ constexpr int kTE = 256;   // elements per 16x16 tile
 constexpr int kLS = 17;    // LDS row stride (bank-conflict padding)
 constexpr int kLT = 16 * kLS;
 
+// Sets of per-wavefront partial-sum tiles the backward substitution keeps
+// (FP32 two, one barrier per step; FP64 one, two barriers per step).
+template <typename T>
+__host__ __device__ constexpr int sbk_sets() { return sizeof(T) == 4 ? 2 : 1; }
+
 // Lane-major tiles (the left-looking factorisation): lane l keeps its four
 // values X[l%16][row(q, l/16)], q = 0..3, contiguous at 4 l.  One vector
 // access per lane is then the A operand of X, the B operand of X^T (both with
@@ -345,7 +350,7 @@ __host__ __device__ inline size_t blk_lds_bytes(int nmax, int tmax, int nw, int 
   const int m = npc < KT ? npc : KT;
   size_t pan = (size_t)pan_tiles(KT, m) * Pan<T>::kT * sizeof(T);
   size_t crd = (size_t)3 * NP * sizeof(double);
-  size_t red = ((size_t)NP * kLS + (size_t)nw * kLT) * sizeof(T);  // phase-2 right-hand sides, per-wave scratch
+  size_t red = ((size_t)NP * kLS + (size_t)sbk_sets<T>() * nw * kTE) * sizeof(T);  // phase-2 right-hand sides, partial sums
   size_t region = pan > crd ? pan : crd;
   region = region > red ? region : red;
   region = (region + 15) & ~(size_t)15;
@@ -367,8 +372,13 @@ __device__ void core_rows(long long c, const Real* __restrict__ Wb, const Real* 
   // 168 cap, 3% faster); in FP64 the same change costs 10%, so it stays a vector
   const int wave = sizeof(Real) == 4 ? __builtin_amdgcn_readfirstlane(tid / 64) : tid / 64;
   const int lr = lane & 15, lq = lane >> 4;
-  Real* S = (Real*)lds_raw + (size_t)(16 * KT) * kLS;  // per-wave scratch tiles, past X
-  Real* Sw = S + wave * kLT;
+  // per-wave partial sums of the backward substitution, past X: FP32 keeps
+  // two sets used in alternate steps, so that a step needs one barrier (a
+  // wavefront writing step I-1's partial cannot overwrite what wavefront 0
+  // still reads of step I); FP64 keeps one set (LDS for two workgroups per CU)
+  // and a second barrier
+  constexpr int kSB = sbk_sets<Real>();
+  Real* S = (Real*)lds_raw + (size_t)(16 * KT) * kLS;
   // ---- phase 2: X = L^{-T} L^{-1} E_core, 16 right-hand sides at a time,
   // X in LDS (row stride 17) in the region the panel used
   Real* X = (Real*)lds_raw;
@@ -393,17 +403,19 @@ __device__ void core_rows(long long c, const Real* __restrict__ Wb, const Real* 
               acc = MF<Real>::mma(tij[MF<Real>::off(lr, 4 * q + lq)], X[(16 * J + 4 * q + lq) * kLS + lr], acc);
           }
         }
+        // R = E_I - sum in accumulator layout; register q is the B operand of
+        // product q with K index row(q, lq), so no re-layout through LDS
+        Real rv[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           int row = 16 * I + MF<Real>::row(r, lq), col = 16 * tc + lr;
           Real e = (col < T && row == ncore0 + col) ? Real(1) : Real(0);
-          Sw[MF<Real>::row(r, lq) * kLS + lr] = e - acc[r];
+          rv[r] = e - acc[r];
         }
-        wave_fence();
         const Real* li = Wd + (long long)I * kTE;
         v4 y = {0, 0, 0, 0};
 #pragma unroll
-        for (int q = 0; q < 4; ++q) y = MF<Real>::mma(li[lr * 16 + 4 * q + lq], Sw[(4 * q + lq) * kLS + lr], y);
+        for (int q = 0; q < 4; ++q) y = MF<Real>::mma(li[lr * 16 + MF<Real>::row(q, lq)], rv[q], y);
 #pragma unroll
         for (int r = 0; r < 4; ++r) X[(16 * I + MF<Real>::row(r, lq)) * kLS + lr] = y[r];
         wave_fence();
@@ -428,10 +440,10 @@ __device__ void core_rows(long long c, const Real* __restrict__ Wb, const Real* 
           for (int q = 0; q < 4; ++q) nx[g][q] = tji[tile_off<Real, LM>(4 * q + lq, lr)];
         }
       }
-      if (wave == 0) {
+      if (wave == 0) {  // (L_II^{-1})^T as the A operand, K index row(q, lq)
         const Real* li = Wd + (long long)I * kTE;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) nli[q] = li[(4 * q + lq) * 16 + lr];
+        for (int q = 0; q < 4; ++q) nli[q] = li[MF<Real>::row(q, lq) * 16 + lr];
       }
     };
     load_col(KT - 1);
@@ -461,30 +473,30 @@ __device__ void core_rows(long long c, const Real* __restrict__ Wb, const Real* 
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc = MF<Real>::mma(a[q], X[(16 * J + 4 * q + lq) * kLS + lr], acc);
       }
+      Real* Ss = S + (kSB == 2 ? (I & 1) * NW : 0) * kTE;  // this step's set
 #pragma unroll
-      for (int r = 0; r < 4; ++r) Sw[r * 64 + lane] = acc[r];
+      for (int r = 0; r < 4; ++r) Ss[wave * kTE + r * 64 + lane] = acc[r];
       __syncthreads();
       if (wave == 0) {
         Real rv[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          Real sum = S[r * 64 + lane];
-          for (int w = 1; w < NW; ++w) sum += S[w * kLT + r * 64 + lane];
+          Real sum = Ss[r * 64 + lane];
+          for (int w = 1; w < NW; ++w) sum += Ss[w * kTE + r * 64 + lane];
           Real yv = (I >= I0) ? X[(16 * I + MF<Real>::row(r, lq)) * kLS + lr] : Real(0);
           rv[r] = yv - sum;
         }
-        wave_fence();
-#pragma unroll
-        for (int r = 0; r < 4; ++r) Sw[MF<Real>::row(r, lq) * kLS + lr] = rv[r];
-        wave_fence();
         v4 x = {0, 0, 0, 0};
 #pragma unroll
-        for (int q = 0; q < 4; ++q) x = MF<Real>::mma(al[q], Sw[(4 * q + lq) * kLS + lr], x);
+        for (int q = 0; q < 4; ++q) x = MF<Real>::mma(al[q], rv[q], x);
 #pragma unroll
         for (int r = 0; r < 4; ++r) X[(16 * I + MF<Real>::row(r, lq)) * kLS + lr] = x[r];
       }
-      __syncthreads();
+      // X_I is read next by wavefront 0 itself (column I of step I-1) and by
+      // the others only after step I-1's barrier
+      if constexpr (kSB == 1) __syncthreads();
     }
+    if constexpr (kSB == 2) __syncthreads();
     // ---- phase 3: M_c[16 tc + tt][b] = X[pos(b)][tt], rows past T zero, in
     // the apply's storage order (apply_offset), written contiguously
     const int tcn = min(16, T - 16 * tc), R = apply_chunk_rows(T, tc);
@@ -532,7 +544,7 @@ __device__ void factor_block(
   // and the flag of regularised pivots
   typedef Pan<Real> PT;
   size_t pan = (size_t)pan_tiles(KT, min(NPC, KT)) * PT::kT * sizeof(Real), crd = (size_t)3 * NP * sizeof(double),
-         red = ((size_t)NP * kLS + (size_t)NW * kLT) * sizeof(Real);
+         red = ((size_t)NP * kLS + (size_t)sbk_sets<Real>() * NW * kTE) * sizeof(Real);
   size_t region = pan > crd ? pan : crd;
   region = region > red ? region : red;
   region = (region + 15) & ~(size_t)15;
@@ -736,7 +748,7 @@ __host__ __device__ constexpr bool ll_rows_global(int npc) { return npc >= (size
 template <typename T>
 __host__ __device__ inline size_t ll_region(int KT, int nw, int npc) {
   size_t rows = ll_rows_global<T>(npc) ? 0 : (size_t)ll_row_tiles(KT, npc) * kTE * sizeof(T);
-  size_t red = ((size_t)16 * KT * kLS + (size_t)nw * kLT) * sizeof(T);
+  size_t red = ((size_t)16 * KT * kLS + (size_t)sbk_sets<T>() * nw * kTE) * sizeof(T);
   size_t region = rows > red ? rows : red;
   return (region + 31) & ~(size_t)31;
 }
