@@ -217,8 +217,8 @@ prbf_status prbf_rasm_setup_pc(prbf_ctx* ctx, int pc);
  * workgroup), PRBF_TUNE_CHOL_PANEL (1 to 4 16-wide tile columns per panel),
  * PRBF_TUNE_CHOL_ORDER (prbf_chol_order; right-looking unless set).
  * Right-looking: FP32 has every pair; FP64 has 4 or 8 wavefronts with 1, 2
- * or 4 columns.  Left-looking: 4 or 8 wavefronts with 1 or 2 columns in both
- * precisions.  A forced configuration without a kernel, or whose LDS does not
+ * or 4 columns.  Left-looking: 4 or 8 wavefronts with 1, 2 or 4 columns in
+ * both precisions.  A forced configuration without a kernel, or whose LDS does not
  * fit, runs right-looking as 4 wavefronts with 1 column.  No reference
  * counterpart. */
 prbf_status prbf_set_tuning(prbf_ctx* ctx, int key, int value);
