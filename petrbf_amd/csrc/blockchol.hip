@@ -190,13 +190,6 @@ __device__ inline float gauss_entry<float>(double dx, double dy, double dz, doub
   return __expf(-a) * (float)norm;
 }
 
-// Value of v in lane P of this lane's 16-lane row (DPP row_newbcast:P, one
-// VALU move, no LDS and no readlane).
-template <int P>
-__device__ __forceinline__ float row_bcast(float v) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x150 + P, 0xf, 0xf, true));
-}
-
 // 1/sqrt: the hardware instruction in FP32 (the FP32 blocks are a
 // preconditioner), the library's full-precision one in FP64.
 __device__ __forceinline__ float pivot_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
@@ -221,29 +214,63 @@ __device__ __forceinline__ Real diag_shift(double norm, int attempt) {
   return sizeof(Real) == 4 ? (Real)ldexp(norm, -16 + 4 * attempt) : (Real)ldexp(norm, -44 + 8 * attempt);
 }
 
-// Step P of the symmetric elimination on [D | I]: every 16-lane row of the
-// wavefront holds the whole 16x16 problem, lane i of the row holding row i.
-// Row P is read through row_newbcast; rows below P subtract
+// acc + (acc in lane P of this 16-lane row) * nm as one v_fmac_f32_dpp: the
+// row_newbcast broadcast is the FMAC's DPP operand, with no separate move
+// (the compiler emits a v_mov_b32_dpp, the FMA and a nop for each update).
+// The DPP read-after-write hazard wants any VALU write of a register this
+// instruction reads at least two wait states ahead; NOP puts an s_nop 1 in
+// front where the elimination's order does not already keep such writes
+// that far away (the step's multiplier, written just before its first
+// update, and the right half's newest column, written by compiler code).
+template <int P, bool NOP>
+__device__ __forceinline__ float fmac_row_bcast(float acc, float nm) {
+  if constexpr (NOP)
+    asm volatile("s_nop 1\n\tv_fmac_f32_dpp %0, %0, %1 row_newbcast:%2 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+                 : "+v"(acc)
+                 : "v"(nm), "n"(P));
+  else
+    asm volatile("v_fmac_f32_dpp %0, %0, %1 row_newbcast:%2 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+                 : "+v"(acc)
+                 : "v"(nm), "n"(P));
+  return acc;
+}
+// v in lane P of this 16-lane row, after two wait states (an asm statement,
+// so that it stays behind the previous step's updates of v)
+template <int P>
+__device__ __forceinline__ float row_bcast_nop(float v) {
+  float r;
+  asm volatile("s_nop 1\n\tv_mov_b32_dpp %0, %1 row_newbcast:%2 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+               : "=&v"(r)
+               : "v"(v), "n"(P));
+  return r;
+}
+
+// Step P of the symmetric elimination on [D | I] in FP32: every 16-lane row
+// of the wavefront holds the whole 16x16 problem, lane i of the row holding
+// row i.  Row P is read through row_newbcast; rows below P subtract
 // (a_iP / a_PP) x row P from both halves (rows at or above P multiply by 0),
-// and lane P keeps 1/sqrt(a_PP), the scale of its row of L^{-1}.
+// and lane P keeps 1/sqrt(a_PP), the scale of its row of L^{-1}.  Column P of
+// the right half is (i == P) until this step, so its update is a select.
+// (FP32 4x4 setup kernel 1.3% faster than the move + FMA form; the same bits.)
 template <typename Real, int P>
 struct DiagStep {
   __device__ static __forceinline__ void run(Real (&a)[16], Real (&e)[16], Real& rs, bool& bad, int i, Real pfloor) {
-    e[P] = (i == P) ? Real(1) : Real(0);  // column P of the right half is untouched until now
-    const Real piv = row_bcast<P>(a[P]);
+    const Real piv = row_bcast_nop<P>(a[P]);
     const bool b = !(piv > pfloor);  // (NaN included)
     bad |= b;
     const Real rinv = pivot_rsqrt(b ? pfloor : piv);  // (branch-free: one block per step)
     rs = i == P ? rinv : rs;
-    const Real m = i > P ? a[P] * (rinv * rinv) : Real(0);
+    const Real nm = i > P ? -(a[P] * (rinv * rinv)) : Real(0);
+    if constexpr (P + 1 < 16) a[P + 1] = fmac_row_bcast<P, true>(a[P + 1], nm);
 #pragma unroll
-    for (int q = P + 1; q < 16; ++q) {
-      a[q] -= m * row_bcast<P>(a[q]);
-    }
+    for (int q = P + 2; q < 16; ++q) a[q] = fmac_row_bcast<P, false>(a[q], nm);
 #pragma unroll
-    for (int q = 0; q <= P; ++q) {
-      e[q] -= m * row_bcast<P>(e[q]);
+    for (int q = 0; q + 1 < P; ++q) {
+      if (q == 0 && P + 1 == 16) e[q] = fmac_row_bcast<P, true>(e[q], nm);  // (no a update at the last step)
+      else e[q] = fmac_row_bcast<P, false>(e[q], nm);
     }
+    if constexpr (P > 0) e[P - 1] = fmac_row_bcast<P, true>(e[P - 1], nm);  // (set by the select of step P-1)
+    e[P] = (i == P) ? Real(1) : nm;
     DiagStep<Real, P + 1>::run(a, e, rs, bad, i, pfloor);
   }
 };
@@ -425,15 +452,25 @@ __device__ void core_rows(long long c, const Real* __restrict__ Wb, const Real* 
     __syncthreads();
     // backward: X_I = L_II^{-T} (Y_I - sum_{J>I} L_JI^T X_J); the sum is split
     // over the wavefronts and combined by wavefront 0 through the scratch
-    // tiles.  Each wavefront's first kPF tiles of column I-1 (and wavefront
-    // 0's L_{I-1,I-1}^{-1}) are loaded while step I runs, so the global
-    // loads leave the step's critical path.
-    constexpr int kPF = 4;
+    // tiles.  In FP64 wavefront 0, which also combines the sums and applies
+    // the diagonal inverse, takes only J = I + 1 (whose X_J it wrote itself at
+    // the previous step) and the others share J >= I + 2 (FP64 4x2 setup 3.8%
+    // faster); FP32 deals J round-robin over all wavefronts (the split form
+    // spills there).  Each wavefront's first kPF tiles of column I-1 (and
+    // wavefront 0's L_{I-1,I-1}^{-1}) are loaded while step I runs, so the
+    // global loads leave the step's critical path.
+    constexpr bool kSplit = sizeof(Real) == 8;
+    constexpr int kPF = kSplit ? 6 : 4;
+    constexpr int kStride = kSplit ? NW - 1 : NW;
+    auto jof = [&](int I, int g) {
+      if constexpr (kSplit) return wave == 0 ? (g == 0 ? I + 1 : KT) : I + 1 + wave + g * (NW - 1);
+      else return I + 1 + wave + g * NW;
+    };
     Real nx[kPF][4], nli[4];
     auto load_col = [&](int I) {
 #pragma unroll
       for (int g = 0; g < kPF; ++g) {
-        const int J = I + 1 + wave + g * NW;
+        const int J = jof(I, g);
         if (J < KT) {
           const Real* tji = Wb + (long long)ltile(J, I) * kTE;
 #pragma unroll
@@ -459,13 +496,14 @@ __device__ void core_rows(long long c, const Real* __restrict__ Wb, const Real* 
       v4 acc = {0, 0, 0, 0};
 #pragma unroll
       for (int g = 0; g < kPF; ++g) {
-        const int J = I + 1 + wave + g * NW;
+        const int J = jof(I, g);
         if (J < KT) {
 #pragma unroll
           for (int q = 0; q < 4; ++q) acc = MF<Real>::mma(av[g][q], X[(16 * J + 4 * q + lq) * kLS + lr], acc);
         }
       }
-      for (int J = I + 1 + wave + kPF * NW; J < KT; J += NW) {  // (blocks beyond kPF x NW tiles)
+      // (blocks beyond the prefetched tiles)
+      for (int J = (kSplit && wave == 0) ? KT : I + 1 + wave + kPF * kStride; J < KT; J += kStride) {
         const Real* tji = Wb + (long long)ltile(J, I) * kTE;
         Real a[4];
 #pragma unroll

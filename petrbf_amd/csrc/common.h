@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -108,28 +109,28 @@ __device__ inline double exp_nonpos(double x) {
   return ldexp(p, (int)kf);
 }
 
-// exp(x) for -700 < x <= 0 in FP64, table-driven: x = (64 m + j) ln2/64 + r
-// with a two-part ln2/64 (high part 33 significant bits), |r| <= ln2/128,
-// e^r by its degree-5 Taylor polynomial (truncation < 4e-17 relative), times
-// tab[j] = 2^(j/64) and 2^m.  Within 3 ulp of exp over [-80, 0] (checked
-// against long double expl on 2e7 points); 11 FP64 operations against the
-// 17 of exp_nonpos.  tab is a 64-entry table in LDS.  k = rint(x 64/ln2) is
-// taken by adding 1.5 * 2^52 in one FMA, which leaves k in the sum's low word
-// (|k| < 2^31), in place of a multiply, a round and a conversion; k differs
-// from rint of the rounded product only where that product lies within half
-// an ulp of a half-integer (none in 2e7 sampled points), and the reduction
-// stays exact either way.
-__device__ inline double exp_nonpos_tab(double x, const double* tab) {
+// 2^(y/64) for -2^20 < y <= 0 in FP64, for an argument already in units of
+// ln2/64 (the matvec scales its coordinates so that y = -|dx|^2 (64/ln2) /
+// (2 sigma^2) comes straight out of the squared distance): k = rint(y) by the
+// 1.5 * 2^52 shift (adding it leaves k in the sum's low word, |k| < 2^31, in
+// place of a round and a conversion), r = y - k EXACTLY (|r| <= 1/2,
+// Sterbenz), 2^(r/64) by its degree-5 Taylor polynomial in r (coefficients
+// (ln2/64)^n / n!, correctly rounded from 60-digit decimal arithmetic;
+// truncation < 4e-17 relative),
+// times tab[k & 63] = 2^((k & 63)/64) and 2^(k >> 6).  With no reduction
+// constant there is no reduction error: the result's error is the
+// polynomial's and the two products' roundings (10 FP64 operations; a
+// table-driven exp of x = -r^2/(2 sigma^2) needs 11, two of them for a
+// two-part ln2/64 reduction).
+__device__ inline double exp2_64_nonpos(double y, const double* tab) {
   constexpr double kShift = 6755399441055744.0;  // 1.5 * 2^52
-  const double kd = fma(x, 92.33248261689366, kShift);
+  const double kd = y + kShift;
   const int k = __double2loint(kd);
-  const double kf = kd - kShift;
-  double r = fma(-kf, 0.010830424695086549, x);
-  r = fma(-kf, 1.162596423439437e-12, r);
-  double p = fma(r, 0.008333333333333333, 0.041666666666666664);
-  p = fma(p, r, 0.16666666666666666);
-  p = fma(p, r, 0.5);
-  p = fma(p, r, 1.0);
+  const double r = y - (kd - kShift);
+  double p = fma(r, 0x1.5d87fe78a6731p-40, 0x1.3b2ab6fba4e77p-31);
+  p = fma(p, r, 0x1.c6b08d704a0c0p-23);
+  p = fma(p, r, 0x1.ebfbdff82c58fp-15);
+  p = fma(p, r, 0x1.62e42fefa39efp-7);
   p = fma(p, r, 1.0);
   return ldexp(p * tab[k & 63], k >> 6);
 }

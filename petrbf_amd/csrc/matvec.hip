@@ -8,18 +8,23 @@
 // 2d/get_trunc.h:63) is staged in LDS as SoA FP64 tiles; each lane owns two
 // targets and a residue class of the source tile, accumulates in FP64 with
 // FMA, and the per-target partial sums are combined in a fixed order, so the
-// result is deterministic.  FP64 VALU-bound: one exp (exp_nonpos_tab, with
-// its 2^(j/64) table in LDS) per pair.
+// result is deterministic.  FP64 VALU-bound: one exponential per pair.  The
+// coordinates enter as x' = (x - o) S, o the cluster's first target and
+// S = sqrt(64 / (2 sigma^2 ln2)), so that -|x_i' - x_j'|^2 is the exponent in
+// units of ln2/64 and exp2_64_nonpos (its 2^(j/64) table in LDS) finishes it
+// without a reduction constant: 15 FP64 operations per pair (2D).  Local
+// coordinates keep the differences accurate: |x - o| is at most the
+// truncation box's reach, so each x' carries a relative rounding of 2^-53 of
+// a number of the differences' own size.
 #include "common.h"
 #include "kernels.h"
 
 namespace prbf {
 
-constexpr int kTile = 256;  // sources staged per LDS tile
+constexpr int kTile = 128;  // sources staged per LDS tile (smaller tiles: more wavefronts per SIMD)
 
 // 2^(j/64), j = 0..63, correctly rounded (from 60-digit decimal arithmetic):
-// the table exp_nonpos_tab's 3-ulp bound was checked with, read once per
-// workgroup instead of 64 FP64 exp2 calls
+// read once per workgroup instead of 64 FP64 exp2 calls
 __constant__ double kExp2Tab64[64] = {
     0x1.0000000000000p+0, 0x1.02c9a3e778061p+0, 0x1.059b0d3158574p+0, 0x1.0874518759bc8p+0,
     0x1.0b5586cf9890fp+0, 0x1.0e3ec32d3d1a2p+0, 0x1.11301d0125b51p+0, 0x1.1429aaea92de0p+0,
@@ -50,7 +55,7 @@ __global__ __launch_bounds__(64) void k_matvec_wave(
     const double* __restrict__ xi, const double* __restrict__ yi, const double* __restrict__ zi,
     const double* __restrict__ xj, const double* __restrict__ yj, const double* __restrict__ zj,
     const double* __restrict__ g, long long ldg, double* __restrict__ w, long long ldw,
-    double inv_two_s2, double norm) {
+    double scale, double norm) {
   __shared__ double sx[kTile], sy[kTile], sz[DIM == 3 ? kTile : 1];
   __shared__ double sg[NRHS][kTile];
   __shared__ double part[NRHS][128];
@@ -61,7 +66,8 @@ __global__ __launch_bounds__(64) void k_matvec_wave(
   if (t0 > t1) return;
   const long long s0 = tptr[c], s1 = tptr[c + 1];
   const int T = t1 - t0 + 1;
-  const double nc = -inv_two_s2;
+  // local origin: the cluster's first target
+  const double ox = xi[t0], oy = yi[t0], oz = DIM == 3 ? zi[t0] : 0.0;
   etab[lane] = kExp2Tab64[lane];  // (visible after the first tile's barrier)
   for (int tb = 0; tb < T; tb += 128) {
     const int Tc = min(128, T - tb);
@@ -73,13 +79,13 @@ __global__ __launch_bounds__(64) void k_matvec_wave(
     const bool hasb = tb2 < tb + Tc;
     double ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
     if (active) {
-      ax = xi[t0 + ta];
-      ay = yi[t0 + ta];
-      if (DIM == 3) az = zi[t0 + ta];
+      ax = (xi[t0 + ta] - ox) * scale;
+      ay = (yi[t0 + ta] - oy) * scale;
+      if (DIM == 3) az = (zi[t0 + ta] - oz) * scale;
       const int tbb = hasb ? tb2 : ta;
-      bx = xi[t0 + tbb];
-      by = yi[t0 + tbb];
-      if (DIM == 3) bz = zi[t0 + tbb];
+      bx = (xi[t0 + tbb] - ox) * scale;
+      by = (yi[t0 + tbb] - oy) * scale;
+      if (DIM == 3) bz = (zi[t0 + tbb] - oz) * scale;
     }
     double accA[NRHS], accB[NRHS];
 #pragma unroll
@@ -89,9 +95,9 @@ __global__ __launch_bounds__(64) void k_matvec_wave(
       __syncthreads();
       for (int k = lane; k < n; k += 64) {
         int j = tidx[sb + k];
-        sx[k] = xj[j];
-        sy[k] = yj[j];
-        if (DIM == 3) sz[k] = zj[j];
+        sx[k] = (xj[j] - ox) * scale;
+        sy[k] = (yj[j] - oy) * scale;
+        if (DIM == 3) sz[k] = (zj[j] - oz) * scale;
 #pragma unroll
         for (int r = 0; r < NRHS; ++r) sg[r][k] = g[r * ldg + j];
       }
@@ -112,8 +118,8 @@ __global__ __launch_bounds__(64) void k_matvec_wave(
             ra2 = fma(az - z2, az - z2, ra2);
             rb2 = fma(bz - z2, bz - z2, rb2);
           }
-          const double ea1 = exp_nonpos_tab(ra1 * nc, etab), eb1 = exp_nonpos_tab(rb1 * nc, etab);
-          const double ea2 = exp_nonpos_tab(ra2 * nc, etab), eb2 = exp_nonpos_tab(rb2 * nc, etab);
+          const double ea1 = exp2_64_nonpos(-ra1, etab), eb1 = exp2_64_nonpos(-rb1, etab);
+          const double ea2 = exp2_64_nonpos(-ra2, etab), eb2 = exp2_64_nonpos(-rb2, etab);
 #pragma unroll
           for (int r = 0; r < NRHS; ++r) {
             const double g1 = sg[r][k], g2 = sg[r][k2];
@@ -132,7 +138,7 @@ __global__ __launch_bounds__(64) void k_matvec_wave(
             ra1 = fma(az - z1, az - z1, ra1);
             rb1 = fma(bz - z1, bz - z1, rb1);
           }
-          const double ea1 = exp_nonpos_tab(ra1 * nc, etab), eb1 = exp_nonpos_tab(rb1 * nc, etab);
+          const double ea1 = exp2_64_nonpos(-ra1, etab), eb1 = exp2_64_nonpos(-rb1, etab);
 #pragma unroll
           for (int r = 0; r < NRHS; ++r) {
             accA[r] = fma(sg[r][k], ea1, accA[r]);
@@ -169,10 +175,12 @@ void matvec(hipStream_t st, int dim, int nrhs, long long ncell, const int* tsta,
             const double* xj, const double* yj, const double* zj, const double* g, long long ldg, double* w,
             long long ldw, double sigma) {
   Kern k = make_kern(sigma);
+  // coordinate scale: |dx'|^2 = |dx|^2 (64/ln2) / (2 sigma^2)
+  const double scale = std::sqrt(k.inv_two_s2 * 92.33248261689366);
   dim3 grid((unsigned)ncell), block(64);
 #define PRBF_MV(D, R)                                                                              \
   hipLaunchKernelGGL((k_matvec_wave<D, R>), grid, block, 0, st, ncell, tsta, tend, tptr, tidx, xi, yi, \
-                     zi, xj, yj, zj, g, ldg, w, ldw, k.inv_two_s2, k.norm)
+                     zi, xj, yj, zj, g, ldg, w, ldw, scale, k.norm)
   if (dim == 2) {
     if (nrhs == 1) PRBF_MV(2, 1);
     else if (nrhs == 2) PRBF_MV(2, 2);
