@@ -144,7 +144,8 @@ def main():
     ap.add_argument("--mode", choices=["plain", "mixed", "both", "learned"], default="both",
                     help="learned: the C4 leg (network trained on ~2M pairs harvested from another seeded "
                          "lattice of the same size, then the learned FGMRES step against the mixed one); "
-                         "its own JSON line")
+                         "its own JSON line.  With --workload c5: the 3D network in the batched three-RHS "
+                         "FGMRES after the exact one")
     ap.add_argument("--learned-vectors", type=int, default=18, help="Krylov vectors harvested (x ncell pairs)")
     ap.add_argument("--learned-steps", type=int, default=2000, help="Adam steps of the training")
     ap.add_argument("--chol-waves", type=int, default=0, help="4 or 8 wavefronts per block factorisation")
@@ -554,14 +555,70 @@ def bench_c5(args, torch, problems, Context, dev):
         comps.append({"its": inf.its, "reason": inf.reason, "rel_residual": rel})
         log(f"component {c}: its {inf.its} reason {inf.reason} residual {rel:.3e}")
     log(f"batched three-RHS FGMRES: {t_solve:.3f}s for {max(i.its for i in infos)} iterations")
-    print(json.dumps({
+    res = {
         "metric": "3D 160^3 three-RHS workload: lists, batched matvec, FP32 setup, FGMRES progress",
         "n": p.n, "ncell": info["ncell"], "max_buffer": ci["max_buffer"], "t_lists_s": t_lists,
         "matvec_3rhs_ms": t_mv * 1e3, "pairs_per_matvec": pairs, "pair_evals_per_s": pairs / t_mv,
         "matvec_tflops": flops / t_mv / 1e12, "setup_fp32_s": t_setup, "chol_kernel_s": ci["chol_kernel_s"],
         "chol_tflops": ci["chol_flops"] / ci["chol_kernel_s"] / 1e12, "fgmres_batched_3rhs": comps,
         "fgmres_batched_s": t_solve, "fgmres_iteration_budget": args.c5_its,
-    }), flush=True)
+    }
+    if args.mode == "learned":
+        res["learned"] = bench_c5_learned(args, torch, problems, Context, dev, ctx, w)
+    print(json.dumps(res), flush=True)
+
+
+def bench_c5_learned(args, torch, problems, Context, dev, ctx, w):
+    """The 3D network (5x5x5-slot patch -> 27 core corrections) in the batched
+    three-RHS FGMRES of the 160^3 workload: pairs harvested from the first
+    Krylov vectors of an exact FP64 FGMRES run (x component) on another seeded
+    160^3 lattice, the network trained on the device, attached to every
+    eligible cluster of ctx, and the batched learned solve run with the same
+    iteration budget as the exact one."""
+    from petrbf_amd.learned import Mlp, harvest
+    n_side = 160
+    log(f"c5 learned: harvesting on lattice {n_side}^3 (seed 0x5EED0006)")
+    pt = problems.lattice_3d_problem(n_side, seed=0x5EED0006)
+    ct = Context.from_problem(pt)
+    ct.set_points_dev(*(torch.from_numpy(a).to(dev) for a in (pt.x, pt.y, pt.z)))
+    t0 = time.perf_counter()
+    X, T, osc = harvest(ct, torch.from_numpy(pt.w[0]).to(dev), args.learned_vectors, torch)
+    t_harvest = time.perf_counter() - t0
+    ct.close()
+    del pt
+    net = Mlp(seed=7, dim=3)
+    net.set_params(net.params()[0], osc)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = net.train_dev(X, T, batch=4096, steps=args.learned_steps, lr=1e-3, seed=3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    npairs = int(X.shape[0])
+    log(f"c5 learned: {npairs} pairs in {t_harvest:.1f}s; {args.learned_steps} Adam steps in {t_train:.1f}s, "
+        f"loss {float(hist[:20].mean()):.3f} -> {float(hist[-50:].mean()):.3f}")
+    del X, T
+    t0 = time.perf_counter()
+    n_net = ctx.learned_attach(net, 1e30)
+    torch.cuda.synchronize()
+    t_attach = time.perf_counter() - t0
+    gam = torch.empty_like(w)
+    t0 = time.perf_counter()
+    infos = ctx.solve_multi_dev(w, gam, 3, ksp="fgmres", pc="learned", rtol=1e-10, restart=30, max_it=args.c5_its)
+    ctx.synchronize()
+    t_solve = time.perf_counter() - t0
+    comps = []
+    for c, inf in enumerate(infos):
+        rel = inf.rnorm / inf.rnorm0 if inf.rnorm0 > 0 else 0.0
+        comps.append({"its": inf.its, "reason": inf.reason, "rel_residual": rel})
+        log(f"c5 learned component {c}: its {inf.its} reason {inf.reason} residual {rel:.3e}")
+    log(f"c5 learned: attached {n_net} clusters ({t_attach:.2f}s), on the network at the end "
+        f"{infos[0].n_network}, moved to exact blocks {infos[0].n_net_fallback}; solve {t_solve:.3f}s")
+    net.close()
+    return {"pairs": npairs, "harvest_s": t_harvest, "train_steps": args.learned_steps, "train_s": t_train,
+            "loss_first": float(hist[:20].mean()), "loss_last": float(hist[-50:].mean()),
+            "clusters_attached": n_net, "attach_s": t_attach, "clusters_on_network_at_end": infos[0].n_network,
+            "clusters_moved_to_exact_blocks": infos[0].n_net_fallback, "fgmres_batched_3rhs": comps,
+            "fgmres_batched_s": t_solve}
 
 
 if __name__ == "__main__":

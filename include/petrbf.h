@@ -256,33 +256,46 @@ prbf_status prbf_solve_dev(prbf_ctx* ctx, const prbf_solve_opts* opts, const dou
  * results equal nrhs single solves run kernel by kernel, bit for bit), while
  * every iteration runs one matvec over all of them (one exponential per
  * pair) and one preconditioner apply that streams each block's core rows
- * once.  Kernel by kernel whatever opts->graph says; not with
- * PRBF_PC_LEARNED. */
+ * once.  Kernel by kernel whatever opts->graph says.  With PRBF_PC_LEARNED
+ * (FGMRES) the network runs on each right-hand side and a network cluster
+ * moves to its exact block at a restart when its core residual, summed over
+ * the right-hand sides still running, fell by less than 1000x over the
+ * cycle. */
 prbf_status prbf_solve_multi(prbf_ctx* ctx, const prbf_solve_opts* opts, int nrhs, const double* w, double* gamma,
                              prbf_solve_info* info, double* hist, int hist_cap);
 prbf_status prbf_solve_multi_dev(prbf_ctx* ctx, const prbf_solve_opts* opts, int nrhs, const double* w,
                                  double* gamma, prbf_solve_info* info, double* hist, int hist_cap);
 
-/* ---- learned preconditioner (2D; no counterpart in the reference) --------
- * A small MLP (192 -> 256 -> 256 -> 256 -> 16, GELU, BF16 weights and
+/* ---- learned preconditioner (no counterpart in the reference) -----------
+ * A small MLP (nin -> 256 -> 256 -> 256 -> nout, GELU, BF16 weights and
  * activations on the BF16 matrix cores, FP32 accumulation) maps the residual
- * on the 8x8 lattice-slot patch around a cluster of 4x4 lattice spacings
- * (scaled by its RMS) and the particles' offsets from their slots to a
- * correction on the cluster's 16 core particles.  The network is trained on
- * the device (Adam, FP32 master weights) on pairs (patch, exact RASM core
- * solve) harvested from FGMRES runs with the exact preconditioner.  Device
- * arrays; these calls synchronise. */
+ * on the lattice-slot patch around a cluster (scaled by its RMS) and the
+ * particles' offsets from their slots to a correction on the cluster's core
+ * particles.  2D: 8x8 patch around a cluster of 4x4 lattice spacings,
+ * nin = 192 (64 residuals, 128 offsets), nout = 16.  3D: 5x5x5 patch around
+ * a cluster of 3x3x3 spacings, nin = 512 (125 residuals, 375 offsets, 12
+ * zeros), nout = 32 (27 core corrections; the other 5 outputs are trained
+ * toward zero and unused).  The network is trained on the device (Adam, FP32
+ * master weights) on pairs (patch, exact RASM core solve) harvested from
+ * FGMRES runs with the exact preconditioner.  Device arrays; these calls
+ * synchronise. */
 typedef struct prbf_mlp prbf_mlp;
+/* parameter count of the 2D network, and of the network of dimension dim
+ * (2 or 3; -1 otherwise) */
 int64_t prbf_mlp_num_params(void);
+int64_t prbf_mlp_num_params_dim(int dim);
+/* a 2D network; prbf_mlp_create_dim: of dimension 2 or 3 (Glorot-uniform
+ * weights from a splitmix64 stream of seed, zero biases) */
 prbf_status prbf_mlp_create(prbf_mlp** net, uint64_t seed);
+prbf_status prbf_mlp_create_dim(prbf_mlp** net, uint64_t seed, int dim);
 prbf_status prbf_mlp_destroy(prbf_mlp* net);
 const char* prbf_mlp_last_error(const prbf_mlp* net);
-/* params: host, prbf_mlp_num_params() floats (W0, b0, ..., W3, b3; W_l row-major in x out) */
+/* params: host, prbf_mlp_num_params_dim(dim) floats (W0, b0, ..., W3, b3; W_l row-major in x out) */
 prbf_status prbf_mlp_set_params(prbf_mlp* net, const float* params, double out_scale);
 prbf_status prbf_mlp_get_params(prbf_mlp* net, float* params, double* out_scale);
-/* Y (m x 16) = net(X (m x 192)), row-major device arrays */
+/* Y (m x nout) = net(X (m x nin)), row-major device arrays */
 prbf_status prbf_mlp_forward_dev(prbf_mlp* net, int64_t m, const float* X, float* Y);
-/* mean-square loss against T (m x 16) and its gradient (device, num_params floats); m % 16 == 0 */
+/* mean-square loss against T (m x nout) and its gradient (device, num_params floats); m % 16 == 0 */
 prbf_status prbf_mlp_loss_grad_dev(prbf_mlp* net, int64_t m, const float* X, const float* T, float* loss,
                                    float* grad);
 /* `steps` Adam steps on minibatches of `batch` pairs drawn (seeded) from the
@@ -291,18 +304,20 @@ prbf_status prbf_mlp_train_dev(prbf_mlp* net, int64_t npairs, const float* X, co
                                float lr, uint64_t seed, float* loss_hist);
 /* number of clusters with a complete patch (the network's candidates) */
 prbf_status prbf_learned_eligible(prbf_ctx* ctx, int64_t* n);
-/* For the sorted vector r: features X (n x 192) of the eligible clusters, the
+/* For the sorted vector r: features X (n x nin) of the eligible clusters, the
  * patch RMS s (n), and with z != NULL the targets T = z_core / (s out_scale)
- * (n x 16); at most cap rows. */
+ * (n x nout, zeros past the core); at most cap rows.  nin, nout: those of
+ * the context's dimension. */
 prbf_status prbf_learned_pairs_dev(prbf_ctx* ctx, const double* r, const double* z, double out_scale, float* X,
                                    float* T, double* s, int64_t cap, int64_t* n);
 /* rho = |r - A z| / |r| on each eligible cluster's patch for the core values z */
 prbf_status prbf_learned_rho_dev(prbf_ctx* ctx, const double* r, const double* z, double* rho, int64_t cap,
                                  int64_t* n);
-/* Attach a trained network: an eligible cluster uses it when its patch
- * residual reduction rho stays <= tau on two seeded probe residuals; every
- * other cluster falls back to its exact block, factorised in FP32 here.
- * n_network receives the number of clusters on the network. */
+/* Attach a trained network of the context's dimension: an eligible cluster
+ * uses it when its patch residual reduction rho stays <= tau on two seeded
+ * probe residuals; every other cluster falls back to its exact block,
+ * factorised in FP32 here.  n_network receives the number of clusters on
+ * the network. */
 prbf_status prbf_learned_attach(prbf_ctx* ctx, prbf_mlp* net, double tau, int64_t* n_network);
 /* Copies basis vector j of the last FGMRES cycle and its preconditioned
  * image into the device arrays v and z (sorted order; for harvesting pairs). */

@@ -1,7 +1,8 @@
 """CPU oracle of the learned preconditioner's network (test infrastructure only).
 
-The network has no counterpart in the reference: it is the 192 -> 256 -> 256
--> 256 -> 16 GELU (tanh form) MLP of the library's learned preconditioner.
+The network has no counterpart in the reference: it is the nin -> 256 -> 256
+-> 256 -> nout GELU (tanh form) MLP of the library's learned preconditioner,
+2D 192 -> 16, 3D 512 -> 32 (the parameter vector's length tells them apart).
 This module restates its forward pass, mean-square loss and gradient in
 float64 numpy.  With ``bf16=True`` it rounds to bfloat16 (nearest even) at
 exactly the points where the device kernels store BF16 values: the input,
@@ -15,17 +16,21 @@ from __future__ import annotations
 
 import numpy as np
 
-DIMS = [(192, 256), (256, 256), (256, 256), (256, 16)]
+LAYERS = {2: [(192, 256), (256, 256), (256, 256), (256, 16)],
+          3: [(512, 256), (256, 256), (256, 256), (256, 32)]}
+DIMS = LAYERS[2]
 K0, K1 = 0.7978845608028654, 0.044715
 
 
-def num_params():
-    return sum(k * n + n for k, n in DIMS)
+def num_params(dim=2):
+    return sum(k * n + n for k, n in LAYERS[dim])
 
 
 def unpack(P):
+    dims = LAYERS[2] if len(P) == num_params(2) else LAYERS[3]
+    assert len(P) == sum(k * n + n for k, n in dims)
     out, o = [], 0
-    for k, n in DIMS:
+    for k, n in dims:
         W = P[o:o + k * n].reshape(k, n)
         o += k * n
         b = P[o:o + n]

@@ -32,7 +32,7 @@ EXPORTS = [
     "prbf_rasm_setup", "prbf_rasm_setup_pc", "prbf_set_tuning", "prbf_rasm_apply", "prbf_rasm_apply_dev", "prbf_rasm_get_core_rows",
     "prbf_rasm_get_block",
     "prbf_solve", "prbf_solve_dev", "prbf_solve_multi", "prbf_solve_multi_dev", "prbf_rbf_interpolation_f32",
-    "prbf_mlp_num_params", "prbf_mlp_create", "prbf_mlp_destroy", "prbf_mlp_last_error", "prbf_mlp_set_params",
+    "prbf_mlp_num_params", "prbf_mlp_num_params_dim", "prbf_mlp_create", "prbf_mlp_create_dim", "prbf_mlp_destroy", "prbf_mlp_last_error", "prbf_mlp_set_params",
     "prbf_mlp_get_params", "prbf_mlp_forward_dev", "prbf_mlp_loss_grad_dev", "prbf_mlp_train_dev",
     "prbf_learned_eligible", "prbf_learned_pairs_dev", "prbf_learned_rho_dev", "prbf_learned_attach",
     "prbf_krylov_vectors_dev",
@@ -119,7 +119,9 @@ def load():
                                              C.POINTER(PrbfSolveOpts), C.POINTER(i32)]
     net = C.c_void_p
     L.prbf_mlp_num_params.argtypes = []
+    L.prbf_mlp_num_params_dim.argtypes = [i32]
     L.prbf_mlp_create.argtypes = [C.POINTER(net), C.c_uint64]
+    L.prbf_mlp_create_dim.argtypes = [C.POINTER(net), C.c_uint64, i32]
     L.prbf_mlp_destroy.argtypes = [net]
     L.prbf_mlp_last_error.restype = C.c_char_p
     L.prbf_mlp_last_error.argtypes = [net]
@@ -137,5 +139,6 @@ def load():
         if f not in ("prbf_solve_opts_default", "prbf_last_error", "prbf_mlp_last_error"):
             getattr(L, f).restype = C.c_int
     L.prbf_mlp_num_params.restype = C.c_int64
+    L.prbf_mlp_num_params_dim.restype = C.c_int64
     _lib = L
     return L

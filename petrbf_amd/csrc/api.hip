@@ -60,6 +60,7 @@ double now_s() {
 // gradient and Adam moments, BF16 weight copies in both orientations, and the
 // activations of the largest batch run so far.
 struct prbf_mlp {
+  LGeo q = lgeo(2);  // patch geometry and layer sizes (2D or 3D network)
   hipStream_t st = nullptr;
   std::string err;
   DBuf<float> P, G, m1, m2;
@@ -72,13 +73,13 @@ struct prbf_mlp {
   DBuf<long long> idx;
   void ensure(long long m) {
     if (m <= cap) return;
-    Xb.ensure((size_t)m * kMlpIn);
+    Xb.ensure((size_t)m * q.nin);
     act.ensure((size_t)3 * m * kMlpHidden);
     pre.ensure((size_t)3 * m * kMlpHidden);
-    Y.ensure((size_t)m * kMlpOut);
-    Tb.ensure((size_t)m * kMlpOut);
-    dYf.ensure((size_t)m * kMlpOut);
-    dYb.ensure((size_t)m * kMlpOut);
+    Y.ensure((size_t)m * q.nout);
+    Tb.ensure((size_t)m * q.nout);
+    dYf.ensure((size_t)m * q.nout);
+    dYb.ensure((size_t)m * q.nout);
     dHf.ensure((size_t)2 * m * kMlpHidden);
     dHb.ensure((size_t)2 * m * kMlpHidden);
     part.ensure(256);
@@ -918,15 +919,15 @@ void rasm_setup_impl(prbf_ctx* c, int pc) {
 void network_apply(prbf_ctx* c, const int* list, long long n, const double* r, double* z) {
   if (n <= 0) return;
   prbf_mlp* net = c->net;
-  const double hl = c->g.box / 4.0;
-  patch_features(c->st, n, list, hl, c->prm.sigma, c->pidx, c->cidx, c->pctr, c->xi, c->yi, r, nullptr,
-                 net->out_scale, c->lX, nullptr, c->ls);
-  mlp_to_bf16(c->st, n * kMlpIn, c->lX, net->Xb);
+  const LGeo& q = net->q;
+  patch_features(c->st, q, n, list, c->g.box / q.cs, c->prm.sigma, c->pidx, c->cidx, c->pctr, c->xi, c->yi, c->zi,
+                 r, nullptr, net->out_scale, c->lX, nullptr, c->ls);
+  mlp_to_bf16(c->st, n * q.nin, c->lX, net->Xb);
   float* pr[3];
   unsigned short* ac[3];
   net->layers(pr, ac);
-  mlp_forward(c->st, (int)n, net->P, net->Wt, net->Xb, pr, ac, net->Y);
-  core_scatter(c->st, n, list, c->cidx, net->Y, c->ls, net->out_scale, z);
+  mlp_forward(c->st, q, (int)n, net->P, net->Wt, net->Xb, pr, ac, net->Y);
+  core_scatter(c->st, q, n, list, c->cidx, net->Y, c->ls, net->out_scale, z);
 }
 
 // z = B r for nrhs vectors stored back to back (length ni each); every block's
@@ -935,13 +936,14 @@ void rasm_apply_impl(prbf_ctx* c, const double* r, double* z, int nrhs = 1) {
   need(c->rasm_ready, PRBF_ERR_STATE, "rasm_setup first");
   need(r && z, PRBF_ERR_ARG, "null vector");
   need(nrhs >= 1 && nrhs <= 3, PRBF_ERR_ARG, "nrhs must be 1..3");
-  need(nrhs == 1 || c->rasm_pc != PRBF_PC_LEARNED, PRBF_ERR_ARG, "the learned preconditioner takes one vector");
   if (c->rasm_pc == PRBF_PC_LEARNED) need(c->learned_ready && c->net, PRBF_ERR_STATE, "prbf_learned_attach first");
   rasm_apply_fp32(c->st, nrhs, c->n32, c->l32_all ? nullptr : c->l32.p, c->bptr, c->bidx, c->ista, c->iend, c->mptr,
                   c->M32, r, c->ni, z);
   rasm_apply_fp64(c->st, nrhs, c->n64, c->l64_all ? nullptr : c->l64.p, c->bptr, c->bidx, c->ista, c->iend,
                   c->mptr64, c->M, r, c->ni, z);
-  if (c->rasm_pc == PRBF_PC_LEARNED) network_apply(c, c->mlp_list, c->n_mlp, r, z);
+  if (c->rasm_pc == PRBF_PC_LEARNED)
+    for (int k = 0; k < nrhs; ++k)
+      network_apply(c, c->mlp_list, c->n_mlp, r + (long long)k * c->ni, z + (long long)k * c->ni);
   PRBF_HIP(hipGetLastError());
 }
 // ---------------------------------------------------------------- Krylov
@@ -1158,7 +1160,7 @@ void krylov_impl(prbf_ctx* c, const prbf_solve_opts& o, const double* b, double*
       // its exact block
       const long long nm = c->n_mlp;
       std::vector<double> cur(nm);
-      core_norms(c->st, nm, c->mlp_list, c->cidx, tmp, c->lnorm);
+      core_norms(c->st, c->net->q, nm, c->mlp_list, c->cidx, tmp, c->lnorm);
       PRBF_HIP(hipMemcpyAsync(cur.data(), c->lnorm, sizeof(double) * nm, hipMemcpyDeviceToHost, c->st));
       sync(c);
       std::vector<int> newc;
@@ -1308,8 +1310,8 @@ void krylov_batch_impl(prbf_ctx* c, const prbf_solve_opts& o, int k, const doubl
   need(k >= 1 && k <= 3, PRBF_ERR_ARG, "nrhs must be 1..3");
   need(m >= 1 && m < 63, PRBF_ERR_ARG, "restart must be in 1..62");
   need(o.pc == PRBF_PC_NONE || c->rasm_ready, PRBF_ERR_STATE, "rasm_setup first");
-  need(o.pc != PRBF_PC_LEARNED, PRBF_ERR_ARG, "the learned preconditioner takes one right-hand side");
   const bool flex = (o.ksp == PRBF_KSP_FGMRES);
+  need(flex || o.pc != PRBF_PC_LEARNED, PRBF_ERR_ARG, "the learned preconditioner needs FGMRES");
   c->last_restart = 0;  // the interleaved bases are not single-vector bases
   c->V.ensure((size_t)(m + 1) * kn);
   if (flex) c->Z.ensure((size_t)m * kn);
@@ -1357,6 +1359,7 @@ void krylov_batch_impl(prbf_ctx* c, const prbf_solve_opts& o, int k, const doubl
   };
   const double t0 = now_s();
   fill(c->st, kn, x, 0.0);
+  std::vector<double> prev_core;  // network clusters' core residuals at the last cycle start
   auto any_active = [&]() {
     for (const State& q : S)
       if (!q.reason) return true;
@@ -1377,6 +1380,28 @@ void krylov_batch_impl(prbf_ctx* c, const prbf_solve_opts& o, int k, const doubl
         PRBF_HIP(hipMemcpyAsync(tmp + r * n, b + r * n, sizeof(double) * n, hipMemcpyDeviceToDevice, c->st));
       else
         xpay_sub(c->st, n, b + r * n, tmp2 + r * n, tmp + r * n);
+    }
+    if (o.pc == PRBF_PC_LEARNED && c->n_mlp > 0) {
+      // as krylov_impl, on the network clusters' core residuals summed over
+      // the right-hand sides still running
+      const long long nm = c->n_mlp;
+      std::vector<double> cur(nm, 0.0), one(nm);
+      for (int r = 0; r < k; ++r) {
+        if (S[r].reason) continue;
+        core_norms(c->st, c->net->q, nm, c->mlp_list, c->cidx, tmp + r * n, c->lnorm);
+        PRBF_HIP(hipMemcpyAsync(one.data(), c->lnorm, sizeof(double) * nm, hipMemcpyDeviceToHost, c->st));
+        sync(c);
+        for (long long i = 0; i < nm; ++i) cur[i] += one[i];
+      }
+      std::vector<int> newc;
+      if (!prev_core.empty())
+        for (long long i = 0; i < nm; ++i) {
+          const int cl = c->h_mlp_list[i];
+          if (!(cur[i] <= kNetCycleDrop * kNetCycleDrop * prev_core[cl])) newc.push_back(cl);
+        }
+      prev_core.assign(c->nclus, 0.0);
+      for (long long i = 0; i < nm; ++i) prev_core[c->h_mlp_list[i]] = cur[i];
+      learned_fallback(c, newc);
     }
     if (flex)
       PRBF_HIP(hipMemcpyAsync(Vr(0, 0), tmp, sizeof(double) * kn, hipMemcpyDeviceToDevice, c->st));
@@ -1500,6 +1525,10 @@ void krylov_batch_impl(prbf_ctx* c, const prbf_solve_opts& o, int k, const doubl
     inf[r].rnorm0 = S[r].conv.rnorm0;
     inf[r].rnorm = S[r].last_res;
     inf[r].t_solve_s = now_s() - t0;
+    if (o.pc == PRBF_PC_LEARNED) {
+      inf[r].n_network = (int)c->n_mlp;
+      inf[r].n_net_fallback = c->n_net_fallback;
+    }
     if (info) info[r] = inf[r];
   }
 }
@@ -1533,7 +1562,6 @@ void solve_multi_dev_impl(prbf_ctx* c, const prbf_solve_opts* opts, int k, const
   need(k >= 1 && k <= 3, PRBF_ERR_ARG, "nrhs must be 1..3");
   double t_setup = 0.0;
   const prbf_solve_opts o = checked_opts(c, opts, hist_cap, &t_setup);
-  need(o.pc != PRBF_PC_LEARNED, PRBF_ERR_ARG, "the learned preconditioner takes one right-hand side");
   const long long n = c->ni;
   c->v3.ensure((size_t)2 * k * n);
   double* bs = c->v3;
@@ -2115,20 +2143,21 @@ prbf_status guarded_mlp(prbf_mlp* net, F&& f) {
 
 void mlp_sync(prbf_mlp* net) { PRBF_HIP(hipStreamSynchronize(net->st)); }
 
-void mlp_refresh_bf16(prbf_mlp* net) { mlp_weights_bf16(net->st, net->P, net->Wb, net->Wt); }
+void mlp_refresh_bf16(prbf_mlp* net) { mlp_weights_bf16(net->st, net->q, net->P, net->Wb, net->Wt); }
 
 // Patch slots of every cluster and the list of eligible clusters.
 void ensure_patches(prbf_ctx* c) {
   need(c->has_points && c->same, PRBF_ERR_STATE, "set_points (one set) first");
-  need(c->prm.dim == 2, PRBF_ERR_ARG, "the learned preconditioner is two-dimensional");
   need(!c->tree, PRBF_ERR_ARG, "the learned preconditioner works on the uniform grid's clusters");
   if (c->patches_ready) return;
+  const LGeo q = lgeo(c->prm.dim);
   const long long nc = c->g.ncell;
-  c->pidx.ensure((size_t)nc * kPatch);
-  c->cidx.ensure((size_t)nc * kMlpOut);
+  c->pidx.ensure((size_t)nc * q.np);
+  c->cidx.ensure((size_t)nc * q.nc);
   c->pok.ensure((size_t)nc);
-  c->pctr.ensure((size_t)2 * nc);
-  patch_build(c->st, c->g, c->g.box / 4.0, c->ista, c->iend, c->xi, c->yi, c->pidx, c->cidx, c->pctr, c->pok);
+  c->pctr.ensure((size_t)3 * nc);
+  patch_build(c->st, c->g, q, c->g.box / q.cs, c->ista, c->iend, c->xi, c->yi, c->zi, c->pidx, c->cidx, c->pctr,
+              c->pok);
   PRBF_HIP(hipGetLastError());
   std::vector<unsigned char> h(nc);
   PRBF_HIP(hipMemcpyAsync(h.data(), c->pok, nc, hipMemcpyDeviceToHost, c->st));
@@ -2148,20 +2177,29 @@ void ensure_patches(prbf_ctx* c) {
 
 extern "C" {
 
-int64_t prbf_mlp_num_params(void) { return mlp_nparams(); }
+int64_t prbf_mlp_num_params(void) { return mlp_nparams(lgeo(2)); }
 
-prbf_status prbf_mlp_create(prbf_mlp** out, uint64_t seed) {
+int64_t prbf_mlp_num_params_dim(int dim) { return dim == 2 || dim == 3 ? mlp_nparams(lgeo(dim)) : -1; }
+
+prbf_status prbf_mlp_create(prbf_mlp** out, uint64_t seed) { return prbf_mlp_create_dim(out, seed, 2); }
+
+prbf_status prbf_mlp_create_dim(prbf_mlp** out, uint64_t seed, int dim) {
   if (!out) return PRBF_ERR_ARG;
   *out = nullptr;
-  prbf_mlp* net = new prbf_mlp;
+  if (dim != 2 && dim != 3) return PRBF_ERR_ARG;
+  prbf_mlp* net = new (std::nothrow) prbf_mlp;
+  if (!net) return PRBF_ERR_NOMEM;
+  net->q = lgeo(dim);
   if (hipStreamCreateWithFlags(&net->st, hipStreamNonBlocking) != hipSuccess) {
     (void)hipGetLastError();
     delete net;
     return PRBF_ERR_HIP;
   }
   prbf_status st = guarded_mlp(net, [&] {
-    const long long np = mlp_nparams();
-    // Glorot-uniform weights, zero biases, from a splitmix64 stream
+    const LGeo& q = net->q;
+    const long long np = mlp_nparams(q);
+    // Glorot-uniform weights, zero biases, from a splitmix64 stream (3D: the
+    // weights into the pad outputs are zero, so those outputs start at zero)
     std::vector<float> h(np, 0.0f);
     unsigned long long z = seed;
     auto next = [&]() {
@@ -2174,9 +2212,12 @@ prbf_status prbf_mlp_create(prbf_mlp** out, uint64_t seed) {
     };
     for (int l = 0; l < kMlpLayers; ++l) {
       int K, N;
-      mlp_dims(l, &K, &N);
-      const double a = std::sqrt(6.0 / (K + N));
-      for (long long i = 0; i < (long long)K * N; ++i) h[mlp_w_off(l) + i] = (float)((2.0 * next() - 1.0) * a);
+      mlp_dims(q, l, &K, &N);
+      const double a = std::sqrt(6.0 / (K + (l == kMlpLayers - 1 ? q.nc : N)));
+      for (long long i = 0; i < (long long)K * N; ++i) {
+        const double v = (2.0 * next() - 1.0) * a;
+        h[mlp_w_off(q, l) + i] = (l == kMlpLayers - 1 && i % N >= q.nc) ? 0.0f : (float)v;
+      }
     }
     net->P.ensure(np);
     net->G.ensure(np);
@@ -2213,7 +2254,7 @@ const char* prbf_mlp_last_error(const prbf_mlp* net) { return net ? net->err.c_s
 prbf_status prbf_mlp_set_params(prbf_mlp* net, const float* params, double out_scale) {
   return guarded_mlp(net, [&] {
     need(params != nullptr && out_scale > 0, PRBF_ERR_ARG, "params and a positive out_scale required");
-    const long long np = mlp_nparams();
+    const long long np = mlp_nparams(net->q);
     PRBF_HIP(hipMemcpyAsync(net->P, params, sizeof(float) * np, hipMemcpyHostToDevice, net->st));
     PRBF_HIP(hipMemsetAsync(net->m1, 0, sizeof(float) * np, net->st));
     PRBF_HIP(hipMemsetAsync(net->m2, 0, sizeof(float) * np, net->st));
@@ -2226,7 +2267,8 @@ prbf_status prbf_mlp_set_params(prbf_mlp* net, const float* params, double out_s
 
 prbf_status prbf_mlp_get_params(prbf_mlp* net, float* params, double* out_scale) {
   return guarded_mlp(net, [&] {
-    if (params) PRBF_HIP(hipMemcpyAsync(params, net->P, sizeof(float) * mlp_nparams(), hipMemcpyDeviceToHost, net->st));
+    if (params)
+      PRBF_HIP(hipMemcpyAsync(params, net->P, sizeof(float) * mlp_nparams(net->q), hipMemcpyDeviceToHost, net->st));
     if (out_scale) *out_scale = net->out_scale;
     mlp_sync(net);
   });
@@ -2239,9 +2281,9 @@ prbf_status prbf_mlp_forward_dev(prbf_mlp* net, int64_t m, const float* X, float
     float* pr[3];
     unsigned short* ac[3];
     net->layers(pr, ac);
-    mlp_to_bf16(net->st, m * kMlpIn, X, net->Xb);
-    mlp_forward(net->st, (int)m, net->P, net->Wt, net->Xb, pr, ac, net->Y);
-    PRBF_HIP(hipMemcpyAsync(Y, net->Y, sizeof(float) * m * kMlpOut, hipMemcpyDeviceToDevice, net->st));
+    mlp_to_bf16(net->st, m * net->q.nin, X, net->Xb);
+    mlp_forward(net->st, net->q, (int)m, net->P, net->Wt, net->Xb, pr, ac, net->Y);
+    PRBF_HIP(hipMemcpyAsync(Y, net->Y, sizeof(float) * m * net->q.nout, hipMemcpyDeviceToDevice, net->st));
     PRBF_HIP(hipGetLastError());
     mlp_sync(net);
   });
@@ -2255,12 +2297,12 @@ prbf_status prbf_mlp_loss_grad_dev(prbf_mlp* net, int64_t m, const float* X, con
     float* pr[3];
     unsigned short* ac[3];
     net->layers(pr, ac);
-    mlp_to_bf16(net->st, m * kMlpIn, X, net->Xb);
-    mlp_forward(net->st, (int)m, net->P, net->Wt, net->Xb, pr, ac, net->Y);
-    mlp_backward(net->st, (int)m, net->P, net->Wb, net->Xb, pr, ac, net->Y, T, net->G, net->dYf, net->dYb,
+    mlp_to_bf16(net->st, m * net->q.nin, X, net->Xb);
+    mlp_forward(net->st, net->q, (int)m, net->P, net->Wt, net->Xb, pr, ac, net->Y);
+    mlp_backward(net->st, net->q, (int)m, net->P, net->Wb, net->Xb, pr, ac, net->Y, T, net->G, net->dYf, net->dYb,
                  net->dHf, net->dHb, net->part, net->loss);
     if (grad)
-      PRBF_HIP(hipMemcpyAsync(grad, net->G, sizeof(float) * mlp_nparams(), hipMemcpyDeviceToDevice, net->st));
+      PRBF_HIP(hipMemcpyAsync(grad, net->G, sizeof(float) * mlp_nparams(net->q), hipMemcpyDeviceToDevice, net->st));
     float h = 0.0f;
     PRBF_HIP(hipMemcpyAsync(&h, net->loss, sizeof(float), hipMemcpyDeviceToHost, net->st));
     PRBF_HIP(hipGetLastError());
@@ -2279,12 +2321,12 @@ prbf_status prbf_mlp_train_dev(prbf_mlp* net, int64_t npairs, const float* X, co
     float* pr[3];
     unsigned short* ac[3];
     net->layers(pr, ac);
-    const long long np = mlp_nparams();
+    const long long np = mlp_nparams(net->q);
     for (int s = 0; s < steps; ++s) {
       mlp_random_rows(net->st, batch, npairs, seed, s, net->idx);
-      mlp_gather_rows(net->st, batch, net->idx, X, T, net->Xb, net->Tb);
-      mlp_forward(net->st, batch, net->P, net->Wt, net->Xb, pr, ac, net->Y);
-      mlp_backward(net->st, batch, net->P, net->Wb, net->Xb, pr, ac, net->Y, net->Tb, net->G, net->dYf, net->dYb,
+      mlp_gather_rows(net->st, net->q, batch, net->idx, X, T, net->Xb, net->Tb);
+      mlp_forward(net->st, net->q, batch, net->P, net->Wt, net->Xb, pr, ac, net->Y);
+      mlp_backward(net->st, net->q, batch, net->P, net->Wb, net->Xb, pr, ac, net->Y, net->Tb, net->G, net->dYf, net->dYb,
                    net->dHf, net->dHb, net->part, net->loss);
       copy_f32(net->st, net->loss, net->lhist.p + s);
       mlp_adam(net->st, np, net->P, net->G, net->m1, net->m2, lr, ++net->t);
@@ -2310,8 +2352,9 @@ prbf_status prbf_learned_pairs_dev(prbf_ctx* c, const double* r, const double* z
     need(r && X && s && cap >= 0 && out_scale > 0 && (!z || T), PRBF_ERR_ARG, "invalid arguments");
     ensure_patches(c);
     const long long m = std::min<long long>(cap, c->n_elig);
-    patch_features(c->st, m, c->elig, c->g.box / 4.0, c->prm.sigma, c->pidx, c->cidx, c->pctr, c->xi, c->yi, r, z,
-                   out_scale, X, T, s);
+    const LGeo q = lgeo(c->prm.dim);
+    patch_features(c->st, q, m, c->elig, c->g.box / q.cs, c->prm.sigma, c->pidx, c->cidx, c->pctr, c->xi, c->yi,
+                   c->zi, r, z, out_scale, X, T, s);
     PRBF_HIP(hipGetLastError());
     if (n) *n = m;
   });
@@ -2323,7 +2366,7 @@ prbf_status prbf_learned_rho_dev(prbf_ctx* c, const double* r, const double* z, 
     need(r && z && rho && cap >= 0, PRBF_ERR_ARG, "invalid arguments");
     ensure_patches(c);
     const long long m = std::min<long long>(cap, c->n_elig);
-    patch_rho(c->st, m, c->elig, c->pidx, c->cidx, c->xi, c->yi, r, z, c->prm.sigma, rho);
+    patch_rho(c->st, lgeo(c->prm.dim), m, c->elig, c->pidx, c->cidx, c->xi, c->yi, c->zi, r, z, c->prm.sigma, rho);
     PRBF_HIP(hipGetLastError());
     if (n) *n = m;
   });
@@ -2332,6 +2375,7 @@ prbf_status prbf_learned_rho_dev(prbf_ctx* c, const double* r, const double* z, 
 prbf_status prbf_learned_attach(prbf_ctx* c, prbf_mlp* net, double tau, int64_t* n_network) {
   return guarded(c, [&] {
     need(net != nullptr && tau >= 0, PRBF_ERR_ARG, "a network and tau >= 0 required");
+    need(net->q.dim == c->prm.dim, PRBF_ERR_ARG, "the network's dimension differs from the context's");
     ensure_patches(c);
     c->learned_ready = false;
     c->net = net;
@@ -2340,7 +2384,7 @@ prbf_status prbf_learned_attach(prbf_ctx* c, prbf_mlp* net, double tau, int64_t*
     if (ne > 0) {
       mlp_sync(net);  // the network's own work is done before the context's stream uses it
       net->ensure(ne);
-      c->lX.ensure((size_t)ne * kMlpIn);
+      c->lX.ensure((size_t)ne * net->q.nin);
       c->ls.ensure((size_t)ne);
       DBuf<double> g, r, z, rho;
       g.ensure(n);
@@ -2363,7 +2407,7 @@ prbf_status prbf_learned_attach(prbf_ctx* c, prbf_mlp* net, double tau, int64_t*
         matvec_impl(c, 1, g, r);
         fill(c->st, n, z, 0.0);
         network_apply(c, c->elig, ne, r, z);
-        patch_rho(c->st, ne, c->elig, c->pidx, c->cidx, c->xi, c->yi, r, z, c->prm.sigma, rho);
+        patch_rho(c->st, net->q, ne, c->elig, c->pidx, c->cidx, c->xi, c->yi, c->zi, r, z, c->prm.sigma, rho);
         PRBF_HIP(hipMemcpyAsync(hr.data(), rho, sizeof(double) * ne, hipMemcpyDeviceToHost, c->st));
         sync(c);
         for (long long k = 0; k < ne; ++k)

@@ -141,37 +141,56 @@ void scale_dev(hipStream_t st, long long n, double* v, const double* s);  // v *
 void set_scalar(hipStream_t st, double* p, double a);
 
 // learned.hip -----------------------------------------------------------------
-constexpr int kPatch = 64;      // 8 x 8 lattice slots around a 4 x 4 cluster
-constexpr int kMlpIn = 3 * kPatch;
+// Patch geometry of the learned preconditioner: a cluster of cs^dim lattice
+// sites (its core) inside a patch of ps^dim lattice slots.  2D: 4x4 core in an
+// 8x8 patch; 3D: 3x3x3 core in a 5x5x5 patch.  The network maps nin inputs
+// (np scaled residuals, then dim x np slot offsets, zero-padded to a multiple
+// of 16) through 3 hidden layers of 256 to nout outputs: the nc core
+// corrections, zero-padded to a multiple of 16 (the pad outputs' targets are
+// zero and they are never used).
+struct LGeo {
+  int dim, ps, cs, np, nc, nin, nout;
+};
+inline LGeo lgeo(int dim) {
+  return dim == 3 ? LGeo{3, 5, 3, 125, 27, 512, 32} : LGeo{2, 8, 4, 64, 16, 192, 16};
+}
 constexpr int kMlpHidden = 256;
-constexpr int kMlpOut = 16;     // the cluster's 4 x 4 core
 constexpr int kMlpLayers = 4;
-void mlp_dims(int l, int* K, int* N);
-long long mlp_w_off(int l);
-long long mlp_b_off(int l);
-long long mlp_nparams();
-void mlp_weights_bf16(hipStream_t st, const float* P, unsigned short* Wb, unsigned short* Wt);
-void mlp_forward(hipStream_t st, int M, const float* P, const unsigned short* Wt, const unsigned short* Xb,
-                 float* const pre[3], unsigned short* const act[3], float* Y);
-void mlp_backward(hipStream_t st, int M, const float* P, const unsigned short* Wb, const unsigned short* Xb,
-                  float* const pre[3], unsigned short* const act[3], const float* Y, const float* T, float* G,
-                  float* dYf, unsigned short* dYb, float* dHf, unsigned short* dHb, float* part, float* loss);
-void mlp_gather_rows(hipStream_t st, int M, const long long* idx, const float* X, const float* T,
+constexpr int kMaxPatch = 125;  // the largest np
+constexpr int kMaxCore = 27;    // the largest nc
+void mlp_dims(const LGeo& q, int l, int* K, int* N);
+long long mlp_w_off(const LGeo& q, int l);
+long long mlp_b_off(const LGeo& q, int l);
+long long mlp_nparams(const LGeo& q);
+void mlp_weights_bf16(hipStream_t st, const LGeo& q, const float* P, unsigned short* Wb, unsigned short* Wt);
+void mlp_forward(hipStream_t st, const LGeo& q, int M, const float* P, const unsigned short* Wt,
+                 const unsigned short* Xb, float* const pre[3], unsigned short* const act[3], float* Y);
+void mlp_backward(hipStream_t st, const LGeo& q, int M, const float* P, const unsigned short* Wb,
+                  const unsigned short* Xb, float* const pre[3], unsigned short* const act[3], const float* Y,
+                  const float* T, float* G, float* dYf, unsigned short* dYb, float* dHf, unsigned short* dHb,
+                  float* part, float* loss);
+void mlp_gather_rows(hipStream_t st, const LGeo& q, int M, const long long* idx, const float* X, const float* T,
                      unsigned short* Xb, float* Tb);
 void mlp_to_bf16(hipStream_t st, long long n, const float* a, unsigned short* b);
 void mlp_adam(hipStream_t st, long long n, float* p, const float* g, float* m1, float* m2, float lr, int t);
 void mlp_random_rows(hipStream_t st, int m, long long npairs, unsigned long long seed, long long step,
                      long long* idx);
 void copy_f32(hipStream_t st, const float* a, float* b);
-void patch_build(hipStream_t st, const Grid& g, double hl, const int* sta, const int* end, const double* xs,
-                 const double* ys, int* pidx, int* cidx, double* pctr, unsigned char* pok);
-void patch_features(hipStream_t st, long long nk, const int* list, double hl, double sigma, const int* pidx,
-                    const int* cidx, const double* pctr, const double* xs, const double* ys, const double* r,
-                    const double* z, double out_scale, float* X, float* T, double* s_out);
-void core_scatter(hipStream_t st, long long nk, const int* list, const int* cidx, const float* Y, const double* s,
-                  double out_scale, double* z);
-void core_norms(hipStream_t st, long long nk, const int* list, const int* cidx, const double* r, double* out);
-void patch_rho(hipStream_t st, long long nk, const int* list, const int* pidx, const int* cidx, const double* xs,
-               const double* ys, const double* r, const double* z, double sigma, double* rho);
+// pidx: ncell x np slot particles; cidx: ncell x nc core particles; pctr:
+// ncell x 3 anchors; pok[c] = 1 when cluster c is eligible
+void patch_build(hipStream_t st, const Grid& g, const LGeo& q, double hl, const int* sta, const int* end,
+                 const double* xs, const double* ys, const double* zs, int* pidx, int* cidx, double* pctr,
+                 unsigned char* pok);
+void patch_features(hipStream_t st, const LGeo& q, long long nk, const int* list, double hl, double sigma,
+                    const int* pidx, const int* cidx, const double* pctr, const double* xs, const double* ys,
+                    const double* zs, const double* r, const double* z, double out_scale, float* X, float* T,
+                    double* s_out);
+void core_scatter(hipStream_t st, const LGeo& q, long long nk, const int* list, const int* cidx, const float* Y,
+                  const double* s, double out_scale, double* z);
+void core_norms(hipStream_t st, const LGeo& q, long long nk, const int* list, const int* cidx, const double* r,
+                double* out);
+void patch_rho(hipStream_t st, const LGeo& q, long long nk, const int* list, const int* pidx, const int* cidx,
+               const double* xs, const double* ys, const double* zs, const double* r, const double* z, double sigma,
+               double* rho);
 
 }  // namespace prbf

@@ -1,19 +1,24 @@
-// Learned preconditioner (2D): a small MLP maps a cluster's residual patch
-// to a correction on the cluster's core particles.
+// Learned preconditioner: a small MLP maps a cluster's residual patch to a
+// correction on the cluster's core particles.
 //
-//   patch   the 8x8 lattice slots around a cluster of 4x4 lattice spacings
-//           (the cluster's own 16 particles plus a ring of two spacings),
-//           slot (sx, sy) = floor((x - xm)/hl + 4) with hl = box/4 and xm the
-//           mean of the cluster's particles; a cluster is eligible when every
-//           slot holds exactly one particle and its 16 core particles fill
-//           the inner 4x4 slots;
-//   input   64 residuals divided by their RMS s, then the 64 offsets of the
-//           particles from their slot centres in units of sigma (x then y);
-//   network 192 -> 256 -> 256 -> 256 -> 16, GELU (tanh form) after each
+//   patch   2D: the 8x8 lattice slots around a cluster of 4x4 lattice
+//           spacings (its own 16 particles plus a ring of two spacings);
+//           3D: the 5x5x5 slots around a cluster of 3x3x3 spacings (its 27
+//           particles plus a ring of one spacing).  Slot coordinate
+//           floor((x - xm)/hl + ps/2) per axis with hl = box/cs (the lattice
+//           spacing) and xm the mean of the cluster's particles; a cluster is
+//           eligible when every slot holds exactly one particle and its core
+//           particles fill the inner slots;
+//   input   the np residuals divided by their RMS s, then the np offsets of
+//           the particles from their slot centres in units of sigma (x, y,
+//           then z), zero-padded to a multiple of 16 (2D 192, 3D 512);
+//   network nin -> 256 -> 256 -> 256 -> nout, GELU (tanh form) after each
 //           hidden layer; BF16 weights and activations on the BF16 MFMA
 //           (v_mfma_f32_16x16x16_bf16) with FP32 accumulation, FP32 master
 //           weights trained with Adam;
-//   output  16 core corrections, multiplied back by s and a fixed output scale.
+//   output  the nc core corrections (2D 16, 3D 27; 3D pads to 32 outputs
+//           trained toward zero), multiplied back by s and a fixed output
+//           scale.
 //
 // The GEMM kernel computes C = A B for A(m, k) = A[m*sam + k*sak] and
 // B(k, n) = B[k*sbk + n*sbn] in BF16 with FP32 accumulation: 64x64 output
@@ -224,50 +229,48 @@ __global__ void k_gather_rows(int M, int cx, int ct, const long long* __restrict
 static inline dim3 g256(long long n) { return dim3((unsigned)((n + 255) / 256)); }
 
 // ---------------------------------------------------------------- the network
-// Layer l maps K_l -> N_l: (192, 256), (256, 256), (256, 256), (256, 16).
-// Parameters (FP32): W_0, b_0, W_1, b_1, ... with W_l row-major K_l x N_l.
-void mlp_dims(int l, int* K, int* N) {
-  static const int kK[kMlpLayers] = {kMlpIn, kMlpHidden, kMlpHidden, kMlpHidden};
-  static const int kN[kMlpLayers] = {kMlpHidden, kMlpHidden, kMlpHidden, kMlpOut};
-  *K = kK[l];
-  *N = kN[l];
+// Layer l maps K_l -> N_l: (nin, 256), (256, 256), (256, 256), (256, nout);
+// 2D (192, 16), 3D (512, 32).  Parameters (FP32): W_0, b_0, W_1, b_1, ...
+// with W_l row-major K_l x N_l.
+void mlp_dims(const LGeo& q, int l, int* K, int* N) {
+  *K = l == 0 ? q.nin : kMlpHidden;
+  *N = l == kMlpLayers - 1 ? q.nout : kMlpHidden;
 }
-long long mlp_w_off(int l) {
+long long mlp_w_off(const LGeo& q, int l) {
   long long o = 0;
   for (int i = 0; i < l; ++i) {
     int K, N;
-    mlp_dims(i, &K, &N);
+    mlp_dims(q, i, &K, &N);
     o += (long long)K * N + N;
   }
   return o;
 }
-long long mlp_b_off(int l) {
+long long mlp_b_off(const LGeo& q, int l) {
   int K, N;
-  mlp_dims(l, &K, &N);
-  return mlp_w_off(l) + (long long)K * N;
+  mlp_dims(q, l, &K, &N);
+  return mlp_w_off(q, l) + (long long)K * N;
 }
-long long mlp_nparams() { return mlp_w_off(kMlpLayers); }
+long long mlp_nparams(const LGeo& q) { return mlp_w_off(q, kMlpLayers); }
 
-void mlp_weights_bf16(hipStream_t st, const float* P, unsigned short* Wb, unsigned short* Wt) {
+void mlp_weights_bf16(hipStream_t st, const LGeo& q, const float* P, unsigned short* Wb, unsigned short* Wt) {
   for (int l = 0; l < kMlpLayers; ++l) {
     int K, N;
-    mlp_dims(l, &K, &N);
-    const long long off = mlp_w_off(l);
+    mlp_dims(q, l, &K, &N);
+    const long long off = mlp_w_off(q, l);
     hipLaunchKernelGGL(k_weights_bf16, g256((long long)K * N), dim3(256), 0, st, K, N, P + off, Wb + off, Wt + off);
   }
 }
 
-// Forward over M rows of Xb (BF16, M x 192).  act: per hidden layer M x 256
-// BF16; pre: per hidden layer M x 256 FP32 (nullable at inference: then pre0
-// is a single reused M x 256 buffer); Y: M x 16 FP32.
-void mlp_forward(hipStream_t st, int M, const float* P, const unsigned short* Wt, const unsigned short* Xb,
-                 float* const pre[3], unsigned short* const act[3], float* Y) {
+// Forward over M rows of Xb (BF16, M x nin).  act: per hidden layer M x 256
+// BF16; pre: per hidden layer M x 256 FP32; Y: M x nout FP32.
+void mlp_forward(hipStream_t st, const LGeo& q, int M, const float* P, const unsigned short* Wt,
+                 const unsigned short* Xb, float* const pre[3], unsigned short* const act[3], float* Y) {
   const unsigned short* in = Xb;
   for (int l = 0; l < kMlpLayers; ++l) {
     int K, N;
-    mlp_dims(l, &K, &N);
-    const unsigned short* Bt = Wt + mlp_w_off(l);  // N x K: B(k, n) = Bt[n*K + k]
-    const float* bias = P + mlp_b_off(l);
+    mlp_dims(q, l, &K, &N);
+    const unsigned short* Bt = Wt + mlp_w_off(q, l);  // N x K: B(k, n) = Bt[n*K + k]
+    const float* bias = P + mlp_b_off(q, l);
     if (l < kMlpLayers - 1) {
       gemm_launch<EPI_GELU>(st, M, N, K, in, K, 1, Bt, 1, K, bias, nullptr, pre[l], act[l], N);
       in = act[l];
@@ -278,12 +281,13 @@ void mlp_forward(hipStream_t st, int M, const float* P, const unsigned short* Wt
 }
 
 // Backward of the mean-square loss for targets T: gradient into G (same
-// packing as P).  dY buffers: M x 16; dact/dpre: M x 256 (FP32 and BF16).
-void mlp_backward(hipStream_t st, int M, const float* P, const unsigned short* Wb, const unsigned short* Xb,
-                  float* const pre[3], unsigned short* const act[3], const float* Y, const float* T, float* G,
-                  float* dYf, unsigned short* dYb, float* dHf, unsigned short* dHb, float* part, float* loss) {
+// packing as P).  dY buffers: M x nout; dact/dpre: M x 256 (FP32 and BF16).
+void mlp_backward(hipStream_t st, const LGeo& q, int M, const float* P, const unsigned short* Wb,
+                  const unsigned short* Xb, float* const pre[3], unsigned short* const act[3], const float* Y,
+                  const float* T, float* G, float* dYf, unsigned short* dYb, float* dHf, unsigned short* dHb,
+                  float* part, float* loss) {
   (void)P;
-  const long long ny = (long long)M * kMlpOut;
+  const long long ny = (long long)M * q.nout;
   const int nblk = 256;
   hipLaunchKernelGGL(k_mse_grad, dim3(nblk), dim3(256), 0, st, ny, 1.0f / (float)ny, Y, T, dYf, dYb, part);
   hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, nblk, part, loss, 1.0f / (float)ny);
@@ -292,24 +296,24 @@ void mlp_backward(hipStream_t st, int M, const float* P, const unsigned short* W
   const unsigned short* gb = dYb;
   for (int l = kMlpLayers - 1; l >= 0; --l) {
     int K, N;
-    mlp_dims(l, &K, &N);
+    mlp_dims(q, l, &K, &N);
     const unsigned short* in = l == 0 ? Xb : act[l - 1];
     // dW_l = in^T (K x M) * g (M x N): A(k, m) = in[m*K + k], B(m, n) = g[m*N + n]
-    gemm_launch<EPI_F32>(st, K, N, M, in, 1, K, gb, N, 1, nullptr, nullptr, G + mlp_w_off(l), nullptr, N);
-    hipLaunchKernelGGL(k_colsum, dim3((N + 255) / 256), dim3(256), 0, st, M, N, gf, G + mlp_b_off(l));
+    gemm_launch<EPI_F32>(st, K, N, M, in, 1, K, gb, N, 1, nullptr, nullptr, G + mlp_w_off(q, l), nullptr, N);
+    hipLaunchKernelGGL(k_colsum, dim3((N + 255) / 256), dim3(256), 0, st, M, N, gf, G + mlp_b_off(q, l));
     if (l == 0) break;
     // d pre_{l-1} = (g W_l^T) * gelu'(pre_{l-1}): A(m, n) = g[m*N + n], B(n, k) = W_l[k*N + n]
     float* hf = dHf + (long long)(l & 1) * M * kMlpHidden;
     unsigned short* hb = dHb + (long long)(l & 1) * M * kMlpHidden;
-    gemm_launch<EPI_DGELU>(st, M, K, N, gb, N, 1, Wb + mlp_w_off(l), 1, N, nullptr, pre[l - 1], hf, hb, K);
+    gemm_launch<EPI_DGELU>(st, M, K, N, gb, N, 1, Wb + mlp_w_off(q, l), 1, N, nullptr, pre[l - 1], hf, hb, K);
     gf = hf;
     gb = hb;
   }
 }
 
-void mlp_gather_rows(hipStream_t st, int M, const long long* idx, const float* X, const float* T,
+void mlp_gather_rows(hipStream_t st, const LGeo& q, int M, const long long* idx, const float* X, const float* T,
                      unsigned short* Xb, float* Tb) {
-  hipLaunchKernelGGL(k_gather_rows, dim3(M), dim3(256), 0, st, M, kMlpIn, kMlpOut, idx, X, T, Xb, Tb);
+  hipLaunchKernelGGL(k_gather_rows, dim3(M), dim3(256), 0, st, M, q.nin, q.nout, idx, X, T, Xb, Tb);
 }
 
 void mlp_to_bf16(hipStream_t st, long long n, const float* a, unsigned short* b) {
@@ -323,142 +327,190 @@ void mlp_adam(hipStream_t st, long long n, float* p, const float* g, float* m1, 
 }
 
 // ---------------------------------------------------------------- patches
-// One thread per cluster.  The slots are anchored at the mean (xm, ym) of the
-// cluster's 16 particles, the centre of its 4x4 block of lattice sites:
-// sx = floor((x - xm)/hl + 4), so lattice sites sit mid-slot whatever the
-// offset between the cell grid and the lattice (the 1e-6 added to each cell
-// side accumulates across the grid).  Writes the 64 patch and 16 core slots
-// and the anchor, or pok[c] = 0 when the cluster is not eligible.
-__global__ void k_patch_build(Grid g, double hl, const int* __restrict__ sta, const int* __restrict__ end,
-                              const double* __restrict__ xs, const double* __restrict__ ys, int* __restrict__ pidx,
-                              int* __restrict__ cidx, double* __restrict__ pctr, unsigned char* __restrict__ pok) {
+// One thread per cluster.  The slots are anchored at the mean of the
+// cluster's nc particles, the centre of its block of cs^dim lattice sites:
+// slot coordinate floor((x - xm)/hl + ps/2) per axis, so lattice sites sit
+// mid-slot whatever the offset between the cell grid and the lattice (the
+// 1e-6 added to each cell side accumulates across the grid).  Slot s =
+// sx ps + sy (2D), (sx ps + sy) ps + sz (3D); core slots are those with every
+// coordinate in [ring, ring + cs), ring = (ps - cs)/2.  Writes the np patch
+// and nc core slots and the anchor, or pok[c] = 0 when the cluster is not
+// eligible (a slot empty or doubly filled, or the core slots not exactly the
+// cluster's own particles).
+template <int DIM>
+__global__ void k_patch_build(Grid g, LGeo q, double hl, const int* __restrict__ sta, const int* __restrict__ end,
+                              const double* __restrict__ xs, const double* __restrict__ ys,
+                              const double* __restrict__ zs, int* __restrict__ pidx, int* __restrict__ cidx,
+                              double* __restrict__ pctr, unsigned char* __restrict__ pok) {
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.ncell) return;
-  int* ps = pidx + c * kPatch;
-  int* cs = cidx + c * kMlpOut;
-  for (int s = 0; s < kPatch; ++s) ps[s] = -1;
-  for (int s = 0; s < kMlpOut; ++s) cs[s] = -1;
-  bool ok = end[c] - sta[c] + 1 == kMlpOut;
-  double xm = 0.0, ym = 0.0;
+  const int ps = q.ps, cs = q.cs, ring = (q.ps - q.cs) / 2;
+  int* P = pidx + c * q.np;
+  int* C = cidx + c * q.nc;
+  for (int s = 0; s < q.np; ++s) P[s] = -1;
+  for (int s = 0; s < q.nc; ++s) C[s] = -1;
+  bool ok = end[c] - sta[c] + 1 == q.nc;
+  double xm = 0.0, ym = 0.0, zm = 0.0;
   if (ok) {
     for (int j = sta[c]; j <= end[c]; ++j) {
       xm += xs[j];
       ym += ys[j];
+      if (DIM == 3) zm += zs[j];
     }
-    xm /= kMlpOut;
-    ym /= kMlpOut;
+    xm /= q.nc;
+    ym /= q.nc;
+    zm /= q.nc;
   }
-  pctr[2 * c] = xm;
-  pctr[2 * c + 1] = ym;
+  pctr[3 * c] = xm;
+  pctr[3 * c + 1] = ym;
+  pctr[3 * c + 2] = zm;
+  const double half = 0.5 * ps;
   int ix, iy, iz;
   cell_coords(g, c, &ix, &iy, &iz);
-  for (int jx = ix - 1; jx <= ix + 1 && ok; ++jx)
-    for (int jy = iy - 1; jy <= iy + 1 && ok; ++jy) {
-      if (jx < 0 || jy < 0 || jx >= g.nx || jy >= g.ny) continue;
-      const long long jc = cell_index(g, jx, jy, 0);
-      for (int j = sta[jc]; j <= end[jc]; ++j) {
-        const double fx = floor((xs[j] - xm) / hl + 4.0), fy = floor((ys[j] - ym) / hl + 4.0);
-        if (!(fx >= 0.0 && fx < 8.0 && fy >= 0.0 && fy < 8.0)) continue;
-        const int sx = (int)fx, sy = (int)fy, s = sx * 8 + sy;
-        if (ps[s] >= 0) ok = false;
-        ps[s] = j;
-        const bool incore = sx >= 2 && sx < 6 && sy >= 2 && sy < 6;
-        const bool mine = j >= sta[c] && j <= end[c];
-        if (incore != mine) ok = false;
-        if (incore) cs[(sx - 2) * 4 + (sy - 2)] = j;
+  for (int jz = DIM == 3 ? iz - 1 : 0; jz <= (DIM == 3 ? iz + 1 : 0) && ok; ++jz)
+    for (int jx = ix - 1; jx <= ix + 1 && ok; ++jx)
+      for (int jy = iy - 1; jy <= iy + 1 && ok; ++jy) {
+        if (jx < 0 || jy < 0 || jx >= g.nx || jy >= g.ny) continue;
+        if (DIM == 3 && (jz < 0 || jz >= g.nz)) continue;
+        const long long jc = cell_index(g, jx, jy, jz);
+        for (int j = sta[jc]; j <= end[jc]; ++j) {
+          const double fx = floor((xs[j] - xm) / hl + half), fy = floor((ys[j] - ym) / hl + half);
+          const double fz = DIM == 3 ? floor((zs[j] - zm) / hl + half) : 0.0;
+          if (!(fx >= 0.0 && fx < ps && fy >= 0.0 && fy < ps && fz >= 0.0 && fz < (DIM == 3 ? ps : 1))) continue;
+          const int sx = (int)fx, sy = (int)fy, sz = (int)fz;
+          const int s = DIM == 3 ? (sx * ps + sy) * ps + sz : sx * ps + sy;
+          if (P[s] >= 0) ok = false;
+          P[s] = j;
+          const bool incore = sx >= ring && sx < ring + cs && sy >= ring && sy < ring + cs &&
+                              (DIM == 2 || (sz >= ring && sz < ring + cs));
+          const bool mine = j >= sta[c] && j <= end[c];
+          if (incore != mine) ok = false;
+          if (incore) {
+            const int o = DIM == 3 ? ((sx - ring) * cs + (sy - ring)) * cs + (sz - ring) : (sx - ring) * cs + (sy - ring);
+            C[o] = j;
+          }
+        }
       }
-    }
-  for (int s = 0; s < kPatch && ok; ++s) ok = ps[s] >= 0;
+  for (int s = 0; s < q.np && ok; ++s) ok = P[s] >= 0;
   pok[c] = ok ? 1 : 0;
 }
 
-void patch_build(hipStream_t st, const Grid& g, double hl, const int* sta, const int* end, const double* xs,
-                 const double* ys, int* pidx, int* cidx, double* pctr, unsigned char* pok) {
-  hipLaunchKernelGGL(k_patch_build, g256(g.ncell), dim3(256), 0, st, g, hl, sta, end, xs, ys, pidx, cidx, pctr, pok);
+void patch_build(hipStream_t st, const Grid& g, const LGeo& q, double hl, const int* sta, const int* end,
+                 const double* xs, const double* ys, const double* zs, int* pidx, int* cidx, double* pctr,
+                 unsigned char* pok) {
+  if (q.dim == 3)
+    hipLaunchKernelGGL(k_patch_build<3>, g256(g.ncell), dim3(256), 0, st, g, q, hl, sta, end, xs, ys, zs, pidx, cidx,
+                       pctr, pok);
+  else
+    hipLaunchKernelGGL(k_patch_build<2>, g256(g.ncell), dim3(256), 0, st, g, q, hl, sta, end, xs, ys, zs, pidx, cidx,
+                       pctr, pok);
 }
 
 // Features of the listed clusters for the vector r: one wavefront per
-// cluster, lane s handles slot s.  X row: 64 scaled residuals, 64 x-offsets,
-// 64 y-offsets (units of sigma); s_out[k] = RMS of the patch residual.  With
-// z != nullptr also the targets T[k][o] = z[core o] / (s * out_scale).
-__global__ __launch_bounds__(64) void k_features(long long nk, const int* __restrict__ list, double hl,
+// cluster, lane l handling slots l, l + 64, ...  X row: np scaled residuals,
+// np x-offsets, np y-offsets (3D: np z-offsets) in units of sigma, zeros up
+// to nin; s_out[k] = RMS of the patch residual.  With z != nullptr also the
+// targets T[k][o] = z[core o] / (s * out_scale), zeros for o >= nc.
+template <int DIM>
+__global__ __launch_bounds__(64) void k_features(LGeo q, long long nk, const int* __restrict__ list, double hl,
                                                  double inv_sigma, const int* __restrict__ pidx,
                                                  const int* __restrict__ cidx, const double* __restrict__ pctr,
-                                                 const double* __restrict__ xs,
-                                                 const double* __restrict__ ys, const double* __restrict__ r,
+                                                 const double* __restrict__ xs, const double* __restrict__ ys,
+                                                 const double* __restrict__ zs, const double* __restrict__ r,
                                                  const double* __restrict__ z, double out_scale, float* __restrict__ X,
                                                  float* __restrict__ T, double* __restrict__ s_out) {
   const long long k = blockIdx.x;
   if (k >= nk) return;
   const int lane = threadIdx.x;
   const long long c = list[k];
-  const double xc = pctr[2 * c], yc = pctr[2 * c + 1];
-  const int j = pidx[c * kPatch + lane];
-  const double rv = r[j];
-  double ss = rv * rv;
-  for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off);
-  const double s = sqrt(ss / kPatch);
-  const double inv = s > 0.0 ? 1.0 / s : 0.0;
-  const int sx = lane / 8, sy = lane % 8;
-  const double cx = xc + (sx - 4 + 0.5) * hl, cy = yc + (sy - 4 + 0.5) * hl;
-  float* Xk = X + k * kMlpIn;
-  Xk[lane] = (float)(rv * inv);
-  Xk[kPatch + lane] = (float)((xs[j] - cx) * inv_sigma);
-  Xk[2 * kPatch + lane] = (float)((ys[j] - cy) * inv_sigma);
-  if (lane == 0) s_out[k] = s;
-  if (z && lane < kMlpOut) {
-    const int jc = cidx[c * kMlpOut + lane];
-    T[k * kMlpOut + lane] = (float)(z[jc] * inv / out_scale);
+  const double xc = pctr[3 * c], yc = pctr[3 * c + 1], zc = pctr[3 * c + 2];
+  const int* P = pidx + c * q.np;
+  double ss = 0.0;
+  for (int s = lane; s < q.np; s += 64) {
+    const double rv = r[P[s]];
+    ss = fma(rv, rv, ss);
   }
+  for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off);
+  const double sc = sqrt(ss / q.np);
+  const double inv = sc > 0.0 ? 1.0 / sc : 0.0;
+  const double half = 0.5 * q.ps;
+  float* Xk = X + k * q.nin;
+  for (int s = lane; s < q.np; s += 64) {
+    const int j = P[s];
+    const int sx = DIM == 3 ? s / (q.ps * q.ps) : s / q.ps;
+    const int sy = DIM == 3 ? (s / q.ps) % q.ps : s % q.ps;
+    const int sz = DIM == 3 ? s % q.ps : 0;
+    Xk[s] = (float)(r[j] * inv);
+    Xk[q.np + s] = (float)((xs[j] - (xc + (sx - half + 0.5) * hl)) * inv_sigma);
+    Xk[2 * q.np + s] = (float)((ys[j] - (yc + (sy - half + 0.5) * hl)) * inv_sigma);
+    if (DIM == 3) Xk[3 * q.np + s] = (float)((zs[j] - (zc + (sz - half + 0.5) * hl)) * inv_sigma);
+  }
+  for (int i = (DIM + 1) * q.np + lane; i < q.nin; i += 64) Xk[i] = 0.0f;
+  if (lane == 0) s_out[k] = sc;
+  if (z)
+    for (int o = lane; o < q.nout; o += 64)
+      T[k * q.nout + o] = o < q.nc ? (float)(z[cidx[c * q.nc + o]] * inv / out_scale) : 0.0f;
 }
 
-void patch_features(hipStream_t st, long long nk, const int* list, double hl, double sigma, const int* pidx,
-                    const int* cidx, const double* pctr, const double* xs, const double* ys, const double* r,
-                    const double* z, double out_scale, float* X, float* T, double* s_out) {
+void patch_features(hipStream_t st, const LGeo& q, long long nk, const int* list, double hl, double sigma,
+                    const int* pidx, const int* cidx, const double* pctr, const double* xs, const double* ys,
+                    const double* zs, const double* r, const double* z, double out_scale, float* X, float* T,
+                    double* s_out) {
   if (nk <= 0) return;
-  hipLaunchKernelGGL(k_features, dim3((unsigned)nk), dim3(64), 0, st, nk, list, hl, 1.0 / sigma, pidx, cidx, pctr,
-                     xs, ys, r, z, out_scale, X, T, s_out);
+  if (q.dim == 3)
+    hipLaunchKernelGGL(k_features<3>, dim3((unsigned)nk), dim3(64), 0, st, q, nk, list, hl, 1.0 / sigma, pidx, cidx,
+                       pctr, xs, ys, zs, r, z, out_scale, X, T, s_out);
+  else
+    hipLaunchKernelGGL(k_features<2>, dim3((unsigned)nk), dim3(64), 0, st, q, nk, list, hl, 1.0 / sigma, pidx, cidx,
+                       pctr, xs, ys, zs, r, z, out_scale, X, T, s_out);
 }
 
-// z[core o of cluster list[k]] = Y[k][o] * s[k] * out_scale
-__global__ void k_core_scatter(long long nk, const int* __restrict__ list, const int* __restrict__ cidx,
+// z[core o of cluster list[k]] = Y[k][o] * s[k] * out_scale, o < nc
+__global__ void k_core_scatter(LGeo q, long long nk, const int* __restrict__ list, const int* __restrict__ cidx,
                                const float* __restrict__ Y, const double* __restrict__ s, double out_scale,
                                double* __restrict__ z) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nk * kMlpOut) return;
-  const long long k = i / kMlpOut;
-  const int o = (int)(i % kMlpOut);
-  z[cidx[(long long)list[k] * kMlpOut + o]] = (double)Y[i] * s[k] * out_scale;
+  if (i >= nk * q.nc) return;
+  const long long k = i / q.nc;
+  const int o = (int)(i % q.nc);
+  z[cidx[(long long)list[k] * q.nc + o]] = (double)Y[k * q.nout + o] * s[k] * out_scale;
 }
 
-void core_scatter(hipStream_t st, long long nk, const int* list, const int* cidx, const float* Y, const double* s,
-                  double out_scale, double* z) {
+void core_scatter(hipStream_t st, const LGeo& q, long long nk, const int* list, const int* cidx, const float* Y,
+                  const double* s, double out_scale, double* z) {
   if (nk <= 0) return;
-  hipLaunchKernelGGL(k_core_scatter, g256(nk * kMlpOut), dim3(256), 0, st, nk, list, cidx, Y, s, out_scale, z);
+  hipLaunchKernelGGL(k_core_scatter, g256(nk * q.nc), dim3(256), 0, st, q, nk, list, cidx, Y, s, out_scale, z);
 }
 
 // Local residual reduction of a core correction on its patch:
 // rho[k] = |r_patch - A(patch, core) z_core| / |r_patch| (A the Gaussian
 // block, 1/(2 pi sigma^2) normalisation), one wavefront per cluster.
-__global__ __launch_bounds__(64) void k_patch_rho(long long nk, const int* __restrict__ list,
+template <int DIM>
+__global__ __launch_bounds__(64) void k_patch_rho(LGeo q, long long nk, const int* __restrict__ list,
                                                   const int* __restrict__ pidx, const int* __restrict__ cidx,
                                                   const double* __restrict__ xs, const double* __restrict__ ys,
-                                                  const double* __restrict__ r, const double* __restrict__ z,
-                                                  double inv_two_s2, double norm, double* __restrict__ rho) {
+                                                  const double* __restrict__ zs, const double* __restrict__ r,
+                                                  const double* __restrict__ z, double inv_two_s2, double norm,
+                                                  double* __restrict__ rho) {
   const long long k = blockIdx.x;
   if (k >= nk) return;
   const int lane = threadIdx.x;
   const long long c = list[k];
-  const int j = pidx[c * kPatch + lane];
-  const double xj = xs[j], yj = ys[j];
-  double az = 0.0;
-  for (int o = 0; o < kMlpOut; ++o) {
-    const int jc = cidx[c * kMlpOut + o];
-    const double dx = xj - xs[jc], dy = yj - ys[jc];
-    az = fma(exp_nonpos(-(dx * dx + dy * dy) * inv_two_s2) * norm, z[jc], az);
+  const int* C = cidx + c * q.nc;
+  double num = 0.0, den = 0.0;
+  for (int s = lane; s < q.np; s += 64) {
+    const int j = pidx[c * q.np + s];
+    const double xj = xs[j], yj = ys[j], zj = DIM == 3 ? zs[j] : 0.0;
+    double az = 0.0;
+    for (int o = 0; o < q.nc; ++o) {
+      const int jc = C[o];
+      const double dx = xj - xs[jc], dy = yj - ys[jc], dz = DIM == 3 ? zj - zs[jc] : 0.0;
+      az = fma(exp_nonpos(-(dx * dx + dy * dy + dz * dz) * inv_two_s2) * norm, z[jc], az);
+    }
+    const double rv = r[j];
+    num = fma(rv - az, rv - az, num);
+    den = fma(rv, rv, den);
   }
-  const double rv = r[j];
-  double num = (rv - az) * (rv - az), den = rv * rv;
   for (int off = 32; off > 0; off >>= 1) {
     num += __shfl_xor(num, off);
     den += __shfl_xor(den, off);
@@ -466,12 +518,17 @@ __global__ __launch_bounds__(64) void k_patch_rho(long long nk, const int* __res
   if (lane == 0) rho[k] = den > 0.0 ? sqrt(num / den) : 0.0;
 }
 
-void patch_rho(hipStream_t st, long long nk, const int* list, const int* pidx, const int* cidx, const double* xs,
-               const double* ys, const double* r, const double* z, double sigma, double* rho) {
+void patch_rho(hipStream_t st, const LGeo& q, long long nk, const int* list, const int* pidx, const int* cidx,
+               const double* xs, const double* ys, const double* zs, const double* r, const double* z, double sigma,
+               double* rho) {
   if (nk <= 0) return;
   Kern kk = make_kern(sigma);
-  hipLaunchKernelGGL(k_patch_rho, dim3((unsigned)nk), dim3(64), 0, st, nk, list, pidx, cidx, xs, ys, r, z,
-                     kk.inv_two_s2, kk.norm, rho);
+  if (q.dim == 3)
+    hipLaunchKernelGGL(k_patch_rho<3>, dim3((unsigned)nk), dim3(64), 0, st, q, nk, list, pidx, cidx, xs, ys, zs, r, z,
+                       kk.inv_two_s2, kk.norm, rho);
+  else
+    hipLaunchKernelGGL(k_patch_rho<2>, dim3((unsigned)nk), dim3(64), 0, st, q, nk, list, pidx, cidx, xs, ys, zs, r, z,
+                       kk.inv_two_s2, kk.norm, rho);
 }
 
 }  // namespace prbf
@@ -504,20 +561,21 @@ void copy_f32(hipStream_t st, const float* a, float* b) { hipLaunchKernelGGL(k_c
 
 namespace prbf {
 
-// out[k] = sum over the 16 core particles of cluster list[k] of r^2
-__global__ void k_core_norms(long long nk, const int* __restrict__ list, const int* __restrict__ cidx,
+// out[k] = sum over the nc core particles of cluster list[k] of r^2
+__global__ void k_core_norms(LGeo q, long long nk, const int* __restrict__ list, const int* __restrict__ cidx,
                              const double* __restrict__ r, double* __restrict__ out) {
   const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nk) return;
-  const int* cs = cidx + (long long)list[k] * kMlpOut;
+  const int* cs = cidx + (long long)list[k] * q.nc;
   double s = 0.0;
-  for (int o = 0; o < kMlpOut; ++o) s = fma(r[cs[o]], r[cs[o]], s);
+  for (int o = 0; o < q.nc; ++o) s = fma(r[cs[o]], r[cs[o]], s);
   out[k] = s;
 }
 
-void core_norms(hipStream_t st, long long nk, const int* list, const int* cidx, const double* r, double* out) {
+void core_norms(hipStream_t st, const LGeo& q, long long nk, const int* list, const int* cidx, const double* r,
+                double* out) {
   if (nk <= 0) return;
-  hipLaunchKernelGGL(k_core_norms, g256(nk), dim3(256), 0, st, nk, list, cidx, r, out);
+  hipLaunchKernelGGL(k_core_norms, g256(nk), dim3(256), 0, st, q, nk, list, cidx, r, out);
 }
 
 }  // namespace prbf

@@ -1,11 +1,12 @@
 """The learned preconditioner: the network handle, pair harvesting and
 training, all on the device through the C ABI (include/petrbf.h).
 
-The network maps the residual on the 8x8 lattice-slot patch around a cluster
-of 4x4 lattice spacings (scaled by its RMS) plus the particles' offsets from
-their slots to a correction on the cluster's 16 core particles; it is trained
-on (patch, exact RASM core solve) pairs taken from FGMRES runs with the exact
-FP64 block preconditioner.  torch provides the device memory only.
+The network maps the residual on the lattice-slot patch around a cluster
+(2D: 8x8 slots around 4x4 lattice spacings; 3D: 5x5x5 slots around 3x3x3),
+scaled by its RMS, plus the particles' offsets from their slots to a
+correction on the cluster's core particles; it is trained on (patch, exact
+RASM core solve) pairs taken from FGMRES runs with the exact FP64 block
+preconditioner.  torch provides the device memory only.
 """
 
 from __future__ import annotations
@@ -17,19 +18,24 @@ import numpy as np
 from . import _lib as L
 from .rbf import Context, _p
 
-N_IN, N_OUT = 192, 16
+# (inputs, outputs, core outputs used) per dimension
+DIMS = {2: (192, 16, 16), 3: (512, 32, 27)}
+N_IN, N_OUT = DIMS[2][:2]
 
 
 class Mlp:
-    """192 -> 256 -> 256 -> 256 -> 16 GELU network (BF16 MFMA, FP32 master weights)."""
+    """nin -> 256 -> 256 -> 256 -> nout GELU network (BF16 MFMA, FP32 master
+    weights): 2D 192 -> 16, 3D 512 -> 32 (27 core corrections used)."""
 
-    def __init__(self, seed=0x5EED):
+    def __init__(self, seed=0x5EED, dim=2):
         self._L = L.load()
         h = C.c_void_p()
-        st = self._L.prbf_mlp_create(C.byref(h), C.c_uint64(seed))
+        st = self._L.prbf_mlp_create_dim(C.byref(h), C.c_uint64(seed), int(dim))
         if st != 0:
-            raise L.PrbfError(st, "prbf_mlp_create failed")
+            raise L.PrbfError(st, "prbf_mlp_create_dim failed")
         self._h = h
+        self.dim = int(dim)
+        self.n_in, self.n_out, self.n_core = DIMS[self.dim]
 
     def close(self):
         if getattr(self, "_h", None):
@@ -46,7 +52,7 @@ class Mlp:
 
     @property
     def num_params(self):
-        return int(self._L.prbf_mlp_num_params())
+        return int(self._L.prbf_mlp_num_params_dim(self.dim))
 
     def params(self):
         P = np.zeros(self.num_params, np.float32)
@@ -78,16 +84,18 @@ def harvest(ctx: Context, w_dev, n_vectors, torch, restart=30):
     basis vectors of an FGMRES solve of A g = w (w in the caller's order, a
     device tensor) with the exact FP64 block preconditioner: X the patch
     features of each basis vector, T the exact core solves scaled by the patch
-    RMS and a common output scale."""
+    RMS and a common output scale.  The network's dimension is the context's
+    (ctx.dim)."""
     dev = w_dev.device
     n = ctx.ni
+    n_in, n_out, _ = DIMS[ctx.dim]
     ne = ctx.learned_eligible()
     gamma = torch.empty_like(w_dev)
     torch.cuda.synchronize()
     info = ctx.solve_dev(w_dev, gamma, ksp="fgmres", pc="rasm", rtol=1e-10, restart=restart, max_it=restart)
     nv = min(n_vectors, info.its)
-    X = torch.empty((nv * ne, N_IN), dtype=torch.float32, device=dev)
-    T = torch.empty((nv * ne, N_OUT), dtype=torch.float32, device=dev)
+    X = torch.empty((nv * ne, n_in), dtype=torch.float32, device=dev)
+    T = torch.empty((nv * ne, n_out), dtype=torch.float32, device=dev)
     s = torch.empty(ne, dtype=torch.float64, device=dev)
     v = torch.empty(n, dtype=torch.float64, device=dev)
     z = torch.empty(n, dtype=torch.float64, device=dev)

@@ -16,20 +16,21 @@ def _announce(msg):
     sys.stdout.flush()
 
 
-def _params(seed):
+def _params(seed, dim=2):
     rng = np.random.default_rng(seed)
     P = []
-    for k, n in MO.DIMS:
+    for k, n in MO.LAYERS[dim]:
         a = np.sqrt(6.0 / (k + n))
         P.append(rng.uniform(-a, a, k * n))
         P.append(rng.uniform(-0.1, 0.1, n))
     return np.concatenate(P).astype(np.float32).astype(np.float64)
 
 
-def _batch(seed, m):
+def _batch(seed, m, dim=2):
     rng = np.random.default_rng(seed)
-    X = rng.standard_normal((m, 192)).astype(np.float32).astype(np.float64)
-    T = rng.standard_normal((m, 16)).astype(np.float32).astype(np.float64)
+    nin, nout = MO.LAYERS[dim][0][0], MO.LAYERS[dim][-1][1]
+    X = rng.standard_normal((m, nin)).astype(np.float32).astype(np.float64)
+    T = rng.standard_normal((m, nout)).astype(np.float32).astype(np.float64)
     return X, T
 
 
@@ -43,15 +44,16 @@ def test_bf16_rounding_cases():
     assert np.array_equal(MO.bf16(vals), want)
 
 
-def test_oracle_gradient_matches_torch_autograd():
+@pytest.mark.parametrize("dim", [2, 3])
+def test_oracle_gradient_matches_torch_autograd(dim):
     torch = pytest.importorskip("torch")
-    P = _params(1)
-    X, T = _batch(2, 32)
+    P = _params(1, dim)
+    X, T = _batch(2, 32, dim)
     loss, G = MO.loss_grad(P, X, T)
     Pt = torch.tensor(P, dtype=torch.float64, requires_grad=True)
     h = torch.tensor(X)
     o = 0
-    for l, (k, n) in enumerate(MO.DIMS):
+    for l, (k, n) in enumerate(MO.LAYERS[dim]):
         W = Pt[o:o + k * n].view(k, n)
         o += k * n
         b = Pt[o:o + n]
@@ -66,12 +68,13 @@ def test_oracle_gradient_matches_torch_autograd():
     assert np.linalg.norm(G - g) <= 1e-11 * np.linalg.norm(g)
 
 
-def test_oracle_gradient_central_differences():
-    P = _params(3)
-    X, T = _batch(4, 16)
+@pytest.mark.parametrize("dim", [2, 3])
+def test_oracle_gradient_central_differences(dim):
+    P = _params(3, dim)
+    X, T = _batch(4, 16, dim)
     _, G = MO.loss_grad(P, X, T)
     rng = np.random.default_rng(5)
-    idx = np.concatenate([rng.integers(0, MO.num_params(), 24), [MO.num_params() - 1]])
+    idx = np.concatenate([rng.integers(0, MO.num_params(dim), 24), [MO.num_params(dim) - 1]])
     for i in idx:
         h = 1e-5
         Pp, Pm = P.copy(), P.copy()
@@ -92,36 +95,44 @@ def test_oracle_bf16_mode_stays_close_to_wide_mode():
 
 # ---------------------------------------------------------------- GPU: kernels vs oracle
 @pytest.mark.gpu
-def test_mlp_forward_and_gradient_match_oracle():
+@pytest.mark.parametrize("dim", [2, 3])
+def test_mlp_forward_and_gradient_match_oracle(dim):
     torch = pytest.importorskip("torch")
     from petrbf_amd.learned import Mlp
-    _announce("mlp forward/backward vs oracle")
+    _announce(f"mlp forward/backward vs oracle, {dim}D network")
     dev = torch.device("cuda:0")
-    net = Mlp(seed=11)
-    P = _params(8)
+    net = Mlp(seed=11, dim=dim)
+    assert net.num_params == MO.num_params(dim)
+    P = _params(8, dim)
     net.set_params(P.astype(np.float32), 1.0)
     Pg, s = net.params()
     assert np.array_equal(Pg, P.astype(np.float32)) and s == 1.0
     for m in (64, 208):
-        X, T = _batch(9 + m, m)
+        X, T = _batch(9 + m, m, dim)
         Xd = torch.tensor(X, dtype=torch.float32, device=dev)
         Td = torch.tensor(T, dtype=torch.float32, device=dev)
-        Yd = torch.empty((m, 16), dtype=torch.float32, device=dev)
+        Yd = torch.empty((m, net.n_out), dtype=torch.float32, device=dev)
         torch.cuda.synchronize()
         net.forward_dev(Xd, Yd)
         Y = Yd.cpu().numpy().astype(np.float64)
-        Yo = MO.forward(P, X, bf=True)[0]
-        assert np.linalg.norm(Y - Yo) <= 1e-4 * np.linalg.norm(Yo)
-        Gd = torch.empty(MO.num_params(), dtype=torch.float32, device=dev)
+        # the oracle rounds to BF16 where the kernels store BF16; the kernels
+        # accumulate in FP32, the oracle in FP64, so an activation next to a
+        # BF16 rounding boundary may round the other way: the bound is 5% of
+        # what BF16 rounding itself changes (the wide oracle's distance)
+        Yo, Yw = MO.forward(P, X, bf=True)[0], MO.forward(P, X)[0]
+        ey, by = np.linalg.norm(Y - Yo), max(1e-4 * np.linalg.norm(Yo), 0.05 * np.linalg.norm(Yo - Yw))
+        assert ey <= by, (m, ey, by)
+        Gd = torch.empty(MO.num_params(dim), dtype=torch.float32, device=dev)
         loss = net.loss_grad_dev(Xd, Td, Gd)
-        lo, Go = MO.loss_grad(P, X, T, bf=True)
-        assert abs(loss - lo) <= 1e-4 * lo
+        (lo, Go), (lw, Gw) = MO.loss_grad(P, X, T, bf=True), MO.loss_grad(P, X, T)
+        assert abs(loss - lo) <= max(1e-4 * lo, 0.05 * abs(lo - lw)), (m, loss, lo, lw)
         G = Gd.cpu().numpy().astype(np.float64)
         o = 0
-        for k, n in MO.DIMS:
+        for k, n in MO.LAYERS[dim]:
             for size in (k * n, n):
-                a, b = G[o:o + size], Go[o:o + size]
-                assert np.linalg.norm(a - b) <= 2e-3 * np.linalg.norm(b) + 1e-12, (m, o, size)
+                a, b, c = G[o:o + size], Go[o:o + size], Gw[o:o + size]
+                bound = max(2e-3 * np.linalg.norm(b), 0.05 * np.linalg.norm(b - c)) + 1e-12
+                assert np.linalg.norm(a - b) <= bound, (m, o, size, np.linalg.norm(a - b), bound)
                 o += size
     net.close()
 
@@ -200,5 +211,149 @@ def test_learned_preconditioner_trains_and_solve_reaches_fp64_residual():
     assert out["trained"][0] <= out["exact"][0] + 2 * 30
     net.close()
     net0.close()
+    ctx.close()
+    ct.close()
+
+
+# ---------------------------------------------------------------- 3D network
+def _lattice_slots_3d(ctx, n):
+    """For the 3D jittered lattice (particle i at lattice indices
+    ((i % n^2) // n, i % n, i // n^2), problems.jittered_lattice_3d): the
+    clusters whose cell holds a 3x3x3 block of lattice sites with a whole
+    ring of sites one spacing around it, in increasing cell order, each with
+    the sorted index of the site in patch slot (a, b, c), a, b, c in 0..4."""
+    pi, _ = ctx.permutation()
+    inv = np.empty_like(pi)
+    inv[pi] = np.arange(pi.size, dtype=pi.dtype)
+    ista, iend, _, _ = ctx.cells()
+    out = []
+    for c in range(ista.size):
+        if iend[c] - ista[c] + 1 != 27:
+            continue
+        o = pi[ista[c]:iend[c] + 1]
+        ix, iy, iz = (o % (n * n)) // n, o % n, o // (n * n)
+        i0, j0, k0 = ix.min(), iy.min(), iz.min()
+        if ix.max() - i0 != 2 or iy.max() - j0 != 2 or iz.max() - k0 != 2:
+            continue
+        if min(i0, j0, k0) < 1 or max(i0, j0, k0) + 3 >= n:
+            continue
+        slots = np.empty((5, 5, 5), np.int64)
+        for a in range(5):
+            for b in range(5):
+                for cc in range(5):
+                    slots[a, b, cc] = inv[(k0 - 1 + cc) * n * n + (i0 - 1 + a) * n + (j0 - 1 + b)]
+        out.append((c, slots))
+    return out
+
+
+@pytest.mark.gpu
+def test_learned_3d_patches_match_lattice_slots():
+    """3D patches: the eligible clusters are exactly those whose cell is a
+    3x3x3 block of lattice sites with the whole one-spacing ring around it,
+    and each feature row holds the residuals of the 5x5x5 sites in lattice
+    order scaled by their RMS, the sites' offsets from their slot centres in
+    units of sigma and zeros up to 512; the targets are the core values over
+    the RMS, zeros past the 27 core outputs (slots fixed by the lattice
+    indices, independently of the kernel's floor arithmetic)."""
+    torch = pytest.importorskip("torch")
+    from petrbf_amd import problems
+    from petrbf_amd.rbf import Context
+    _announce("3D learned patches vs lattice slots")
+    dev = torch.device("cuda:0")
+    n = 18
+    p = problems.lattice_3d_problem(n, overlap=1.0)
+    ctx = Context.from_problem(p)
+    ctx.set_points(p.x, p.y, p.z)
+    want = _lattice_slots_3d(ctx, n)
+    ne = ctx.learned_eligible()
+    # 64 interior cells; the jitter moves a few sites across cell faces (the
+    # grid starts at the smallest coordinate, the first plane's jitter), and
+    # those cells drop out on both sides (55 here)
+    assert ne == len(want) and ne >= 48, (ne, len(want))
+    rng = np.random.default_rng(31)
+    r = rng.standard_normal(p.n)
+    z = rng.standard_normal(p.n)
+    X = torch.empty((ne, 512), dtype=torch.float32, device=dev)
+    T = torch.empty((ne, 32), dtype=torch.float32, device=dev)
+    s = torch.empty(ne, dtype=torch.float64, device=dev)
+    got = ctx.learned_pairs_dev(torch.from_numpy(r).to(dev), X, s, z=torch.from_numpy(z).to(dev), T=T, out_scale=1.0)
+    ctx.synchronize()
+    assert got == ne
+    X, T, s = X.cpu().numpy().astype(np.float64), T.cpu().numpy().astype(np.float64), s.cpu().numpy()
+    xs, ys, zs = ctx.sorted_points()
+    hl = (p.nsigma_box * p.sigma + 1e-6) / 3
+    for k, (c, slots) in enumerate(want):
+        sl = slots.ravel()  # slot (a, b, c) at (a 5 + b) 5 + c
+        rms = np.sqrt(np.mean(r[sl] ** 2))
+        assert abs(s[k] - rms) <= 1e-12 * rms
+        np.testing.assert_allclose(X[k, :125], r[sl] / rms, rtol=1e-6, atol=1e-6)
+        core = slots[1:4, 1:4, 1:4].ravel()
+        cen = [v[core].mean() for v in (xs, ys, zs)]
+        a, b, cc = np.meshgrid(np.arange(5), np.arange(5), np.arange(5), indexing="ij")
+        for d, (v, idx) in enumerate(zip((xs, ys, zs), (a, b, cc))):
+            off = (v[sl] - (cen[d] + (idx.ravel() - 2.5 + 0.5) * hl)) / p.sigma
+            np.testing.assert_allclose(X[k, 125 * (d + 1):125 * (d + 2)], off, rtol=1e-5, atol=1e-6)
+        assert np.all(X[k, 500:] == 0)
+        np.testing.assert_allclose(T[k, :27], z[core] / rms, rtol=1e-6, atol=1e-6)
+        assert np.all(T[k, 27:] == 0)
+    ctx.close()
+
+
+@pytest.mark.gpu
+def test_learned_3d_batched_solve():
+    """The 3D network in the batched three-right-hand-side FGMRES: pairs
+    harvested on one seeded lattice (3x3x3 clusters, one-cell buffer, h/sigma
+    = 1 so that the exact method converges), the network trained on the
+    device, then on another seeded lattice: with no cluster accepted the
+    learned solve is the exact FP32 one bit for bit; with every eligible
+    cluster on the network, FGMRES's residual moves the clusters whose core
+    residual stalls back to exact blocks and every component reaches rtol
+    1e-10 with the FP64 true residual."""
+    torch = pytest.importorskip("torch")
+    from petrbf_amd import problems
+    from petrbf_amd.learned import Mlp, harvest
+    from petrbf_amd.rbf import Context
+    _announce("3D learned preconditioner: harvest, train, batched solve")
+    dev = torch.device("cuda:0")
+    pt = problems.lattice_3d_problem(24, seed=0x5EED0006, overlap=1.0)
+    ct = Context.from_problem(pt)
+    ct.set_points(pt.x, pt.y, pt.z)
+    X, T, osc = harvest(ct, torch.from_numpy(pt.w[0]).to(dev), 12, torch)
+    assert X.shape == (12 * ct.learned_eligible(), 512) and T.shape[1] == 32
+    assert float(torch.abs(T[:, 27:]).max()) == 0.0
+    net = Mlp(seed=9, dim=3)
+    net.set_params(net.params()[0], osc)
+    hist = net.train_dev(X, T, batch=256, steps=300, lr=1e-3, seed=4)
+    _announce(f"3D training on {X.shape[0]} pairs: loss {hist[:20].mean():.3f} -> {hist[-50:].mean():.3f}")
+    assert hist[-50:].mean() < hist[:20].mean()
+    P, _ = net.params()
+    assert np.all(P[-32 + 27:] == 0)  # the pad outputs' biases never move
+    p = problems.lattice_3d_problem(24, overlap=1.0)
+    ctx = Context.from_problem(p)
+    ctx.set_points(p.x, p.y, p.z)
+    w = torch.from_numpy(np.ascontiguousarray(p.w)).to(dev)
+    gam = torch.empty_like(w)
+    wv = torch.empty_like(w)
+    res = {}
+    for tag, tau in (("exact", None), ("none-accepted", 0.0), ("all", 1e30)):
+        if tau is not None:
+            n_net = ctx.learned_attach(net, tau)
+            assert (n_net == 0) if tau == 0.0 else (n_net == ctx.learned_eligible() > 0)
+        torch.cuda.synchronize()
+        infs = ctx.solve_multi_dev(w, gam, 3, ksp="fgmres", pc="rasm32" if tau is None else "learned", rtol=1e-10)
+        ctx.evaluate_dev(gam, wv, nrhs=3)
+        ctx.synchronize()
+        res[tag] = (gam.clone(), [(i.its, i.reason, i.n_network, i.n_net_fallback) for i in infs])
+        for r in range(3):
+            nw = float(torch.linalg.norm(w[r]))
+            if nw == 0.0:
+                assert infs[r].reason == 3
+                continue
+            true_rel = float(torch.linalg.norm(wv[r] - w[r])) / nw
+            assert infs[r].reason == 2 and true_rel <= 1.05e-10, (tag, r, infs[r].reason, true_rel)
+    _announce(f"3D batched solves (its, reason, network, moved): { {k: v[1] for k, v in res.items()} }")
+    assert torch.equal(res["exact"][0], res["none-accepted"][0])
+    assert [v[0] for v in res["exact"][1]] == [v[0] for v in res["none-accepted"][1]]
+    net.close()
     ctx.close()
     ct.close()
