@@ -225,6 +225,7 @@ def main():
     run = ["plain", "mixed"] if args.mode == "both" else [args.mode]
     results = {}
     chol = {}
+    gammas = {}
     for mode in run:
         ksp, pc = modes[mode]
 
@@ -248,6 +249,7 @@ def main():
         ctx.evaluate_dev(gamma, wv)
         torch.cuda.synchronize()
         true_rel = float(torch.linalg.norm(wv - w) / torch.linalg.norm(w))
+        gammas[mode] = gamma.clone()
         log(f"{mode}: {args.steps} steps {elapsed:.3f}s; its {inf.its} reason {inf.reason} true rel resid {true_rel:.3e}")
         ci = ctx.info()
         log(f"{mode}: block factorisation config waves={ci['chol_waves']} panel={ci['chol_panel']} "
@@ -260,6 +262,24 @@ def main():
                              t_setup_s=inf.t_setup_s, t_solve_s=inf.t_solve_s, t_matvec_s=inf.t_matvec_s,
                              n_matvec=inf.n_matvec, t_pc_s=inf.t_pc_s, n_pc=inf.n_pc,
                              n_regularised=inf.n_regularised, true_rel_residual=true_rel)
+
+    # ---- interpolation error against the analytic field on the h/2 grid
+    # (2d/main2.cxx:53,126-137: the (2 n_side - 1)^2 grid of half the lattice
+    # spacing, between the particles), for each mode's strengths
+    xt, yt, _ = problems.jittered_lattice_2d(2 * args.n_side - 1, seed=3, jitter=0.0)
+    exact = torch.from_numpy(problems.lamb_oseen(xt, yt)).to(dev)
+    ev = Context.from_problem(p)
+    ev.set_stream(stream.cuda_stream)
+    ev.set_points_dev(torch.from_numpy(xt).to(dev), torch.from_numpy(yt).to(dev), None, x, y)
+    wt = torch.empty_like(exact)
+    for mode, g in gammas.items():
+        ev.evaluate_dev(g, wt)
+        torch.cuda.synchronize()
+        results[mode]["analytic_field_error_h2"] = float(torch.linalg.norm(wt - exact) / torch.linalg.norm(exact))
+        log(f"{mode}: analytic-field error on the h/2 grid ({xt.size} points) "
+            f"{results[mode]['analytic_field_error_h2']:.6e}")
+    ev.close()
+    del xt, yt, exact, wt, gammas
 
     # ---- per-kernel timing with HIP events on the library's stream (FP64 RASM set up last if plain ran)
     ctx.set_points_dev(x, y)
