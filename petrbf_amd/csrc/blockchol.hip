@@ -911,9 +911,71 @@ __device__ void factor_block_ll(
       }
       return v;
     };
-    // diagonal block, one tile per wavefront: acc = (A_{k0+r,k0+q} - sum_j
-    // L_{k0+r,j} L_{k0+q,j}^T)^T
-    for (int u = wave; u < npc * (npc + 1) / 2; u += NW) {
+    // diagonal block: acc = (A_{k0+r,k0+q} - sum_j L_{k0+r,j} L_{k0+q,j}^T)^T.
+    // Panels of up to 2 columns (at most 3 tiles): wavefront 0 sums them all
+    // itself before factorising, while the others already stream their tile
+    // rows (no barrier between); wider panels: one tile per wavefront.
+    constexpr bool kW0D = NPC <= 2;
+    auto diag_w0 = [&]() {
+      const bool two = npc > 1;
+      v4 e00 = aprime(k0, 0, Cc[lr], Cc[16 * NPC + lr], Cc[32 * NPC + lr]), e10 = {0, 0, 0, 0}, e11 = {0, 0, 0, 0};
+      if (two) {
+        e10 = aprime(k0 + 1, 0, Cc[16 + lr], Cc[16 * NPC + 16 + lr], Cc[32 * NPC + 16 + lr]);
+        e11 = aprime(k0 + 1, 1, Cc[16 + lr], Cc[16 * NPC + 16 + lr], Cc[32 * NPC + 16 + lr]);
+      }
+      v4 o00 = {0, 0, 0, 0}, o10 = {0, 0, 0, 0}, o11 = {0, 0, 0, 0};
+      const Real* r0 = (PG ? wtile(k0, 0) : Rw) + lane * 4;
+      const Real* r1 = (PG ? wtile(k0 + 1, 0) : Rw + (size_t)k0 * kTE) + lane * 4;
+      // two columns per step into even and odd accumulators, as the
+      // one-tile-per-wavefront form sums them (the same bits)
+      int j = 0;
+      for (; j + 1 < k0; j += 2) {
+        const v4 x0 = *(const v4*)(r0 + j * kTE), y0 = *(const v4*)(r0 + (j + 1) * kTE);
+        v4 x1 = {0, 0, 0, 0}, y1 = {0, 0, 0, 0};
+        if (two) {
+          x1 = *(const v4*)(r1 + j * kTE);
+          y1 = *(const v4*)(r1 + (j + 1) * kTE);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          e00 = MF<Real>::mma(-x0[q], x0[q], e00);
+          o00 = MF<Real>::mma(-y0[q], y0[q], o00);
+        }
+        if (two) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            e10 = MF<Real>::mma(-x0[q], x1[q], e10);
+            o10 = MF<Real>::mma(-y0[q], y1[q], o10);
+            e11 = MF<Real>::mma(-x1[q], x1[q], e11);
+            o11 = MF<Real>::mma(-y1[q], y1[q], o11);
+          }
+        }
+      }
+      if (j < k0) {
+        const v4 x0 = *(const v4*)(r0 + j * kTE);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) e00 = MF<Real>::mma(-x0[q], x0[q], e00);
+        if (two) {
+          const v4 x1 = *(const v4*)(r1 + j * kTE);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            e10 = MF<Real>::mma(-x0[q], x1[q], e10);
+            e11 = MF<Real>::mma(-x1[q], x1[q], e11);
+          }
+        }
+      }
+      e00 += o00;
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) Dg[MF<Real>::row(rr, lq) * kLS + lr] = e00[rr];
+      if (two) {
+        e10 += o10;
+        e11 += o11;
+        *(v4*)(pd(1, 0) + lane * 4) = e10;
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) Dg[kLT + MF<Real>::row(rr, lq) * kLS + lr] = e11[rr];
+      }
+    };
+    for (int u = wave; !kW0D && u < npc * (npc + 1) / 2; u += NW) {
       int r, q2;
       tri_decode(u, &r, &q2);
       v4 acc = aprime(k0 + r, q2, Cc[16 * r + lr], Cc[16 * NPC + 16 * r + lr], Cc[32 * NPC + 16 * r + lr]);
@@ -1047,7 +1109,7 @@ __device__ void factor_block_ll(
     // the others stream the first (NW - 1) G tile rows below it (the streamed
     // updates do not need the diagonal inverses), then every wavefront
     // finishes its rows and takes the remaining ones NW G at a time
-    __syncthreads();
+    if constexpr (!kW0D) __syncthreads();
     constexpr int SO = NW - 1;
     const int ibo = k0 + npc + wave - 1;
     if (wave > 0 && ibo < KT) stream(ibo, SO);
@@ -1058,6 +1120,10 @@ __device__ void factor_block_ll(
       // the block's serial chain: raised issue priority against the other
       // workgroups' wavefronts on this SIMD while it runs
       __builtin_amdgcn_s_setprio(3);
+      if constexpr (kW0D) {
+        diag_w0();
+        wave_fence();
+      }
       for (int q2 = 0; q2 < npc; ++q2) {
         Real* D = Dg + q2 * kLT;
         wave_fence();
