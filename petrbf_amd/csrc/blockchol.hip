@@ -86,9 +86,18 @@ constexpr int kLS = 17;    // LDS row stride (bank-conflict padding)
 constexpr int kLT = 16 * kLS;
 
 // Sets of per-wavefront partial-sum tiles the backward substitution keeps
-// (FP32 two, one barrier per step; FP64 one, two barriers per step).
+// (FP32 two, one barrier per step; FP64 one, two barriers per step), and the
+// tiles per wavefront in a set (FP32 steps over two tile rows at once).
 template <typename T>
 __host__ __device__ constexpr int sbk_sets() { return sizeof(T) == 4 ? 2 : 1; }
+template <typename T>
+__host__ __device__ constexpr int sbk_tiles(int nw, bool pair) { return sizeof(T) == 4 ? 2 * (pair ? 2 : 1) * nw : nw; }
+// The FP32 left-looking kernels with 2-column panels run the backward
+// substitution two tile rows per step (4x2: 1.3% faster); 4-column panels
+// have no registers to spare for it, and at 1 column it would cost the
+// fourth workgroup per CU.
+template <typename T>
+__host__ __device__ constexpr bool sbk_pair(bool ll, int npc) { return sizeof(T) == 4 && ll && npc == 2; }
 
 // Lane-major tiles (the left-looking factorisation): lane l keeps its four
 // values X[l%16][row(q, l/16)], q = 0..3, contiguous at 4 l.  One vector
@@ -377,7 +386,7 @@ __host__ __device__ inline size_t blk_lds_bytes(int nmax, int tmax, int nw, int 
   const int m = npc < KT ? npc : KT;
   size_t pan = (size_t)pan_tiles(KT, m) * Pan<T>::kT * sizeof(T);
   size_t crd = (size_t)3 * NP * sizeof(double);
-  size_t red = ((size_t)NP * kLS + (size_t)sbk_sets<T>() * nw * kTE) * sizeof(T);  // phase-2 right-hand sides, partial sums
+  size_t red = ((size_t)NP * kLS + (size_t)sbk_tiles<T>(nw, false) * kTE) * sizeof(T);  // phase-2 right-hand sides, partial sums
   size_t region = pan > crd ? pan : crd;
   region = region > red ? region : red;
   region = (region + 15) & ~(size_t)15;
@@ -387,8 +396,8 @@ __host__ __device__ inline size_t blk_lds_bytes(int nmax, int tmax, int nw, int 
 // Phases 2 and 3 of one block whose factor L (tiles in Wb, L_II^{-1} in Wd)
 // is complete: the core rows of A'^{-1} into M.  LM: tiles stored lane-major
 // (lm_off) instead of register-major (MF::off).
-template <typename Real, int NW, bool LM>
-__device__ void core_rows(long long c, const Real* __restrict__ Wb, const Real* __restrict__ Wd,
+template <typename Real, int NW, bool LM, bool PAIR>
+__device__ __forceinline__ void core_rows(long long c, const Real* __restrict__ Wb, const Real* __restrict__ Wd,
                           unsigned char* __restrict__ lds_raw, int n, int T, const long long* __restrict__ mptr,
                           Real* __restrict__ M) {
   typedef typename MF<Real>::v4 v4;
@@ -400,11 +409,12 @@ __device__ void core_rows(long long c, const Real* __restrict__ Wb, const Real* 
   const int wave = sizeof(Real) == 4 ? __builtin_amdgcn_readfirstlane(tid / 64) : tid / 64;
   const int lr = lane & 15, lq = lane >> 4;
   // per-wave partial sums of the backward substitution, past X: FP32 keeps
-  // two sets used in alternate steps, so that a step needs one barrier (a
-  // wavefront writing step I-1's partial cannot overwrite what wavefront 0
-  // still reads of step I); FP64 keeps one set (LDS for two workgroups per CU)
-  // and a second barrier
+  // two sets used in alternate steps (one tile per wavefront, two in the
+  // two-row form), so that a step needs one barrier (a wavefront writing the
+  // next step's partials cannot overwrite what is still read of this one);
+  // FP64 keeps one set (LDS for two workgroups per CU) and a second barrier
   constexpr int kSB = sbk_sets<Real>();
+  static_assert(!PAIR || kSB == 2, "the two-row backward substitution is FP32's");
   Real* S = (Real*)lds_raw + (size_t)(16 * KT) * kLS;
   // ---- phase 2: X = L^{-T} L^{-1} E_core, 16 right-hand sides at a time,
   // X in LDS (row stride 17) in the region the panel used
@@ -450,91 +460,237 @@ __device__ void core_rows(long long c, const Real* __restrict__ Wb, const Real* 
       __builtin_amdgcn_s_setprio(0);
     }
     __syncthreads();
-    // backward: X_I = L_II^{-T} (Y_I - sum_{J>I} L_JI^T X_J); the sum is split
-    // over the wavefronts and combined by wavefront 0 through the scratch
-    // tiles.  In FP64 wavefront 0, which also combines the sums and applies
-    // the diagonal inverse, takes only J = I + 1 (whose X_J it wrote itself at
-    // the previous step) and the others share J >= I + 2 (FP64 4x2 setup 3.8%
-    // faster); FP32 deals J round-robin over all wavefronts (the split form
-    // spills there).  Each wavefront's first kPF tiles of column I-1 (and
-    // wavefront 0's L_{I-1,I-1}^{-1}) are loaded while step I runs, so the
-    // global loads leave the step's critical path.
-    constexpr bool kSplit = sizeof(Real) == 8;
-    constexpr int kPF = kSplit ? 6 : 4;
-    constexpr int kStride = kSplit ? NW - 1 : NW;
-    auto jof = [&](int I, int g) {
-      if constexpr (kSplit) return wave == 0 ? (g == 0 ? I + 1 : KT) : I + 1 + wave + g * (NW - 1);
-      else return I + 1 + wave + g * NW;
-    };
-    Real nx[kPF][4], nli[4];
-    auto load_col = [&](int I) {
+    if constexpr (!PAIR) {
+      // backward: X_I = L_II^{-T} (Y_I - sum_{J>I} L_JI^T X_J); the sum is split
+      // over the wavefronts and combined by wavefront 0 through the scratch
+      // tiles.  In FP64 wavefront 0, which also combines the sums and applies
+      // the diagonal inverse, takes only J = I + 1 (whose X_J it wrote itself at
+      // the previous step) and the others share J >= I + 2 (FP64 4x2 setup 3.8%
+      // faster); FP32 deals J round-robin over all wavefronts (the split form
+      // spills there).  Each wavefront's first kPF tiles of column I-1 (and
+      // wavefront 0's L_{I-1,I-1}^{-1}) are loaded while step I runs, so the
+      // global loads leave the step's critical path.
+      constexpr bool kSplit = sizeof(Real) == 8;
+      constexpr int kPF = kSplit ? 6 : 4;
+      constexpr int kStride = kSplit ? NW - 1 : NW;
+      auto jof = [&](int I, int g) {
+        if constexpr (kSplit) return wave == 0 ? (g == 0 ? I + 1 : KT) : I + 1 + wave + g * (NW - 1);
+        else return I + 1 + wave + g * NW;
+      };
+      Real nx[kPF][4], nli[4];
+      auto load_col = [&](int I) {
 #pragma unroll
-      for (int g = 0; g < kPF; ++g) {
-        const int J = jof(I, g);
-        if (J < KT) {
+        for (int g = 0; g < kPF; ++g) {
+          const int J = jof(I, g);
+          if (J < KT) {
+            const Real* tji = Wb + (long long)ltile(J, I) * kTE;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) nx[g][q] = tji[tile_off<Real, LM>(4 * q + lq, lr)];
+          }
+        }
+        if (wave == 0) {  // (L_II^{-1})^T as the A operand, K index row(q, lq)
+          const Real* li = Wd + (long long)I * kTE;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) nli[q] = li[MF<Real>::row(q, lq) * 16 + lr];
+        }
+      };
+      load_col(KT - 1);
+      for (int I = KT - 1; I >= 0; --I) {
+        Real av[kPF][4], al[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          al[q] = nli[q];
+#pragma unroll
+          for (int g = 0; g < kPF; ++g) av[g][q] = nx[g][q];
+        }
+        if (I > 0) load_col(I - 1);
+        v4 acc = {0, 0, 0, 0};
+#pragma unroll
+        for (int g = 0; g < kPF; ++g) {
+          const int J = jof(I, g);
+          if (J < KT) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc = MF<Real>::mma(av[g][q], X[(16 * J + 4 * q + lq) * kLS + lr], acc);
+          }
+        }
+        // (blocks beyond the prefetched tiles)
+        for (int J = (kSplit && wave == 0) ? KT : I + 1 + wave + kPF * kStride; J < KT; J += kStride) {
           const Real* tji = Wb + (long long)ltile(J, I) * kTE;
+          Real a[4];
 #pragma unroll
-          for (int q = 0; q < 4; ++q) nx[g][q] = tji[tile_off<Real, LM>(4 * q + lq, lr)];
+          for (int q = 0; q < 4; ++q) a[q] = tji[tile_off<Real, LM>(4 * q + lq, lr)];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) acc = MF<Real>::mma(a[q], X[(16 * J + 4 * q + lq) * kLS + lr], acc);
         }
-      }
-      if (wave == 0) {  // (L_II^{-1})^T as the A operand, K index row(q, lq)
-        const Real* li = Wd + (long long)I * kTE;
+        Real* Ss = S + (kSB == 2 ? (I & 1) * NW : 0) * kTE;  // this step's set
 #pragma unroll
-        for (int q = 0; q < 4; ++q) nli[q] = li[MF<Real>::row(q, lq) * 16 + lr];
-      }
-    };
-    load_col(KT - 1);
-    for (int I = KT - 1; I >= 0; --I) {
-      Real av[kPF][4], al[4];
+        for (int r = 0; r < 4; ++r) Ss[wave * kTE + r * 64 + lane] = acc[r];
+        __syncthreads();
+        if (wave == 0) {
+          Real rv[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        al[q] = nli[q];
+          for (int r = 0; r < 4; ++r) {
+            Real sum = Ss[r * 64 + lane];
+            for (int w = 1; w < NW; ++w) sum += Ss[w * kTE + r * 64 + lane];
+            Real yv = (I >= I0) ? X[(16 * I + MF<Real>::row(r, lq)) * kLS + lr] : Real(0);
+            rv[r] = yv - sum;
+          }
+          v4 x = {0, 0, 0, 0};
 #pragma unroll
-        for (int g = 0; g < kPF; ++g) av[g][q] = nx[g][q];
-      }
-      if (I > 0) load_col(I - 1);
-      v4 acc = {0, 0, 0, 0};
+          for (int q = 0; q < 4; ++q) x = MF<Real>::mma(al[q], rv[q], x);
 #pragma unroll
-      for (int g = 0; g < kPF; ++g) {
-        const int J = jof(I, g);
-        if (J < KT) {
-#pragma unroll
-          for (int q = 0; q < 4; ++q) acc = MF<Real>::mma(av[g][q], X[(16 * J + 4 * q + lq) * kLS + lr], acc);
+          for (int r = 0; r < 4; ++r) X[(16 * I + MF<Real>::row(r, lq)) * kLS + lr] = x[r];
         }
+        // X_I is read next by wavefront 0 itself (column I of step I-1) and by
+        // the others only after step I-1's barrier
+        if constexpr (kSB == 1) __syncthreads();
       }
-      // (blocks beyond the prefetched tiles)
-      for (int J = (kSplit && wave == 0) ? KT : I + 1 + wave + kPF * kStride; J < KT; J += kStride) {
-        const Real* tji = Wb + (long long)ltile(J, I) * kTE;
-        Real a[4];
+      if constexpr (kSB == 2) __syncthreads();
+    } else {
+      // FP32: two tile rows per step from the top, (H, H-1), and row 0 alone
+      // last when KT is odd; one barrier per step, half the dependent steps
+      // of one row at a time.  Both columns' sums over J > H are split over
+      // the wavefronts (J = H + 1 + wave + g NW; tiles (J, H-1) and (J, H)
+      // are adjacent in the workspace and share the X_J operand) and written
+      // as partial tiles; after the barrier wavefront 1 forms
+      // X_H = L_HH^{-T} (Y_H - S_H), and wavefront 0 forms X_H the same way
+      // and then X_{H-1} = L_{H-1,H-1}^{-T} (Y_{H-1} - S_{H-1} - L_{H,H-1}^T X_H).
+      // At the next step wavefront 0 reads X_{H-1} first and wavefront 1
+      // X_H, each its own; the others read X tiles written before this
+      // step's barrier.  Each wavefront's first kPF rows of the next step
+      // are loaded during the current step; the diagonal inverses and the
+      // coupling tile L_{H,H-1} at the start of their own step.
+      constexpr int kPF = 4;
+      Real nx[kPF][2][4];
+      auto load_step = [&](int H) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) a[q] = tji[tile_off<Real, LM>(4 * q + lq, lr)];
+        for (int g = 0; g < kPF; ++g) {
+          const int J = H + 1 + wave + g * NW;
+          if (J < KT) {
+            const Real* th = Wb + (long long)ltile(J, H) * kTE;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) acc = MF<Real>::mma(a[q], X[(16 * J + 4 * q + lq) * kLS + lr], acc);
-      }
-      Real* Ss = S + (kSB == 2 ? (I & 1) * NW : 0) * kTE;  // this step's set
+            for (int q = 0; q < 4; ++q) {
+              const int o = tile_off<Real, LM>(4 * q + lq, lr);
+              nx[g][0][q] = th[o];
+              nx[g][1][q] = H > 0 ? th[o - kTE] : Real(0);  // tile (J, H-1)
+            }
+          }
+        }
+      };
+      load_step(KT - 1);
+      int u = 0;
+      for (int H = KT - 1; H >= 0; H -= 2, ++u) {
+        const bool two = H > 0;
+        Real av[kPF][2][4];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) Ss[wave * kTE + r * 64 + lane] = acc[r];
-      __syncthreads();
-      if (wave == 0) {
-        Real rv[4];
+        for (int q = 0; q < 4; ++q) {
+#pragma unroll
+          for (int g = 0; g < kPF; ++g) {
+            av[g][0][q] = nx[g][0][q];
+            av[g][1][q] = nx[g][1][q];
+          }
+        }
+        if (H >= 2) load_step(H - 2);
+        // this step's diagonal inverses and coupling tile, needed after the
+        // products and the barrier
+        Real ah[4], al[4], ac[4];
+        if (wave <= 1) {
+          const Real* li = Wd + (long long)H * kTE;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) ah[q] = li[MF<Real>::row(q, lq) * 16 + lr];
+        }
+        if (wave == 0 && two) {
+          const Real* li = Wd + (long long)(H - 1) * kTE;
+          const Real* tc = Wb + (long long)ltile(H, H - 1) * kTE;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            al[q] = li[MF<Real>::row(q, lq) * 16 + lr];
+            ac[q] = tc[tile_off<Real, LM>(MF<Real>::row(q, lq), lr)];
+          }
+        }
+        v4 sh = {0, 0, 0, 0}, sl = {0, 0, 0, 0};
+#pragma unroll
+        for (int g = 0; g < kPF; ++g) {
+          const int J = H + 1 + wave + g * NW;
+          if (J < KT) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const Real xb = X[(16 * J + 4 * q + lq) * kLS + lr];
+              sh = MF<Real>::mma(av[g][0][q], xb, sh);
+              if (two) sl = MF<Real>::mma(av[g][1][q], xb, sl);
+            }
+          }
+        }
+        // (rows beyond the prefetched ones)
+        for (int J = H + 1 + wave + kPF * NW; J < KT; J += NW) {
+          const Real* th = Wb + (long long)ltile(J, H) * kTE;
+          Real a[4], b[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int o = tile_off<Real, LM>(4 * q + lq, lr);
+            a[q] = th[o];
+            b[q] = two ? th[o - kTE] : Real(0);
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const Real xb = X[(16 * J + 4 * q + lq) * kLS + lr];
+            sh = MF<Real>::mma(a[q], xb, sh);
+            if (two) sl = MF<Real>::mma(b[q], xb, sl);
+          }
+        }
+        Real* Ss = S + (size_t)(u & 1) * (2 * NW) * kTE;  // this step's set: (H, H-1) tiles per wavefront
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          Real sum = Ss[r * 64 + lane];
-          for (int w = 1; w < NW; ++w) sum += Ss[w * kTE + r * 64 + lane];
-          Real yv = (I >= I0) ? X[(16 * I + MF<Real>::row(r, lq)) * kLS + lr] : Real(0);
-          rv[r] = yv - sum;
+          Ss[(2 * wave) * kTE + r * 64 + lane] = sh[r];
+          if (two) Ss[(2 * wave + 1) * kTE + r * 64 + lane] = sl[r];
         }
-        v4 x = {0, 0, 0, 0};
+        __syncthreads();
+        Real yh[4], yl[4];
+        if (wave <= 1) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) x = MF<Real>::mma(al[q], rv[q], x);
+          for (int r = 0; r < 4; ++r) {
+            yh[r] = H >= I0 ? X[(16 * H + MF<Real>::row(r, lq)) * kLS + lr] : Real(0);
+            yl[r] = (two && H - 1 >= I0) ? X[(16 * (H - 1) + MF<Real>::row(r, lq)) * kLS + lr] : Real(0);
+          }
+        }
+        // (in the steps over the core's tile rows wavefront 1 overwrites Y_H
+        // with X_H only after wavefront 0 has read Y_H)
+        if (H >= I0) __syncthreads();
+        if (wave <= 1) {
+          Real rv[4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) X[(16 * I + MF<Real>::row(r, lq)) * kLS + lr] = x[r];
+          for (int r = 0; r < 4; ++r) {
+            Real sum = Ss[r * 64 + lane];
+            for (int w = 1; w < NW; ++w) sum += Ss[(2 * w) * kTE + r * 64 + lane];
+            rv[r] = yh[r] - sum;
+          }
+          v4 xh = {0, 0, 0, 0};
+#pragma unroll
+          for (int q = 0; q < 4; ++q) xh = MF<Real>::mma(ah[q], rv[q], xh);
+          if (wave == (two ? 1 : 0)) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) X[(16 * H + MF<Real>::row(r, lq)) * kLS + lr] = xh[r];
+          }
+          if (wave == 0 && two) {
+            v4 t = {0, 0, 0, 0};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) t = MF<Real>::mma(ac[q], xh[q], t);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              Real sum = Ss[kTE + r * 64 + lane];
+              for (int w = 1; w < NW; ++w) sum += Ss[(2 * w + 1) * kTE + r * 64 + lane];
+              rv[r] = yl[r] - sum - t[r];
+            }
+            v4 xl = {0, 0, 0, 0};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) xl = MF<Real>::mma(al[q], rv[q], xl);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) X[(16 * (H - 1) + MF<Real>::row(r, lq)) * kLS + lr] = xl[r];
+          }
+        }
       }
-      // X_I is read next by wavefront 0 itself (column I of step I-1) and by
-      // the others only after step I-1's barrier
-      if constexpr (kSB == 1) __syncthreads();
+      __syncthreads();
     }
-    if constexpr (kSB == 2) __syncthreads();
     // ---- phase 3: M_c[16 tc + tt][b] = X[pos(b)][tt], rows past T zero, in
     // the apply's storage order (apply_offset), written contiguously
     const int tcn = min(16, T - 16 * tc), R = apply_chunk_rows(T, tc);
@@ -582,7 +738,7 @@ __device__ void factor_block(
   // and the flag of regularised pivots
   typedef Pan<Real> PT;
   size_t pan = (size_t)pan_tiles(KT, min(NPC, KT)) * PT::kT * sizeof(Real), crd = (size_t)3 * NP * sizeof(double),
-         red = ((size_t)NP * kLS + (size_t)sbk_sets<Real>() * NW * kTE) * sizeof(Real);
+         red = ((size_t)NP * kLS + (size_t)sbk_tiles<Real>(NW, false) * kTE) * sizeof(Real);
   size_t region = pan > crd ? pan : crd;
   region = region > red ? region : red;
   region = (region + 15) & ~(size_t)15;
@@ -760,7 +916,7 @@ __device__ void factor_block(
     break;
   }
   }
-  core_rows<Real, NW, false>(c, Wb, Wd, lds_raw, n, T, mptr, M);
+  core_rows<Real, NW, false, false>(c, Wb, Wd, lds_raw, n, T, mptr, M);
 }
 
 // ---- left-looking variant ------------------------------------------------
@@ -786,7 +942,7 @@ __host__ __device__ constexpr bool ll_rows_global(int npc) { return npc >= (size
 template <typename T>
 __host__ __device__ inline size_t ll_region(int KT, int nw, int npc) {
   size_t rows = ll_rows_global<T>(npc) ? 0 : (size_t)ll_row_tiles(KT, npc) * kTE * sizeof(T);
-  size_t red = ((size_t)16 * KT * kLS + (size_t)sbk_sets<T>() * nw * kTE) * sizeof(T);
+  size_t red = ((size_t)16 * KT * kLS + (size_t)sbk_tiles<T>(nw, sbk_pair<T>(true, npc)) * kTE) * sizeof(T);
   size_t region = rows > red ? rows : red;
   return (region + 31) & ~(size_t)31;
 }
@@ -1188,7 +1344,7 @@ __device__ void factor_block_ll(
     break;
   }
   }
-  core_rows<Real, NW, true>(c, Wb, Wd, lds_raw, n, T, mptr, M);
+  core_rows<Real, NW, true, sbk_pair<Real>(true, NPC)>(c, Wb, Wd, lds_raw, n, T, mptr, M);
 }
 
 // Persistent batched launch: each workgroup owns one workspace slot and takes
