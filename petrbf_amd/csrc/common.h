@@ -109,30 +109,29 @@ __device__ inline double exp_nonpos(double x) {
   return ldexp(p, (int)kf);
 }
 
-// 2^(y/64) for -2^20 < y <= 0 in FP64, for an argument already in units of
-// ln2/64 (the matvec scales its coordinates so that y = -|dx|^2 (64/ln2) /
+// 2^(y/256) for -2^20 < y <= 0 in FP64, for an argument already in units of
+// ln2/256 (the matvec scales its coordinates so that y = -|dx|^2 (256/ln2) /
 // (2 sigma^2) comes straight out of the squared distance): k = rint(y) by the
 // 1.5 * 2^52 shift (adding it leaves k in the sum's low word, |k| < 2^31, in
 // place of a round and a conversion), r = y - k EXACTLY (|r| <= 1/2,
-// Sterbenz), 2^(r/64) by its degree-5 Taylor polynomial in r (coefficients
-// (ln2/64)^n / n!, correctly rounded from 60-digit decimal arithmetic;
-// truncation < 4e-17 relative),
-// times tab[k & 63] = 2^((k & 63)/64) and 2^(k >> 6).  With no reduction
-// constant there is no reduction error: the result's error is the
-// polynomial's and the two products' roundings (10 FP64 operations; a
-// table-driven exp of x = -r^2/(2 sigma^2) needs 11, two of them for a
-// two-part ln2/64 reduction).
-__device__ inline double exp2_64_nonpos(double y, const double* tab) {
+// Sterbenz), 2^(r/256) by its degree-4 Taylor polynomial in r (coefficients
+// (ln2/256)^n / n!, correctly rounded from 60-digit decimal arithmetic;
+// truncation < 4e-17 relative), times tab[k & 255] = 2^((k & 255)/256) and
+// 2^(k >> 8).  With no reduction constant there is no reduction error: the
+// result's error is the polynomial's and the two products' roundings (9 FP64
+// operations; a 64-entry table needs a degree-5 polynomial for the same
+// truncation, a table-driven exp of x = -r^2/(2 sigma^2) 11, two of them for
+// a two-part ln2/64 reduction).
+__device__ inline double exp2_256_nonpos(double y, const double* tab) {
   constexpr double kShift = 6755399441055744.0;  // 1.5 * 2^52
   const double kd = y + kShift;
   const int k = __double2loint(kd);
   const double r = y - (kd - kShift);
-  double p = fma(r, 0x1.5d87fe78a6731p-40, 0x1.3b2ab6fba4e77p-31);
-  p = fma(p, r, 0x1.c6b08d704a0c0p-23);
-  p = fma(p, r, 0x1.ebfbdff82c58fp-15);
-  p = fma(p, r, 0x1.62e42fefa39efp-7);
+  double p = fma(r, 0x1.3b2ab6fba4e77p-39, 0x1.c6b08d704a0c0p-29);
+  p = fma(p, r, 0x1.ebfbdff82c58fp-19);
+  p = fma(p, r, 0x1.62e42fefa39efp-9);
   p = fma(p, r, 1.0);
-  return ldexp(p * tab[k & 63], k >> 6);
+  return ldexp(p * tab[k & 255], k >> 8);
 }
 
 }  // namespace prbf
