@@ -9,7 +9,10 @@
 * the dense block export (mysubmat, 2d/matmult.cxx:53-104) against the
   oracle's block and buffer list;
 * the mixed preconditioner's precision rule (blocks under 32 unknowns in
-  FP64) and its FP32 blocks against an independent FP32 factorisation;
+  FP64) and its FP32 blocks against an independent FP32 factorisation, for
+  both forms of the FP32 backward substitution (two tile rows per step or
+  one), odd and even tile-row counts and two or three right-hand-side
+  chunks;
 * the N = 4*10^6 lattice workload: lists, matvec and the true residual of
   the plain and mixed gamma, all against the oracle.
 """
@@ -191,6 +194,44 @@ def test_fp32_blocks_match_an_independent_fp32_factorisation():
     z32 = ctx.rasm_apply(r)
     _fp32_bound_check(ctx, p, r, z32, z64, "lattice 40x40")
     ctx.close()
+
+
+@pytest.mark.parametrize("cpb,buf", [(5, 5), (6, 5)])
+def test_fp32_backward_substitution_forms(cpb, buf):
+    """Cells of 5x5 and 6x6 spacings with a 5-spacing buffer: interior blocks
+    of 225 and 256 unknowns (15 and 16 tile rows, so the two-row backward
+    substitution of the FP32 2-column kernels ends on a single row or on a
+    pair; edge blocks give other counts) and 25 or 36 core particles (two
+    and three 16-column chunks of right-hand sides).  Both FP32 forms, two
+    rows per step (left-looking, 2-column panels) and one row per step
+    (left-looking 1 and 4 columns, right-looking), are as close to the exact
+    FP64 preconditioner as LAPACK's FP32 factorisation of the same blocks,
+    and the FP64 one matches the oracle's LU."""
+    _announce(f"fp32 substitution forms cells {cpb} buffer {buf}")
+    p = problems.lattice_2d_problem(40, seed=0x5EED0300 + cpb, cells_per_box=cpb, buffer_cells=buf)
+    o = Oracle.from_problem(p)
+    o.rasm_setup()
+    ista, iend, _, _ = o.cells()
+    bptr, _ = o.buffer_csr()
+    kt = set(((np.diff(bptr) + 15) // 16).tolist())
+    assert {15, 16} & kt and int((iend - ista + 1).max()) > 16, (kt, int((iend - ista + 1).max()))
+    assert any(k % 2 for k in kt) and any(k % 2 == 0 for k in kt), kt
+    r = np.random.default_rng(41).standard_normal(p.n)
+    want = o.rasm_apply(r)
+    for order, nw, npc in (("left", 4, 2), ("left", 8, 2), ("left", 4, 1), ("left", 4, 4), ("right", 4, 2)):
+        ctx = Context.from_problem(p)
+        ctx.set_tuning(chol_waves=nw, chol_panel=npc, chol_order=order)
+        ctx.set_points(p.x, p.y)
+        ctx.rasm_setup("rasm")
+        z64 = ctx.rasm_apply(r)
+        assert np.linalg.norm(z64 - want) <= 1e-8 * np.linalg.norm(want), (order, nw, npc)
+        ctx.rasm_setup("rasm32")
+        info = ctx.info()
+        assert (info["chol_order"], info["chol_waves"], info["chol_panel"]) == (
+            1 if order == "right" else 2, nw, npc)
+        z32 = ctx.rasm_apply(r)
+        _fp32_bound_check(ctx, p, r, z32, z64, f"{order} {nw}x{npc}")
+        ctx.close()
 
 
 def test_mixed_precision_rule_small_blocks_in_fp64():
