@@ -46,40 +46,49 @@ __device__ inline void slice(long long n, long long* a, long long* b) {
   *b = min(n, *a + len);
 }
 
-// part[blk * kMaxK + j] = sum over the slice of w * V_j, for j < k
+// part[blk * kMaxK + j] = sum over the slice of w * V_j, for j < k; 16
+// vectors per pass over the slice (w read once for up to 16), so that a
+// thread has 17 loads in flight per element
+constexpr int kDotChunk = 16;
 __global__ __launch_bounds__(kRT) void k_mdot(long long n, const double* __restrict__ w,
                                               const double* __restrict__ V, long long ldv, int k,
                                               double* __restrict__ part) {
-  __shared__ double sh[8 * (kRT / 64)];
+  __shared__ double sh[kDotChunk * (kRT / 64)];
   long long a, b;
   slice(n, &a, &b);
-  for (int j0 = 0; j0 < k; j0 += 8) {
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    int kk = min(8, k - j0);
-    for (long long i = a + threadIdx.x; i < b; i += kRT) {
-      double wi = w[i];
+  for (int j0 = 0; j0 < k; j0 += kDotChunk) {
+    double acc[kDotChunk];
 #pragma unroll
-      for (int j = 0; j < 8; ++j)
+    for (int j = 0; j < kDotChunk; ++j) acc[j] = 0.0;
+    const int kk = min(kDotChunk, k - j0);
+    for (long long i = a + threadIdx.x; i < b; i += kRT) {
+      const double wi = w[i];
+#pragma unroll
+      for (int j = 0; j < kDotChunk; ++j)
         if (j < kk) acc[j] = fma(wi, V[(long long)(j0 + j) * ldv + i], acc[j]);
     }
-    block_sums<8>(acc, sh);
+    block_sums<kDotChunk>(acc, sh);
     if (threadIdx.x == 0)
       for (int j = 0; j < kk; ++j) part[(long long)blockIdx.x * kMaxK + j0 + j] = acc[j];
   }
 }
 
-__global__ void k_reduce_parts(int k, const double* __restrict__ part, double* __restrict__ out) {
-  int j = threadIdx.x;
-  if (j >= k) return;
+// out[j] = sum of the kRedBlocks slice partials of column j: one wavefront
+// per column, lane l summing slices l, l + 64, ... in order, the lane sums
+// then combined by a butterfly (a fixed order: identical run to run)
+__global__ __launch_bounds__(64) void k_reduce_parts(const double* __restrict__ part, double* __restrict__ out) {
+  const int j = blockIdx.x, lane = threadIdx.x;
   double s = 0.0;
-  for (int b = 0; b < kRedBlocks; ++b) s += part[(long long)b * kMaxK + j];
-  out[j] = s;
+  for (int b = lane; b < kRedBlocks; b += 64) s += part[(long long)b * kMaxK + j];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (lane == 0) out[j] = s;
 }
 
 void mdot(hipStream_t st, long long n, const double* w, const double* V, long long ldv, int k, double* part,
           double* out) {
   hipLaunchKernelGGL(k_mdot, dim3(kRedBlocks), dim3(kRT), 0, st, n, w, V, ldv, k, part);
-  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(kMaxK), 0, st, k, part, out);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(k), dim3(64), 0, st, part, out);
 }
 
 // w += sum_j alpha*coef[j] V_j, then part = partial sums of w^2
@@ -108,7 +117,7 @@ __global__ __launch_bounds__(kRT) void k_maxpy_norm2(long long n, double* __rest
 void maxpy_norm2(hipStream_t st, long long n, double* w, const double* V, long long ldv, int k,
                  const double* coef, double alpha, double* part, double* out_norm2) {
   hipLaunchKernelGGL(k_maxpy_norm2, dim3(kRedBlocks), dim3(kRT), 0, st, n, w, V, ldv, k, coef, alpha, part);
-  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(kMaxK), 0, st, 1, part, out_norm2);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(64), 0, st, part, out_norm2);
 }
 
 __global__ __launch_bounds__(kRT) void k_norm2sq(long long n, const double* __restrict__ v,
@@ -125,7 +134,7 @@ __global__ __launch_bounds__(kRT) void k_norm2sq(long long n, const double* __re
 
 void norm2sq(hipStream_t st, long long n, const double* v, double* part, double* out) {
   hipLaunchKernelGGL(k_norm2sq, dim3(kRedBlocks), dim3(kRT), 0, st, n, v, part);
-  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(kMaxK), 0, st, 1, part, out);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(64), 0, st, part, out);
 }
 
 __global__ void k_scale(long long n, double* __restrict__ v, double s) {
