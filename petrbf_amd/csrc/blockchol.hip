@@ -397,7 +397,7 @@ __host__ __device__ inline size_t blk_lds_bytes(int nmax, int tmax, int nw, int 
 // is complete: the core rows of A'^{-1} into M.  LM: tiles stored lane-major
 // (lm_off) instead of register-major (MF::off).
 template <typename Real, int NW, bool LM, bool PAIR>
-__device__ __forceinline__ void core_rows(long long c, const Real* __restrict__ Wb, const Real* __restrict__ Wd,
+__device__ __forceinline__ void core_rows_body(long long c, const Real* __restrict__ Wb, const Real* __restrict__ Wd,
                           unsigned char* __restrict__ lds_raw, int n, int T, const long long* __restrict__ mptr,
                           Real* __restrict__ M) {
   typedef typename MF<Real>::v4 v4;
@@ -706,6 +706,27 @@ __device__ __forceinline__ void core_rows(long long c, const Real* __restrict__ 
     }
     __syncthreads();
   }
+}
+
+// Phases 2 and 3 as the factorisations call them: inlined in FP32 (a call
+// spills there), a call of its own in FP64, where inlining it raises the
+// factorisation's spills at the 256-VGPR cap from 17 to 33 (4x2 setup
+// 250.8 against 253.0 ms)
+template <typename Real, int NW, bool LM, bool PAIR>
+__device__ __attribute__((noinline)) void core_rows_call(long long c, const Real* __restrict__ Wb,
+                                                         const Real* __restrict__ Wd,
+                                                         unsigned char* __restrict__ lds_raw, int n, int T,
+                                                         const long long* __restrict__ mptr, Real* __restrict__ M) {
+  core_rows_body<Real, NW, LM, PAIR>(c, Wb, Wd, lds_raw, n, T, mptr, M);
+}
+template <typename Real, int NW, bool LM, bool PAIR>
+__device__ __forceinline__ void core_rows(long long c, const Real* __restrict__ Wb, const Real* __restrict__ Wd,
+                                          unsigned char* __restrict__ lds_raw, int n, int T,
+                                          const long long* __restrict__ mptr, Real* __restrict__ M) {
+  if constexpr (sizeof(Real) == 4)
+    core_rows_body<Real, NW, LM, PAIR>(c, Wb, Wd, lds_raw, n, T, mptr, M);
+  else
+    core_rows_call<Real, NW, LM, PAIR>(c, Wb, Wd, lds_raw, n, T, mptr, M);
 }
 
 // One block: cluster c, workspace Wb (blk_ws elements), LDS as laid out below.
