@@ -34,7 +34,7 @@ CPU_VALIDATION_FILE = os.path.join(ROOT, "profiles", "cpu_baseline_validation.tx
 
 def cpu_validation():
     """The host-core baseline's estimator checked against full oracle solves
-    (the committed summary of `bench.py --cpu-validate` on the GPU box's host),
+    (the committed summary of `bench.py --cpu-validate` on the benchmark machine's host cores),
     or None when the summary is absent."""
     try:
         for line in open(CPU_VALIDATION_FILE):
