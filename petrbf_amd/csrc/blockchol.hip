@@ -477,7 +477,7 @@ __device__ __forceinline__ void core_rows_body(long long c, const Real* __restri
         if constexpr (kSplit) return wave == 0 ? (g == 0 ? I + 1 : KT) : I + 1 + wave + g * (NW - 1);
         else return I + 1 + wave + g * NW;
       };
-      Real nx[kPF][4], nli[4];
+      Real nx[kPF][4] = {}, nli[4] = {};
       auto load_col = [&](int I) {
 #pragma unroll
         for (int g = 0; g < kPF; ++g) {
@@ -561,7 +561,7 @@ __device__ __forceinline__ void core_rows_body(long long c, const Real* __restri
       // are loaded during the current step; the diagonal inverses and the
       // coupling tile L_{H,H-1} at the start of their own step.
       constexpr int kPF = 4;
-      Real nx[kPF][2][4];
+      Real nx[kPF][2][4] = {};
       auto load_step = [&](int H) {
 #pragma unroll
         for (int g = 0; g < kPF; ++g) {
