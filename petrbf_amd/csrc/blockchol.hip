@@ -999,7 +999,7 @@ __device__ void factor_block_ll(
     int* __restrict__ blk_status) {
   typedef typename MF<Real>::v4 v4;
   constexpr bool PG = ll_rows_global<Real>(NPC);  // panel rows from the workspace, not LDS
-  constexpr int G = NPC > 2 ? 2 : (sizeof(Real) == 4 ? 3 : 4);  // tile rows per wavefront pass
+  constexpr int G = sizeof(Real) == 4 ? 2 : (NPC > 2 ? 2 : 4);  // tile rows per wavefront pass
   const long long b0 = bptr[c];
   const int n = (int)(bptr[c + 1] - b0);
   if (n == 0) return;
