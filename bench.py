@@ -523,7 +523,9 @@ def bench_c3(args, torch, problems, Context, dev):
                      "t_solve_s": inf.t_solve_s, "residual_estimate": inf.rnorm / inf.rnorm0,
                      "true_rel_residual": true_rel, "regularised_blocks": inf.n_regularised,
                      "chol_kernel_s": ci["chol_kernel_s"],
-                     "chol_tflops": ci["chol_flops"] / ci["chol_kernel_s"] / 1e12 if ci["chol_kernel_s"] else None}
+                     "chol_tflops": ci["chol_flops"] / ci["chol_kernel_s"] / 1e12 if ci["chol_kernel_s"] else None,
+                     "chol_config": f"{ci['chol_waves']}x{ci['chol_panel']} "
+                                    f"{'left' if ci['chol_order'] == 2 else 'right'}"}
         log(f"{mode}: step {t_step:.3f}s (setup {inf.t_setup_s:.3f}s, solve {inf.t_solve_s:.3f}s), its {inf.its} "
             f"reason {inf.reason}, residual estimate {inf.rnorm/inf.rnorm0:.3e}, true {true_rel:.3e}, "
             f"regularised blocks {inf.n_regularised}")
